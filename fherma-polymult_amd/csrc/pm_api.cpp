@@ -1,0 +1,502 @@
+// The library's boundary: the extern "C" pm_* API (include/fherma_polymult.h)
+// and the FHERMA C++ entry points fherma_init / fherma_run / fherma_free
+// (include/fherma.h), over one plan per point.
+//
+// A plan owns: the host plan (primes, constants), the device copies of its
+// tables, a K x N residue workspace per operand, N x L device limb buffers and
+// pinned host staging for host-pointer calls, and one HIP stream at the default
+// priority. Twiddle tables depend only on (N, primes), so plans that share
+// them (every product of a batch at one N) share one device copy.
+#include "../../include/fherma.h"
+#include "../../include/fherma_polymult.h"
+#include "pm_arith.h"
+#include "pm_device.h"
+#include "pm_plan.h"
+#include "pm_mfma_host.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return PM_OK;
+    } catch (const HipError& e) {
+        g_err = e.what();
+        return PM_EHIP;
+    } catch (const std::bad_alloc& e) {
+        g_err = "out of memory";
+        return PM_ENOMEM;
+    } catch (const std::invalid_argument& e) {
+        g_err = e.what();
+        return PM_EINVAL;
+    } catch (const std::runtime_error& e) {
+        g_err = e.what();
+        return PM_EINVAL;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return PM_EINTERNAL;
+    }
+}
+
+void* dev_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (bytes == 0) bytes = 4;
+    check(hipMalloc(&p, bytes), "hipMalloc");
+    return p;
+}
+
+// Device twiddle tables for (N, prime list), shared between plans.
+struct Twiddles {
+    uint2* tw = nullptr;
+    uint2* itw = nullptr;
+    ~Twiddles() {
+        if (tw) (void)hipFree(tw);
+        if (itw) (void)hipFree(itw);
+    }
+};
+
+std::mutex g_tw_mutex;
+std::map<std::vector<uint32_t>, std::weak_ptr<Twiddles>> g_tw_cache;
+
+std::shared_ptr<Twiddles> twiddles_for(const pm::HostPlan& pl) {
+    std::vector<uint32_t> key(pl.p);
+    key.push_back(pl.N);
+    std::lock_guard<std::mutex> lock(g_tw_mutex);
+    auto it = g_tw_cache.find(key);
+    if (it != g_tw_cache.end()) {
+        if (auto sp = it->second.lock()) return sp;
+    }
+    auto t = std::make_shared<Twiddles>();
+    const size_t words = 2 * static_cast<size_t>(pl.K) * pl.N;
+    std::vector<uint32_t> host(words);
+    t->tw = static_cast<uint2*>(dev_alloc(words * sizeof(uint32_t)));
+    t->itw = static_cast<uint2*>(dev_alloc(words * sizeof(uint32_t)));
+    pm::build_twiddles(pl, true, 0, pl.K, host.data());
+    check(hipMemcpy(t->tw, host.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice), "upload twiddles");
+    pm::build_twiddles(pl, false, 0, pl.K, host.data());
+    check(hipMemcpy(t->itw, host.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice), "upload twiddles");
+    g_tw_cache[key] = t;
+    return t;
+}
+
+}  // namespace
+
+struct pm_plan {
+    pm::HostPlan host;
+    pm::DevPlan dev{};
+    int form = PM_FORM_PLAIN;
+    std::shared_ptr<Twiddles> tw;
+    std::vector<void*> owned;            // device allocations
+    uint32_t* A = nullptr;               // K x N residues of a, then y
+    uint32_t* B = nullptr;               // K x N residues of b
+    uint32_t* d_limbs = nullptr;         // 3 x N x L: a, b, c for host-pointer calls
+    uint32_t* h_stage = nullptr;         // pinned host staging, 3 x N x L
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[PM_NPHASES + 1] = {};
+    bool mfma_ready = false;
+
+    ~pm_plan() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+        for (void* p : owned) (void)hipFree(p);
+        if (h_stage) (void)hipHostFree(h_stage);
+    }
+
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        T* d = static_cast<T*>(dev_alloc(v.size() * sizeof(T)));
+        owned.push_back(d);
+        if (!v.empty()) check(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+        return d;
+    }
+
+    size_t words() const { return static_cast<size_t>(host.N) * host.L; }
+    size_t residues() const { return static_cast<size_t>(host.K) * host.N; }
+
+    void ensure_mfma() {
+        if (mfma_ready) return;
+        pm::prepare_mfma(host, dev, [this](const void* src, size_t bytes) {
+            void* d = dev_alloc(bytes);
+            owned.push_back(d);
+            check(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice), "upload mfma tables");
+            return d;
+        });
+        mfma_ready = true;
+    }
+};
+
+namespace {
+
+hipStream_t stream_of(pm_plan* pl, void* s) { return s ? static_cast<hipStream_t>(s) : pl->stream; }
+
+void need(bool ok, const char* msg) {
+    if (!ok) throw std::invalid_argument(msg);
+}
+
+pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q) {
+    need(q != nullptr, "q is null");
+    std::unique_ptr<pm_plan> pl(new pm_plan());
+    pl->host = pm::make_plan(N, W, q, static_cast<int>(L));
+    const pm::HostPlan& h = pl->host;
+    const int K = h.K;
+
+    std::vector<pm::PrimeConst> pc(K);
+    for (int j = 0; j < K; ++j) {
+        const uint32_t p = h.p[j];
+        pc[j].p = p;
+        pc[j].pneg = h.pneg[j];
+        pc[j].c32 = static_cast<uint32_t>((1ull << 32) % p);
+        pc[j].c32s = pm::shoup_companion(pc[j].c32, p);
+        pc[j].c64 = static_cast<uint32_t>(static_cast<uint64_t>(pc[j].c32) * pc[j].c32 % p);
+        pc[j].one_s = pm::shoup_companion(1, p);
+        pc[j].scale = h.scale[j];
+        pc[j].scale_s = h.scale_s[j];
+    }
+    pl->tw = twiddles_for(h);
+
+    pm::DevPlan& d = pl->dev;
+    d.N = h.N;
+    d.logN = h.logN;
+    d.logS = h.logS;
+    d.logR = h.logR;
+    d.L = h.L;
+    d.K = K;
+    d.q_drop = h.q_drop;
+    d.q_top = h.q_top;
+    d.pc = pl->upload(pc);
+    d.pow2 = pl->upload(h.pow2);
+    d.tw = pl->tw->tw;
+    d.itw = pl->tw->itw;
+    d.p_recip = pl->upload(h.p_recip);
+    d.M = pl->upload(h.M);
+    d.negvP = pl->upload(h.negvP);
+    d.q = pl->upload(h.q);
+
+    pl->A = static_cast<uint32_t*>(dev_alloc(pl->residues() * sizeof(uint32_t)));
+    pl->owned.push_back(pl->A);
+    pl->B = static_cast<uint32_t*>(dev_alloc(pl->residues() * sizeof(uint32_t)));
+    pl->owned.push_back(pl->B);
+    check(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto& e : pl->ev) check(hipEventCreate(&e), "hipEventCreate");
+    pl->ensure_mfma();
+    pl->form = PM_FORM_MFMA;
+    return pl.release();
+}
+
+// The five phases of one product on device pointers, enqueued on st. When ev
+// is given, ev[i] is recorded before phase i and ev[PM_NPHASES] after the last.
+void run_device(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st,
+                hipEvent_t* ev = nullptr) {
+    const pm::DevPlan& P = pl->dev;
+    const bool mfma = pl->form == PM_FORM_MFMA;
+    auto mark = [&](int i) {
+        if (ev) check(hipEventRecord(ev[i], st), "hipEventRecord");
+    };
+    mark(0);
+    if (mfma) pm::launch_decompose_mfma(P, a, pl->A, st); else pm::launch_decompose_valu(P, a, pl->A, st);
+    mark(1);
+    if (mfma) pm::launch_decompose_mfma(P, b, pl->B, st); else pm::launch_decompose_valu(P, b, pl->B, st);
+    mark(2);
+    pm::launch_ntt_cols(P, pl->A, true, st);
+    mark(3);
+    pm::launch_ntt_cols(P, pl->B, true, st);
+    mark(4);
+    pm::launch_ntt_rows_fused(P, pl->A, pl->B, st);
+    mark(5);
+    pm::launch_ntt_cols(P, pl->A, false, st);
+    mark(6);
+    if (mfma) pm::launch_crt_mfma(P, pl->A, c, st); else pm::launch_crt_valu(P, pl->A, c, st);
+    mark(7);
+    check(hipGetLastError(), "kernel launch");
+}
+
+const char* const kPhaseNames[PM_NPHASES] = {"decompose_a", "decompose_b", "ntt_cols_fwd_a", "ntt_cols_fwd_b",
+                                             "ntt_rows_fused", "ntt_cols_inv", "crt"};
+
+}  // namespace
+
+extern "C" {
+
+const char* pm_last_error(void) { return g_err.c_str(); }
+
+int pm_plan_create(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, pm_plan** out) {
+    return guarded([&] {
+        need(out != nullptr, "out is null");
+        *out = nullptr;
+        *out = make_plan_object(N, W, L, q);
+    });
+}
+
+void pm_plan_destroy(pm_plan* plan) { delete plan; }
+
+int pm_plan_query(const pm_plan* plan, uint32_t* N, uint32_t* L, uint32_t* K) {
+    return guarded([&] {
+        need(plan != nullptr, "plan is null");
+        if (N) *N = plan->host.N;
+        if (L) *L = static_cast<uint32_t>(plan->host.L);
+        if (K) *K = static_cast<uint32_t>(plan->host.K);
+    });
+}
+
+int pm_plan_primes(const pm_plan* plan, uint32_t* out, uint32_t cap) {
+    return guarded([&] {
+        need(plan != nullptr && out != nullptr, "null argument");
+        need(cap >= static_cast<uint32_t>(plan->host.K), "cap is smaller than K");
+        std::memcpy(out, plan->host.p.data(), plan->host.p.size() * sizeof(uint32_t));
+    });
+}
+
+int pm_choose_primes(uint32_t N, uint32_t qbits, uint32_t* out, uint32_t cap, uint32_t* K) {
+    return guarded([&] {
+        need(out != nullptr && K != nullptr, "null argument");
+        need(N >= 2 && (N & (N - 1)) == 0 && N <= (1u << pm::kMaxLogN), "N must be a power of two in [2, 2^22]");
+        const std::vector<uint32_t> p = pm::choose_primes(N, static_cast<int>(qbits));
+        need(p.size() <= cap, "cap is smaller than K");
+        std::memcpy(out, p.data(), p.size() * sizeof(uint32_t));
+        *K = static_cast<uint32_t>(p.size());
+    });
+}
+
+int pm_plan_set_form(pm_plan* plan, int form) {
+    return guarded([&] {
+        need(plan != nullptr, "plan is null");
+        need(form == PM_FORM_PLAIN || form == PM_FORM_MFMA, "form must be PM_FORM_PLAIN or PM_FORM_MFMA");
+        if (form == PM_FORM_MFMA) plan->ensure_mfma();
+        plan->form = form;
+    });
+}
+
+int pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, void* stream) {
+    return guarded([&] {
+        need(plan && a && b && c, "null argument");
+        run_device(plan, a, b, c, stream_of(plan, stream));
+    });
+}
+
+int pm_multiply(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c) {
+    return guarded([&] {
+        need(plan && a && b && c, "null argument");
+        const size_t words = plan->words();
+        const size_t bytes = words * sizeof(uint32_t);
+        if (!plan->d_limbs) {
+            plan->d_limbs = static_cast<uint32_t*>(dev_alloc(3 * bytes));
+            plan->owned.push_back(plan->d_limbs);
+        }
+        if (!plan->h_stage) check(hipHostMalloc(reinterpret_cast<void**>(&plan->h_stage), 3 * bytes, 0), "hipHostMalloc");
+        uint32_t* da = plan->d_limbs;
+        uint32_t* db = da + words;
+        uint32_t* dc = db + words;
+        hipStream_t st = plan->stream;
+        std::memcpy(plan->h_stage, a, bytes);
+        check(hipMemcpyAsync(da, plan->h_stage, bytes, hipMemcpyHostToDevice, st), "copy a");
+        std::memcpy(plan->h_stage + words, b, bytes);
+        check(hipMemcpyAsync(db, plan->h_stage + words, bytes, hipMemcpyHostToDevice, st), "copy b");
+        run_device(plan, da, db, dc, st);
+        check(hipMemcpyAsync(plan->h_stage + 2 * words, dc, bytes, hipMemcpyDeviceToHost, st), "copy c");
+        check(hipStreamSynchronize(st), "synchronize");
+        std::memcpy(c, plan->h_stage + 2 * words, bytes);
+    });
+}
+
+int pm_batch_multiply_device(int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
+                             uint32_t* const* c, void* stream) {
+    return guarded([&] {
+        need(count >= 0 && (count == 0 || (plans && a && b && c)), "null argument");
+        for (int i = 0; i < count; ++i) {
+            need(plans[i] && a[i] && b[i] && c[i], "null entry in batch");
+            run_device(plans[i], a[i], b[i], c[i], stream_of(plans[i], stream));
+        }
+    });
+}
+
+int pm_stage_decompose(pm_plan* plan, const uint32_t* limbs, uint32_t* res, void* stream) {
+    return guarded([&] {
+        need(plan && limbs && res, "null argument");
+        if (plan->form == PM_FORM_MFMA)
+            pm::launch_decompose_mfma(plan->dev, limbs, res, stream_of(plan, stream));
+        else
+            pm::launch_decompose_valu(plan->dev, limbs, res, stream_of(plan, stream));
+        check(hipGetLastError(), "kernel launch");
+    });
+}
+
+int pm_stage_ntt(pm_plan* plan, uint32_t* res, int forward, void* stream) {
+    return guarded([&] {
+        need(plan && res, "null argument");
+        hipStream_t st = stream_of(plan, stream);
+        if (forward) {
+            pm::launch_ntt_cols(plan->dev, res, true, st);
+            pm::launch_ntt_rows(plan->dev, res, true, st);
+        } else {
+            pm::launch_ntt_rows(plan->dev, res, false, st);
+            pm::launch_ntt_cols(plan->dev, res, false, st);
+        }
+        check(hipGetLastError(), "kernel launch");
+    });
+}
+
+int pm_stage_pointwise(pm_plan* plan, uint32_t* A, const uint32_t* B, void* stream) {
+    return guarded([&] {
+        need(plan && A && B, "null argument");
+        pm::launch_pointwise(plan->dev, A, B, stream_of(plan, stream));
+        check(hipGetLastError(), "kernel launch");
+    });
+}
+
+int pm_stage_crt(pm_plan* plan, const uint32_t* res, uint32_t* c, void* stream) {
+    return guarded([&] {
+        need(plan && res && c, "null argument");
+        if (plan->form == PM_FORM_MFMA)
+            pm::launch_crt_mfma(plan->dev, res, c, stream_of(plan, stream));
+        else
+            pm::launch_crt_valu(plan->dev, res, c, stream_of(plan, stream));
+        check(hipGetLastError(), "kernel launch");
+    });
+}
+
+int pm_time_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, int warmup,
+                            int iters, double* ms_per_product, void* stream) {
+    return guarded([&] {
+        need(plan && a && b && c && ms_per_product && iters > 0 && warmup >= 0, "bad argument");
+        hipStream_t st = stream_of(plan, stream);
+        for (int i = 0; i < warmup; ++i) run_device(plan, a, b, c, st);
+        check(hipEventRecord(plan->ev[0], st), "hipEventRecord");
+        for (int i = 0; i < iters; ++i) run_device(plan, a, b, c, st);
+        check(hipEventRecord(plan->ev[1], st), "hipEventRecord");
+        check(hipEventSynchronize(plan->ev[1]), "hipEventSynchronize");
+        float ms = 0;
+        check(hipEventElapsedTime(&ms, plan->ev[0], plan->ev[1]), "hipEventElapsedTime");
+        *ms_per_product = static_cast<double>(ms) / iters;
+    });
+}
+
+int pm_profile_phases(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, int iters,
+                      double* ms_per_phase, void* stream) {
+    return guarded([&] {
+        need(plan && a && b && c && ms_per_phase && iters > 0, "bad argument");
+        hipStream_t st = stream_of(plan, stream);
+        for (int i = 0; i < PM_NPHASES; ++i) ms_per_phase[i] = 0;
+        for (int it = 0; it < iters; ++it) {
+            run_device(plan, a, b, c, st, plan->ev);
+            check(hipEventSynchronize(plan->ev[PM_NPHASES]), "hipEventSynchronize");
+            for (int i = 0; i < PM_NPHASES; ++i) {
+                float ms = 0;
+                check(hipEventElapsedTime(&ms, plan->ev[i], plan->ev[i + 1]), "hipEventElapsedTime");
+                ms_per_phase[i] += ms;
+            }
+        }
+        for (int i = 0; i < PM_NPHASES; ++i) ms_per_phase[i] /= iters;
+    });
+}
+
+const char* pm_phase_name(int phase) { return phase >= 0 && phase < PM_NPHASES ? kPhaseNames[phase] : ""; }
+
+int pm_device_alloc(size_t bytes, void** out) {
+    return guarded([&] {
+        need(out != nullptr, "out is null");
+        *out = dev_alloc(bytes);
+    });
+}
+
+int pm_device_free(void* ptr) {
+    return guarded([&] {
+        if (ptr) check(hipFree(ptr), "hipFree");
+    });
+}
+
+int pm_copy_to_device(void* dst, const void* src, size_t bytes) {
+    return guarded([&] { check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D"); });
+}
+
+int pm_copy_to_host(void* dst, const void* src, size_t bytes) {
+    return guarded([&] { check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H"); });
+}
+
+int pm_stream_create(void** out) {
+    return guarded([&] {
+        need(out != nullptr, "out is null");
+        hipStream_t s = nullptr;
+        check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        *out = s;
+    });
+}
+
+int pm_stream_destroy(void* stream) {
+    return guarded([&] {
+        if (stream) check(hipStreamDestroy(static_cast<hipStream_t>(stream)), "hipStreamDestroy");
+    });
+}
+
+int pm_stream_synchronize(void* stream) {
+    return guarded([&] { check(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize"); });
+}
+
+int pm_device_synchronize(void) {
+    return guarded([&] { check(hipDeviceSynchronize(), "hipDeviceSynchronize"); });
+}
+
+int pm_device_info(char* name, size_t len, int* compute_units) {
+    return guarded([&] {
+        int dev = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        hipDeviceProp_t prop;
+        check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+        if (name && len) {
+            std::strncpy(name, prop.gcnArchName, len - 1);
+            name[len - 1] = 0;
+        }
+        if (compute_units) *compute_units = prop.multiProcessorCount;
+    });
+}
+
+}  // extern "C"
+
+// --- FHERMA C++ boundary ------------------------------------------------------
+
+void* fherma_init(const fherma::Point& p) {
+    if (p.q.data.size() != p.L) throw std::runtime_error("q must have L limbs");
+    pm_plan* plan = nullptr;
+    if (pm_plan_create(p.N, p.W, p.L, p.q.data.data(), &plan) != PM_OK) throw std::runtime_error(g_err);
+    return plan;
+}
+
+fherma::Outputs fherma_run(void* state, const fherma::Inputs& in) {
+    auto* plan = static_cast<pm_plan*>(state);
+    if (!plan) throw std::runtime_error("fherma_run: null state");
+    const size_t words = plan->words();
+    if (in.a.data.size() != words || in.b.data.size() != words) {
+        throw std::runtime_error("operand size does not match the point");
+    }
+    fherma::Outputs out;
+    out.c.shape = {static_cast<int64_t>(plan->host.N), static_cast<int64_t>(plan->host.L)};
+    out.c.data.resize(words);
+    if (pm_multiply(plan, in.a.data.data(), in.b.data.data(), out.c.data.data()) != PM_OK) {
+        throw std::runtime_error(g_err);
+    }
+    return out;
+}
+
+void fherma_free(void* state) { pm_plan_destroy(static_cast<pm_plan*>(state)); }

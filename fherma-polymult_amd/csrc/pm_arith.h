@@ -1,0 +1,84 @@
+// Word arithmetic modulo a prime p < 2^31, shared by the host plan and the
+// gfx950 kernels.
+//
+// Every residue is kept canonical in [0, p): with p this close to 2^31 there is
+// no headroom in a 32-bit word for lazily reduced sums (2p + 2p > 2^32), so each
+// butterfly reduces its sum and difference with one unsigned min each.
+//
+// Reference counterparts: rns.cuh:46-93 (mulhi, add_mod, sub_mod, mul_shoup,
+// mont_mul, bitrev_bits).
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define PM_HD __host__ __device__ __forceinline__
+#else
+#define PM_HD inline
+#endif
+
+namespace pm {
+
+constexpr int kMaxL = 64;        // limbs per coefficient (W <= 2048), rns.cuh:37
+constexpr int kMaxK = 160;       // primes, rns.cuh:38
+constexpr int kMaxLogN = 22;     // rns.cuh:39
+constexpr int kMarginBits = 20;  // sum_j y_j / p_j lies within 2^-21 of an integer, rns.cuh:41
+
+PM_HD uint32_t mulhi32(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umulhi(a, b);
+#else
+    return static_cast<uint32_t>((static_cast<uint64_t>(a) * b) >> 32);
+#endif
+}
+
+// Reduce r in [0, 2p) to [0, p): if r < p then r - p wraps above r.
+PM_HD uint32_t csub(uint32_t r, uint32_t p) {
+    const uint32_t t = r - p;
+    return t < r ? t : r;
+}
+
+PM_HD uint32_t add_mod(uint32_t a, uint32_t b, uint32_t p) { return csub(a + b, p); }
+
+PM_HD uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t p) {
+    const uint32_t d = a - b;
+    const uint32_t e = d + p;
+    return e < d ? e : d;   // a < b: d wrapped high and d + p is the answer, below d
+}
+
+// a * w mod p, w < p constant with ws = floor(w 2^32 / p); any a < 2^32.
+PM_HD uint32_t mul_shoup(uint32_t a, uint32_t w, uint32_t ws, uint32_t p) {
+    return csub(a * w - mulhi32(a, ws) * p, p);
+}
+
+// The same, left in [0, 2p).
+PM_HD uint32_t mul_shoup_lazy(uint32_t a, uint32_t w, uint32_t ws, uint32_t p) {
+    return a * w - mulhi32(a, ws) * p;
+}
+
+// a * b * 2^-32 mod p for a, b < p; pneg = -p^-1 mod 2^32.
+PM_HD uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t p, uint32_t pneg) {
+    const uint32_t lo = a * b;
+    const uint32_t hi = mulhi32(a, b);
+    const uint32_t m = lo * pneg;
+    // (a b + m p) / 2^32: the low words cancel, with a carry out unless lo == 0.
+    const uint32_t u = hi + mulhi32(m, p) + (lo != 0u);
+    return csub(u, p);
+}
+
+PM_HD uint32_t shoup_companion(uint32_t w, uint32_t p) {
+    return static_cast<uint32_t>((static_cast<uint64_t>(w) << 32) / p);
+}
+
+PM_HD uint32_t bitrev32(uint32_t x, uint32_t bits) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __brev(x) >> (32 - bits);
+#else
+    uint32_t r = 0;
+    for (uint32_t i = 0; i < bits; ++i) r |= ((x >> i) & 1u) << (bits - 1 - i);
+    return r;
+#endif
+}
+
+}  // namespace pm

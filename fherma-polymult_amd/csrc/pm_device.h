@@ -1,0 +1,58 @@
+// Device-side view of a plan, and the kernel launchers (pm_device.hip,
+// pm_mfma.hip). Residues are K x N u32, prime-major; limbs are N x L u32,
+// coefficient-major with little-endian limbs (the spec's wire layout).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pm {
+
+// Per-prime constants, 32 bytes, read with scalar loads.
+struct PrimeConst {
+    uint32_t p;       // the prime, p < 2^31, p = 1 mod max(2N, 2^16)
+    uint32_t pneg;    // -p^-1 mod 2^32
+    uint32_t c32;     // 2^32 mod p
+    uint32_t c32s;    // its Shoup companion
+    uint32_t c64;     // 2^64 mod p
+    uint32_t one_s;   // Shoup companion of 1: floor(2^32 / p)
+    uint32_t scale;   // N^-1 (P/p_j)^-1 2^32 mod p
+    uint32_t scale_s; // its Shoup companion
+};
+
+struct DevPlan {
+    uint32_t N, logN, logS, logR;
+    int L, K, q_drop, pad;
+    double q_top;
+    const PrimeConst* pc;     // K
+    const uint32_t* pow2;     // K x L: 2^(32 l) mod p_j
+    const uint2* tw;          // K x N: (psi^bitrev(i), Shoup companion)
+    const uint2* itw;         // K x N: (psi^-bitrev(i), Shoup companion)
+    const double* p_recip;    // K
+    const uint32_t* M;        // K x L: P/p_j mod q
+    const uint32_t* negvP;    // (K + 1) x L: -vP mod q
+    const uint32_t* q;        // L
+    // int8 matrix-core forms (pm_mfma.hip); null until prepared.
+    const int8_t* dec_digits; // [ceil(K/4)][dec_ksteps][64 lanes][16]: MFMA A fragments
+    const uint32_t* dec_corr; // K: 128 sum_t (256^t mod p_j) mod p_j
+    const int8_t* crt_digits; // [crt_stiles][crt_ksteps][64 lanes][16]: MFMA A fragments
+    const uint32_t* crt_corr; // L + 2 limbs: 128 (2^32 - 1)/255 sum_j M_j, exact
+    int dec_ksteps;           // ceil(4L / 64)
+    int crt_ksteps;           // ceil(4K / 64)
+    int crt_stiles;           // ceil((4L + 4) / 16)
+    int pad2;
+};
+
+// Plain form (VALU): every launcher enqueues on `st` and returns.
+void launch_decompose_valu(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st);
+void launch_ntt_cols(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st);
+void launch_ntt_rows(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st);
+void launch_ntt_rows_fused(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st);
+void launch_pointwise(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st);
+void launch_crt_valu(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream_t st);
+
+// Matrix-core form (int8 MFMA, exact).
+void launch_decompose_mfma(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st);
+void launch_crt_mfma(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream_t st);
+
+}  // namespace pm

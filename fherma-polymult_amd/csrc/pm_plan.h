@@ -1,0 +1,56 @@
+// Host-side plan for one point (N, q): the prime set, K, and every constant the
+// kernels read. Built once per point, outside any timed region.
+//
+// The prime and K rule is the reference's (rns_plan.h:138-155): primes
+// p < 2^31 with p = 1 (mod max(2N, 2^16)), taken downward from the largest,
+// until sum log2 p exceeds 2 * bits(q) + log2 N + 1 + 20. Keeping the same rule
+// keeps the same K, so the product is a drop-in for the reference's plan.
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace pm {
+
+struct HostPlan {
+    uint32_t N = 0, logN = 0, W = 0;
+    int L = 0, K = 0, qbits = 0;
+    std::vector<uint32_t> q;          // L limbs
+
+    // q_top: the top three limbs of q as a double (relative to 2^(32 q_drop)),
+    // for the quotient estimate of the final reduction mod q.
+    double q_top = 0;
+    int q_drop = 0;
+
+    std::vector<uint32_t> p, pneg;    // K; pneg = -p^-1 mod 2^32
+    std::vector<uint32_t> psi, ipsi;  // K: primitive 2N-th roots of unity and inverses
+    std::vector<uint32_t> pow2, pow2_s;    // K x L: 2^(32 l) mod p_j, Shoup companions
+    // Pointwise constant N^-1 (P/p_j)^-1 2^32 mod p_j (the 2^32 undoes the
+    // Montgomery product), with its Shoup companion.
+    std::vector<uint32_t> scale, scale_s;
+    std::vector<double> p_recip;      // K: 1 / p_j
+    std::vector<uint32_t> M;          // K x L: P / p_j mod q
+    std::vector<uint32_t> negvP;      // (K + 1) x L: -v P mod q
+
+    // Geometry of the two-pass NTT: N = R x S, rows of S contiguous elements.
+    uint32_t logS = 0, logR = 0;
+};
+
+// Throws std::runtime_error with the reference's messages for invalid points
+// (rns_plan.h:120-155).
+HostPlan make_plan(uint32_t N, uint32_t W, const uint32_t* q, int L);
+
+// The prime walk alone: K primes for (N, bits(q)), or an exception.
+std::vector<uint32_t> choose_primes(uint32_t N, int qbits);
+
+// Twiddles psi^bitrev_N(i) (forward) or psi^-bitrev_N(i) (inverse), i < N,
+// as (w, floor(w 2^32 / p)) pairs, for primes [j0, j1) of the plan.
+void build_twiddles(const HostPlan& pl, bool forward, int j0, int j1, uint32_t* out_pairs);
+
+// Balanced base-256 digits: x = sum_i d_i 256^i with d_i in [-128, 127].
+// Used for the int8 matrix-core forms.
+void balanced_digits(const uint32_t* x, int limbs, int8_t* out, int ndigits);
+
+}  // namespace pm

@@ -1,0 +1,248 @@
+"""Python face of libfherma_polymult.so (ctypes over include/fherma_polymult.h).
+
+The product path is the HIP library: there is no CPU fallback. If the built
+library is missing, importing this module raises, naming the build command.
+
+    plan = Plan(N, W, q)               # fherma_init      (reference solve.cu:710)
+    c = plan.multiply(a, b)            # fherma_run       (reference solve.cu:871)
+    plan.close()                       # fherma_free      (reference solve.cu:1061)
+
+Operands are numpy uint32 arrays of shape (N, L): little-endian limbs,
+coefficient-major, as the spec's wire format (reference fherma.h:34-41).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfherma_polymult.so")
+
+PM_OK, PM_EINVAL, PM_EHIP, PM_ENOMEM, PM_EINTERNAL = 0, 1, 2, 3, 4
+FORM_PLAIN, FORM_MFMA = 0, 1
+NPHASES = 7
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+
+
+class PolymulError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__("%s (status %d)" % (msg, code))
+        self.code = code
+
+
+def _load() -> ctypes.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libfherma_polymult.so is not built: run `python fherma-polymult_amd/build.py` "
+                          "(or __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "pm_last_error": (ctypes.c_char_p, []),
+        "pm_plan_create": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p,
+                                          ctypes.POINTER(_vp)]),
+        "pm_plan_destroy": (None, [_vp]),
+        "pm_plan_query": (ctypes.c_int, [_vp, _u32p, _u32p, _u32p]),
+        "pm_plan_primes": (ctypes.c_int, [_vp, _u32p, ctypes.c_uint32]),
+        "pm_plan_set_form": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "pm_choose_primes": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, _u32p, ctypes.c_uint32, _u32p]),
+        "pm_multiply": (ctypes.c_int, [_vp, _u32p, _u32p, _u32p]),
+        "pm_multiply_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+        "pm_batch_multiply_device": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                    ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+        "pm_stage_decompose": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+        "pm_stage_ntt": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
+        "pm_stage_pointwise": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+        "pm_stage_crt": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+        "pm_time_multiply_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.POINTER(ctypes.c_double), _vp]),
+        "pm_profile_phases": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                             _vp]),
+        "pm_phase_name": (ctypes.c_char_p, [ctypes.c_int]),
+        "pm_device_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
+        "pm_device_free": (ctypes.c_int, [_vp]),
+        "pm_copy_to_device": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+        "pm_copy_to_host": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+        "pm_stream_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+        "pm_stream_destroy": (ctypes.c_int, [_vp]),
+        "pm_stream_synchronize": (ctypes.c_int, [_vp]),
+        "pm_device_synchronize": (ctypes.c_int, []),
+        "pm_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+lib = _load()
+
+
+def _check(rc: int) -> None:
+    if rc != PM_OK:
+        raise PolymulError(rc, lib.pm_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray):
+    if a.dtype != np.uint32 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("expected a C-contiguous uint32 array")
+    return a.ctypes.data_as(_u32p)
+
+
+def choose_primes(N: int, qbits: int) -> np.ndarray:
+    """The plan's prime rule on the host alone (no device needed)."""
+    buf = np.zeros(256, dtype=np.uint32)
+    K = ctypes.c_uint32(0)
+    _check(lib.pm_choose_primes(int(N), int(qbits), _ptr(buf), 256, ctypes.byref(K)))
+    return buf[:K.value].copy()
+
+
+# --------------------------------------------------------------------------
+# Device memory
+
+class DeviceBuffer:
+    """A device allocation of `nbytes`, freed on close()."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = _vp()
+        _check(lib.pm_device_alloc(self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, arr: np.ndarray) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        buf = cls(arr.nbytes)
+        buf.upload(arr)
+        return buf
+
+    def upload(self, arr: np.ndarray) -> None:
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        _check(lib.pm_copy_to_device(_vp(self.ptr), arr.ctypes.data_as(_vp), arr.nbytes))
+
+    def download(self, shape, dtype=np.uint32) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib.pm_copy_to_host(out.ctypes.data_as(_vp), _vp(self.ptr), out.nbytes))
+        return out
+
+    def close(self) -> None:
+        if self.ptr:
+            _check(lib.pm_device_free(_vp(self.ptr)))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synchronize() -> None:
+    _check(lib.pm_device_synchronize())
+
+
+def device_info():
+    name = ctypes.create_string_buffer(128)
+    cus = ctypes.c_int(0)
+    _check(lib.pm_device_info(name, 128, ctypes.byref(cus)))
+    return name.value.decode(), cus.value
+
+
+# --------------------------------------------------------------------------
+# Plans
+
+class Plan:
+    """One point (N, W, q): primes, tables and workspace on the device."""
+
+    def __init__(self, N: int, W: int, q: np.ndarray, form: int | None = None):
+        q = np.ascontiguousarray(q, dtype=np.uint32)
+        h = _vp()
+        _check(lib.pm_plan_create(int(N), int(W), int(q.shape[0]), _ptr(q), ctypes.byref(h)))
+        self._h = h.value
+        n, l, k = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib.pm_plan_query(_vp(self._h), ctypes.byref(n), ctypes.byref(l), ctypes.byref(k)))
+        self.N, self.L, self.K, self.W = n.value, l.value, k.value, int(W)
+        self.form = None
+        if form is not None:
+            self.set_form(form)
+
+    @property
+    def handle(self):
+        return _vp(self._h)
+
+    def set_form(self, form: int) -> None:
+        _check(lib.pm_plan_set_form(self.handle, int(form)))
+        self.form = form
+
+    @property
+    def primes(self) -> np.ndarray:
+        out = np.zeros(self.K, dtype=np.uint32)
+        _check(lib.pm_plan_primes(self.handle, _ptr(out), self.K))
+        return out
+
+    def _operand(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.uint32)
+        if x.shape != (self.N, self.L):
+            raise ValueError("operand must have shape (N, L) = (%d, %d)" % (self.N, self.L))
+        return x
+
+    def multiply(self, a: np.ndarray, b: np.ndarray) -> np.ndarray:
+        a, b = self._operand(a), self._operand(b)
+        c = np.empty((self.N, self.L), dtype=np.uint32)
+        _check(lib.pm_multiply(self.handle, _ptr(a), _ptr(b), _ptr(c)))
+        return c
+
+    def multiply_device(self, a: DeviceBuffer, b: DeviceBuffer, c: DeviceBuffer, stream=None) -> None:
+        _check(lib.pm_multiply_device(self.handle, _vp(a.ptr), _vp(b.ptr), _vp(c.ptr), _vp(stream)))
+
+    # the five steps, device to device
+    def stage_decompose(self, limbs: DeviceBuffer, res: DeviceBuffer, stream=None) -> None:
+        _check(lib.pm_stage_decompose(self.handle, _vp(limbs.ptr), _vp(res.ptr), _vp(stream)))
+
+    def stage_ntt(self, res: DeviceBuffer, forward: bool, stream=None) -> None:
+        _check(lib.pm_stage_ntt(self.handle, _vp(res.ptr), 1 if forward else 0, _vp(stream)))
+
+    def stage_pointwise(self, A: DeviceBuffer, B: DeviceBuffer, stream=None) -> None:
+        _check(lib.pm_stage_pointwise(self.handle, _vp(A.ptr), _vp(B.ptr), _vp(stream)))
+
+    def stage_crt(self, res: DeviceBuffer, c: DeviceBuffer, stream=None) -> None:
+        _check(lib.pm_stage_crt(self.handle, _vp(res.ptr), _vp(c.ptr), _vp(stream)))
+
+    def time_device(self, a: DeviceBuffer, b: DeviceBuffer, c: DeviceBuffer, warmup: int, iters: int,
+                    stream=None) -> float:
+        ms = ctypes.c_double(0)
+        _check(lib.pm_time_multiply_device(self.handle, _vp(a.ptr), _vp(b.ptr), _vp(c.ptr), int(warmup),
+                                           int(iters), ctypes.byref(ms), _vp(stream)))
+        return ms.value
+
+    def profile_phases(self, a: DeviceBuffer, b: DeviceBuffer, c: DeviceBuffer, iters: int, stream=None) -> dict:
+        out = (ctypes.c_double * NPHASES)()
+        _check(lib.pm_profile_phases(self.handle, _vp(a.ptr), _vp(b.ptr), _vp(c.ptr), int(iters), out,
+                                     _vp(stream)))
+        return {lib.pm_phase_name(i).decode(): out[i] for i in range(NPHASES)}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.pm_plan_destroy(_vp(self._h))
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def batch_multiply_device(plans, a_bufs, b_bufs, c_bufs, stream=None) -> None:
+    n = len(plans)
+    arr = lambda xs: (_vp * n)(*[_vp(x) for x in xs])  # noqa: E731
+    _check(lib.pm_batch_multiply_device(n, arr([p._h for p in plans]), arr([x.ptr for x in a_bufs]),
+                                        arr([x.ptr for x in b_bufs]), arr([x.ptr for x in c_bufs]), _vp(stream)))

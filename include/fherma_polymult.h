@@ -1,0 +1,121 @@
+/*
+ * fherma_polymult.h -- C ABI of the MI355X-native exact negacyclic product
+ *     c = a * b mod (X^N + 1, q)
+ * with N a power of two up to 2^22 and q of up to 2048 bits (L <= 64 u32 limbs).
+ *
+ * Plain pointers and sizes only. Operands are N x L little-endian u32 limbs,
+ * coefficient-major (the spec's wire layout, reference fherma.h:34-41); every
+ * coefficient of a and b must be below q. The output is canonical in [0, q).
+ *
+ * Every function returns PM_OK (0) or a non-zero pm_status; pm_last_error()
+ * then holds a message (per thread). A plan is not re-entrant: calls on one plan
+ * must not overlap (the reference's state is not either, solve.cu:871).
+ *
+ * The C++ entry points of the FHERMA boundary (include/fherma.h) are thin
+ * wrappers over pm_plan_create / pm_multiply / pm_plan_destroy.
+ */
+#ifndef FHERMA_POLYMULT_H
+#define FHERMA_POLYMULT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pm_plan pm_plan;
+
+enum pm_status {
+    PM_OK = 0,
+    PM_EINVAL = 1,    /* invalid point or argument (the reference throws here) */
+    PM_EHIP = 2,      /* a HIP runtime call failed */
+    PM_ENOMEM = 3,    /* device or host allocation failed */
+    PM_EINTERNAL = 4
+};
+
+enum pm_form {
+    PM_FORM_PLAIN = 0, /* VALU decompose and CRT */
+    PM_FORM_MFMA = 1   /* exact int8 matrix-core decompose and CRT (default) */
+};
+
+/* Last error message of the calling thread ("" if none). */
+const char* pm_last_error(void);
+
+/* --- plan: fherma_init (reference solve.cu:710-869, rns_plan.h:118-249) ----
+ * Builds the prime set with the reference's rule (primes p < 2^31,
+ * p = 1 mod max(2N, 2^16), downward, until sum log2 p > 2 bits(q) + log2 N + 21),
+ * every table, and the device workspace. Refuses N outside [2, 2^22] or not a
+ * power of two, L outside [1, 64], q whose top limb is zero, and points that
+ * need more primes than exist. W is recorded, not checked (as in the reference). */
+int pm_plan_create(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, pm_plan** out);
+
+/* fherma_free (reference solve.cu:1061-1083). Null is accepted. */
+void pm_plan_destroy(pm_plan* plan);
+
+/* N, L and K of a plan; any output pointer may be null. */
+int pm_plan_query(const pm_plan* plan, uint32_t* N, uint32_t* L, uint32_t* K);
+
+/* The plan's K primes, in the reference's order (rns_plan.h:144-153). */
+int pm_plan_primes(const pm_plan* plan, uint32_t* out, uint32_t cap);
+
+/* The prime rule alone, on the host (no device): the K primes for a point with
+ * N coefficients and a q of `qbits` bits. */
+int pm_choose_primes(uint32_t N, uint32_t qbits, uint32_t* out, uint32_t cap, uint32_t* K);
+
+/* Selects the plain or the matrix-core form for later calls on this plan. */
+int pm_plan_set_form(pm_plan* plan, int form);
+
+/* --- products -------------------------------------------------------------
+ * pm_multiply: host pointers, synchronous; the fherma_run equivalent
+ * (reference solve.cu:871-1059). Inputs cross PCIe through pinned staging.
+ * pm_multiply_device: device pointers, enqueued on `stream` (a hipStream_t;
+ * null means the plan's own stream), returns without waiting. */
+int pm_multiply(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c);
+int pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, void* stream);
+
+/* A batch of independent products, each with its own plan (N and q may differ),
+ * all device pointers, enqueued on `stream`. No counterpart in the reference:
+ * it is for callers with many products per call (uneven batches). */
+int pm_batch_multiply_device(int count, pm_plan* const* plans, const uint32_t* const* a,
+                             const uint32_t* const* b, uint32_t* const* c, void* stream);
+
+/* --- the five steps, one at a time (device pointers, enqueued on `stream`) --
+ * Residues are K x N u32, prime-major. The forward NTT leaves the merged
+ * Cooley-Tukey transform's bit-reversed order (rns_kernels.cuh:159-174);
+ * pointwise multiplies by N^-1 (P/p_j)^-1, so the inverse NTT ends in
+ * y_j = c * (P/p_j)^-1 mod p_j; crt maps y back to N x L limbs mod q. */
+int pm_stage_decompose(pm_plan* plan, const uint32_t* limbs, uint32_t* res, void* stream);
+int pm_stage_ntt(pm_plan* plan, uint32_t* res, int forward, void* stream);
+int pm_stage_pointwise(pm_plan* plan, uint32_t* A, const uint32_t* B, void* stream);
+int pm_stage_crt(pm_plan* plan, const uint32_t* res, uint32_t* c, void* stream);
+
+/* --- timing (HIP events on the stream the kernels run on) -------------------
+ * pm_time_multiply_device: mean milliseconds per product over `iters` calls
+ * after `warmup` untimed ones.
+ * pm_profile_phases: mean milliseconds per phase over `iters` products, for
+ * PM_NPHASES phases named by pm_phase_name(). */
+#define PM_NPHASES 7
+int pm_time_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, int warmup,
+                            int iters, double* ms_per_product, void* stream);
+int pm_profile_phases(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, int iters,
+                      double* ms_per_phase, void* stream);
+const char* pm_phase_name(int phase);
+
+/* --- device memory and streams, for callers without an allocator -----------*/
+int pm_device_alloc(size_t bytes, void** out);
+int pm_device_free(void* ptr);
+int pm_copy_to_device(void* dst, const void* src, size_t bytes);
+int pm_copy_to_host(void* dst, const void* src, size_t bytes);
+int pm_stream_create(void** out);
+int pm_stream_destroy(void* stream);
+int pm_stream_synchronize(void* stream);
+int pm_device_synchronize(void);
+/* Device name and compute-unit count of the current device. */
+int pm_device_info(char* name, size_t len, int* compute_units);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FHERMA_POLYMULT_H */

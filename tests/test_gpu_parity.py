@@ -1,0 +1,133 @@
+"""GPU parity: the HIP library against the oracle and the gmpy2 digests.
+
+Every product is compared bit for bit: with the committed sha256 of the exact
+product (tests/golden/golden.json, computed with GMP) and, step by step, with
+the oracle's C restatement of the reference's algorithm on the same inputs.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import pointgen
+
+polymult = pytest.importorskip("polymult")
+
+pytestmark = pytest.mark.gpu
+
+FORMS = [("plain", 0), ("mfma", 1)]
+
+
+def say(*args):
+    print(*args, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    name, cus = polymult.device_info()
+    say("device", name, "CUs", cus)
+    yield
+    polymult.synchronize()
+
+
+SMALL = ["matrix-n2-w3-random", "matrix-n2-w32-prime", "matrix-n2-w33-random", "matrix-n4-w30-prime",
+         "matrix-n4-w64-random", "matrix-n8-w97-prime", "matrix-n16-w127-random", "matrix-n32-w31-prime",
+         "matrix-n64-w2048-random", "matrix-n128-w1025-prime", "matrix-n256-w65-random",
+         "matrix-n512-w257-prime", "matrix-n1024-w868-random", "matrix-n2048-w129-prime"]
+CONFIGS = ["config-n4096-w256-s1", "config-n4096-w256-s2", "config-n32768-w868-s1", "config-n65536-w1747-s1",
+           "config-n65536-w2048-00", "matrix-n8192-w868-prime", "matrix-n16384-w868-random",
+           "matrix-n131072-w512-prime", "matrix-n4096-w2048-random"]
+PATTERNS = ["pattern-%s-n%d-w%d" % (p, n, w) for p in ("all_q_minus_1", "zero", "wrap", "identity", "alternating")
+            for n, w in ((4, 64), (64, 100), (256, 1747), (4096, 256), (32768, 868), (65536, 2048))]
+
+
+def run_case(golden, name, form):
+    case = golden[name]
+    q, a, b = pointgen.operands(case)
+    say("case", name, "N", case["N"], "L", case["L"], "form", form)
+    with polymult.Plan(case["N"], case["W"], q, form=form) as plan:
+        c = plan.multiply(a, b)
+    got = oracle.digest(c)
+    if got != case["c_sha"]:
+        ref = oracle.Plan(case["N"], q).multiply(a, b)
+        bad = np.nonzero(np.any(ref != c, axis=1))[0]
+        pytest.fail("%s: %d of %d coefficients differ, first %s" % (name, bad.size, case["N"], bad[:5]))
+
+
+@pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
+@pytest.mark.parametrize("name", SMALL + CONFIGS)
+def test_product_matches_gmp(golden, name, form):
+    run_case(golden, name, form)
+
+
+@pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
+@pytest.mark.parametrize("name", PATTERNS)
+def test_patterns_match_gmp(golden, name, form):
+    run_case(golden, name, form)
+
+
+@pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
+@pytest.mark.parametrize("N,W,seed", [(64, 100, 3), (4096, 256, 5), (8192, 868, 6), (32768, 868, 7),
+                                      (65536, 2048, 8)])
+def test_stages_match_oracle(N, W, seed, form):
+    """Each of the five steps, device to device, against the oracle's step."""
+    say("stages", N, W, seed, "form", form)
+    q, a, b = oracle.gen_point(seed, N, W)
+    ref = oracle.Plan(N, q)
+    with polymult.Plan(N, W, q, form=form) as plan:
+        assert plan.K == ref.K and np.array_equal(plan.primes, ref.primes)
+        K, L = plan.K, plan.L
+        da, db = polymult.DeviceBuffer.from_array(a), polymult.DeviceBuffer.from_array(b)
+        A = polymult.DeviceBuffer(4 * K * N)
+        B = polymult.DeviceBuffer(4 * K * N)
+        C = polymult.DeviceBuffer(4 * N * L)
+
+        plan.stage_decompose(da, A)
+        plan.stage_decompose(db, B)
+        rA, rB = ref.decompose(a), ref.decompose(b)
+        assert np.array_equal(A.download((K, N)), rA), "decompose(a)"
+        assert np.array_equal(B.download((K, N)), rB), "decompose(b)"
+
+        plan.stage_ntt(A, True)
+        plan.stage_ntt(B, True)
+        fA, fB = ref.ntt_forward(rA), ref.ntt_forward(rB)
+        assert np.array_equal(A.download((K, N)), fA), "forward NTT"
+        assert np.array_equal(B.download((K, N)), fB), "forward NTT"
+
+        plan.stage_pointwise(A, B)
+        pw = ref.pointwise(fA, fB)
+        assert np.array_equal(A.download((K, N)), pw), "pointwise"
+
+        plan.stage_ntt(A, False)
+        y = ref.ntt_inverse(pw)
+        assert np.array_equal(A.download((K, N)), y), "inverse NTT"
+
+        plan.stage_crt(A, C)
+        c_ref, _ = ref.crt(y)
+        assert np.array_equal(C.download((N, L)), c_ref), "crt"
+
+        plan.multiply_device(da, db, C)
+        polymult.synchronize()
+        assert np.array_equal(C.download((N, L)), c_ref), "fused device product"
+
+
+@pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
+def test_uneven_batch_matches_gmp(golden, form):
+    names = sorted(n for n in golden if n.startswith("config-uneven-"))[:16]
+    plans, bufs, cases = [], [], []
+    for name in names:
+        case = golden[name]
+        q, a, b = pointgen.operands(case)
+        plans.append(polymult.Plan(case["N"], case["W"], q, form=form))
+        bufs.append((polymult.DeviceBuffer.from_array(a), polymult.DeviceBuffer.from_array(b),
+                     polymult.DeviceBuffer(a.nbytes)))
+        cases.append(case)
+    say("batch", len(plans), "form", form)
+    polymult.batch_multiply_device(plans, [x[0] for x in bufs], [x[1] for x in bufs], [x[2] for x in bufs])
+    polymult.synchronize()
+    for case, (da, db, dc), plan in zip(cases, bufs, plans):
+        c = dc.download((case["N"], case["L"]))
+        assert oracle.digest(c) == case["c_sha"], case["name"]
+        plan.close()

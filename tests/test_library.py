@@ -1,0 +1,74 @@
+"""CPU checks of the product library: it loads, exports every entry point its
+headers declare (the C ABI and the FHERMA C++ symbols), and its host-side plan
+logic agrees with the oracle. No kernel is launched here."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+import polymult
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def exported_symbols():
+    out = subprocess.run(["nm", "-D", "--defined-only", polymult.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if line.strip()}
+
+
+def declared_c_functions():
+    text = open(os.path.join(ROOT, "include", "fherma_polymult.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(pm_[a-z0-9_]+)\s*\(", text))
+
+
+def test_every_declared_c_function_is_exported():
+    syms = exported_symbols()
+    declared = declared_c_functions()
+    assert len(declared) >= 25
+    missing = sorted(d for d in declared if d not in syms)
+    assert not missing, missing
+    for name in declared:
+        getattr(polymult.lib, name)   # ctypes resolves it
+
+
+def test_fherma_cpp_entry_points_are_exported():
+    # The reference's boundary has C++ linkage (fherma.h:46-48): these are the
+    # symbols a harness built from its main.cpp binds.
+    syms = exported_symbols()
+    for mangled in ("_Z11fherma_initRKN6fherma5PointE", "_Z10fherma_runPvRKN6fherma6InputsE", "_Z11fherma_freePv"):
+        assert mangled in syms, mangled
+
+
+@pytest.mark.parametrize("N,W", [(2, 3), (64, 2048), (4096, 256), (32768, 868), (65536, 1747), (65536, 2048),
+                                 (1 << 17, 512), (1 << 22, 32)])
+def test_host_prime_rule_matches_oracle(N, W):
+    assert np.array_equal(polymult.choose_primes(N, W), oracle.choose_primes(N, W))
+
+
+def test_invalid_points_are_refused_before_any_device_work():
+    q = np.array([5], dtype=np.uint32)
+    for N in (0, 1, 3, 1 << 23):
+        with pytest.raises(polymult.PolymulError) as e:
+            polymult.Plan(N, 3, q)
+        assert e.value.code == polymult.PM_EINVAL
+    with pytest.raises(polymult.PolymulError) as e:
+        polymult.Plan(64, 2049, np.ones(65, dtype=np.uint32))
+    assert e.value.code == polymult.PM_EINVAL
+    with pytest.raises(polymult.PolymulError) as e:
+        polymult.Plan(64, 64, np.array([7, 0], dtype=np.uint32))
+    assert e.value.code == polymult.PM_EINVAL
+    assert "fewer limbs" in str(e.value)
+
+
+def test_not_enough_primes_is_refused():
+    # N = 2^22 leaves primes = 1 mod 2^23 below 2^31: far fewer than 2048 bits need.
+    with pytest.raises(polymult.PolymulError) as e:
+        polymult.choose_primes(1 << 22, 2048)
+    assert e.value.code == polymult.PM_EINVAL
+    with pytest.raises(ValueError):
+        oracle.choose_primes(1 << 22, 2048)
