@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Throughput of the exact negacyclic product on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--form mfma|plain] [--no-cpu]
+
+Workload: a batch of 16 independent products c = a b mod (X^N + 1, q) with
+N = 65536 and a seeded random odd 2048-bit q per product (64 u32 limbs, K = 134
+primes each), a and b uniform in [0, q). One step = the whole batch, device
+resident (inputs already in HBM when the timed region starts). Prints one JSON
+line: products/s of the matrix-core form (value) beside the plain form, the
+roofline of the dominant kernel from HIP-event timings, and the CPU baseline
+(the reference's solve_cpu.cpp, or the oracle's port of it where that library
+is absent) on a bounded sample. Every product of the batch is checked against
+its GMP digest in tests/golden/golden.json.
+"""
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fherma-polymult_amd"))
+
+N, W, BATCH, SEED0 = 65536, 2048, 16, 5000
+HBM_PEAK_GBS = 8000.0
+G = 0x9E3779B97F4A7C15
+H = 0xD1B54A32D192ED03
+M64 = (1 << 64) - 1
+
+
+# --- seeded inputs (counter-based SplitMix64; the fixtures use the same stream)
+def _mix_int(z):
+    z &= M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def _words(key, idx):
+    with np.errstate(over="ignore"):
+        z = np.uint64(key) + (idx + np.uint64(1)) * np.uint64(G)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(32)).astype(np.uint32)
+
+
+def make_point(seed, n, w):
+    L = (w + 31) // 32
+    mask = np.uint32((1 << (w % 32)) - 1 if w % 32 else 0xFFFFFFFF)
+    q = _words(_mix_int(seed * G), np.arange(L, dtype=np.uint64))
+    q[L - 1] &= mask
+    q[L - 1] |= np.uint32(1 << ((w - 1) % 32))
+    q[0] |= np.uint32(1)
+
+    def operand(stream):
+        key = _mix_int(seed * G + stream * H)
+        out = np.zeros((n, L), dtype=np.uint32)
+        pending = np.arange(n, dtype=np.uint64)
+        lanes = np.arange(L, dtype=np.uint64)
+        for t in range(64):
+            if pending.size == 0:
+                break
+            cand = _words(key, ((pending * np.uint64(64) + np.uint64(t)) * np.uint64(L))[:, None] + lanes[None, :])
+            cand[:, L - 1] &= mask
+            lt = np.zeros(pending.size, dtype=bool)
+            eq = np.ones(pending.size, dtype=bool)
+            for l in range(L - 1, -1, -1):
+                lt |= eq & (cand[:, l] < q[l])
+                eq &= cand[:, l] == q[l]
+            out[pending[lt].astype(np.int64)] = cand[lt]
+            pending = pending[~lt]
+        return out
+
+    return q, operand(1), operand(2)
+
+
+def digest(x):
+    return hashlib.sha256(np.ascontiguousarray(x, dtype="<u4").tobytes()).hexdigest()
+
+
+def golden_digests():
+    path = os.path.join(ROOT, "tests", "golden", "golden.json")
+    with open(path) as f:
+        cases = json.load(f)["cases"]
+    return {c["seed"]: c["c_sha"] for c in cases if c["name"].startswith("config-n65536-w2048-")}
+
+
+# --- algorithmic bytes per launch of each phase (K x N residues, N x L limbs)
+def phase_bytes(K, L):
+    kn, nl = 4 * K * N, 4 * N * L
+    return {"decompose_a": nl + kn, "decompose_b": nl + kn, "ntt_cols_fwd_a": 2 * kn, "ntt_cols_fwd_b": 2 * kn,
+            "ntt_rows_fused": 3 * kn, "ntt_cols_inv": 2 * kn, "crt": kn + nl}
+
+
+def cpu_baseline(points, budget_s=12.0):
+    """The reference's CPU path (oracle/_ref) on the box's cores, else the oracle port."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # noqa: E402  (the checker's CPU port, used only for this leg)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    q, a, b = points[0]
+    if oracle.ref_available():
+        kind, solver = "reference", oracle.RefSolver(N, W, q)
+        run = solver.run
+    else:
+        kind, plan = "port", oracle.Plan(N, q)
+        run = plan.multiply
+    run(a, b)   # warm
+    done, t0 = 0, time.perf_counter()
+    while True:
+        run(a, b)
+        done += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or done >= 64:
+            break
+    return {"value": done / el, "unit": "polymuls/s", "cores": threads, "kind": kind,
+            "sample": "%d products at N=65536, W=2048 (K=134), %.1f s, OMP threads=%d" % (done, el, threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import polymult  # noqa: E402
+
+    def log(*a):
+        print(*a, file=sys.stderr, flush=True)
+
+    log("generating %d points" % BATCH)
+    points = [make_point(SEED0 + i, N, W) for i in range(BATCH)]
+    plans = [polymult.Plan(N, W, q, form=polymult.FORM_MFMA) for q, _, _ in points]
+    K, L = plans[0].K, plans[0].L
+    da = [polymult.DeviceBuffer.from_array(a) for _, a, _ in points]
+    db = [polymult.DeviceBuffer.from_array(b) for _, _, b in points]
+    dc = [polymult.DeviceBuffer(4 * N * L) for _ in points]
+
+    def step():
+        polymult.batch_multiply_device(plans, da, db, dc)
+
+    def timed(steps, warmup):
+        for _ in range(warmup):
+            step()
+        polymult.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        polymult.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    # correctness of the batch against the GMP digests
+    step()
+    polymult.synchronize()
+    gold = golden_digests()
+    mismatches = sum(digest(dc[i].download((N, L))) != gold.get(SEED0 + i) for i in range(BATCH))
+    log("mfma batch: %d mismatches vs GMP" % mismatches)
+
+    sec = timed(args.steps, args.warmup)
+    value = BATCH / sec
+    log("mfma: %.3f ms per step, %.1f products/s" % (sec * 1e3, value))
+
+    plain_value = None
+    if not args.no_plain:
+        for p in plans:
+            p.set_form(polymult.FORM_PLAIN)
+        step()
+        polymult.synchronize()
+        mismatches += sum(digest(dc[i].download((N, L))) != gold.get(SEED0 + i) for i in range(BATCH))
+        plain_sec = timed(max(3, args.steps // 4), 1)
+        plain_value = BATCH / plain_sec
+        log("plain: %.3f ms per step, %.1f products/s" % (plain_sec * 1e3, plain_value))
+        for p in plans:
+            p.set_form(polymult.FORM_MFMA)
+
+    # per-phase HIP-event timings on the plan's stream (mean over the batch)
+    phases = {}
+    for p, a, b, c in zip(plans, da, db, dc):
+        for k, v in p.profile_phases(a, b, c, 1).items():
+            phases[k] = phases.get(k, 0.0) + v / BATCH
+    log("phases (ms): " + ", ".join("%s %.4f" % kv for kv in phases.items()))
+    nbytes = phase_bytes(K, L)
+    dom = max(phases, key=phases.get)
+    achieved = nbytes[dom] / (phases[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "kernel_ms": round(phases[dom], 5), "algorithmic_bytes": nbytes[dom]}
+    prod_bytes = 12 * N * L + 56 * K * N
+    cpu = None if args.no_cpu else cpu_baseline(points[:1])
+    name, cus = polymult.device_info()
+    out = {
+        "metric": "polymuls/s at N=65536, 2048-bit q: batched+MFMA vs plain HIP; % HBM roofline",
+        "value": round(value, 2), "unit": "polymuls/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32", "data": "synthetic",
+        "config": {"workload": "batch of 16 products, N=65536, random odd 2048-bit q (L=64, K=134), "
+                               "device-resident", "form": "mfma", "batch": BATCH, "N": N, "W": W, "L": L, "K": K,
+                   "parallelism": "single GPU", "device": name, "cus": cus},
+        "plain_value": None if plain_value is None else round(plain_value, 2),
+        "mfma_over_plain": None if plain_value is None else round(value / plain_value, 3),
+        "mismatches": int(mismatches),
+        "effective_GBps_per_product_yardstick": round(prod_bytes * value / 1e9, 1),
+        "phases_ms": {k: round(v, 5) for k, v in phases.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

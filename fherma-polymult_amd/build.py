@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfherma_polymult.so")
+SOLUTION = os.path.join(HERE, "solution")   # point-directory driver (harness/solution_main.cpp)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -53,6 +54,15 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
     os.replace(tmp, LIB)
+    # The driver is host-only C++ against the FHERMA boundary: g++ with the same
+    # libstdc++ ABI as the library, found next to it at run time.
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(HERE, "..", "include"),
+           os.path.join(HERE, "harness", "solution_main.cpp"), "-o", SOLUTION + ".tmp",
+           "-L" + HERE, "-lfherma_polymult", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed: %s\n%s%s" % (" ".join(cmd), r.stdout, r.stderr))
+    os.replace(SOLUTION + ".tmp", SOLUTION)
     return LIB
 
 

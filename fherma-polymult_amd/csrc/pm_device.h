@@ -51,6 +51,13 @@ void launch_ntt_rows_fused(const DevPlan& P, uint32_t* A, const uint32_t* B, hip
 void launch_pointwise(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st);
 void launch_crt_valu(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream_t st);
 
+// The same NTT passes with one LDS barrier per stage, for any sub-transform
+// length up to 4096 (pm_device.hip); the launchers above use them when the
+// register kernels of pm_ntt.hip (lengths 16 .. 256) do not apply.
+void launch_ntt_cols_generic(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st);
+void launch_ntt_rows_generic(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st);
+void launch_ntt_rows_fused_generic(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st);
+
 // Matrix-core form (int8 MFMA, exact).
 void launch_decompose_mfma(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st);
 void launch_crt_mfma(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream_t st);

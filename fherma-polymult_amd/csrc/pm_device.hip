@@ -259,7 +259,7 @@ void launch_decompose_valu(const DevPlan& P, const uint32_t* limbs, uint32_t* re
     hipLaunchKernelGGL(k_decompose_valu, grid, dim3(kDecThreads), lds, st, limbs, res, P);
 }
 
-void launch_ntt_cols(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
+void launch_ntt_cols_generic(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     if (P.logR == 0) return;
     const TileGeom g = col_geom(P);
     const dim3 grid(1u << (P.logS - g.logC), P.K);
@@ -270,7 +270,7 @@ void launch_ntt_cols(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st
         hipLaunchKernelGGL(k_ntt_pass<false>, grid, dim3(kNttThreads), lds, st, X, P, g);
 }
 
-void launch_ntt_rows(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
+void launch_ntt_rows_generic(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     const TileGeom g = row_geom(P, 13);
     const dim3 grid(1u << (P.logR - g.logU), P.K);
     const size_t lds = sizeof(uint32_t) << (g.logC + g.logU);
@@ -280,7 +280,7 @@ void launch_ntt_rows(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st
         hipLaunchKernelGGL(k_ntt_pass<false>, grid, dim3(kNttThreads), lds, st, X, P, g);
 }
 
-void launch_ntt_rows_fused(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st) {
+void launch_ntt_rows_fused_generic(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st) {
     const TileGeom g = row_geom(P, 12);
     const dim3 grid(1u << (P.logR - g.logU), P.K);
     const size_t lds = 2 * (sizeof(uint32_t) << (g.logC + g.logU));

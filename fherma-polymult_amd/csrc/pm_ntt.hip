@@ -1,0 +1,342 @@
+// Register-resident NTT passes for sub-transform lengths M = 2^m, 4 <= m <= 8.
+//
+// Same transform as the generic kernels in pm_device.hip (the merged
+// Cooley-Tukey NTT, twiddle index (N + k) / (2t) for a butterfly of distance t
+// whose lower element is at k), scheduled for wave64: every thread owns 16
+// elements of one sub-transform (a column of R elements with stride S, or a row
+// of S contiguous ones) and runs several stages on them in registers, with a
+// single LDS exchange per pass:
+//   phase A  stages of distance 2^(m-1) .. 16: register i GA + k holds element
+//            u = t + (M/16) i + 16 k of the sub-transform (GA = M/16 per group)
+//   phase B  stages of distance 8 .. 1: register k holds element u = 16 t + k
+// (forward runs A then B, inverse B then A). In both phases a butterfly block's
+// twiddle index is (B >> (e + 1)) plus a block number known at compile time
+// (phase A) or from t (phase B), where 2^e is the distance and B = R for the
+// column pass, B = N + rho S for row rho.
+//
+// LDS holds element u of a sub-transform at u + u/16 (one pad word per 16), so
+// that the phase B reads (16 lanes with stride 16) hit distinct banks.
+#include "pm_device.h"
+
+#include "pm_arith.h"
+
+namespace pm {
+
+void launch_ntt_cols_generic(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st);
+void launch_ntt_rows_generic(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st);
+void launch_ntt_rows_fused_generic(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st);
+
+namespace {
+
+__device__ __forceinline__ void ct(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
+    const uint32_t v = mul_shoup(y, w.x, w.y, p);
+    y = sub_mod(x, v, p);
+    x = add_mod(x, v, p);
+}
+
+__device__ __forceinline__ void gs(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
+    const uint32_t u = x;
+    x = add_mod(u, y, p);
+    y = mul_shoup(sub_mod(u, y, p), w.x, w.y, p);
+}
+
+template <bool kFwd>
+__device__ __forceinline__ void bfly(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
+    if (kFwd)
+        ct(x, y, w, p);
+    else
+        gs(x, y, w, p);
+}
+
+// Phase A on one or two register sets (kTwo: x and y share the twiddles).
+template <int LOGM, bool kFwd, bool kTwo>
+__device__ __forceinline__ void phase_a(uint32_t (&x)[16], uint32_t (&y)[16], const uint2* __restrict__ tw,
+                                        uint32_t B, uint32_t p) {
+    constexpr int GA = 1 << (LOGM - 4), NG = 16 / GA;
+#pragma unroll
+    for (int s = 0; s < LOGM - 4; ++s) {
+        const int e = kFwd ? LOGM - 1 - s : 4 + s;
+        const int d = 1 << (e - 4);
+#pragma unroll
+        for (int blk = 0; blk < GA / (2 * d); ++blk) {
+            const uint2 w = tw[(B >> (e + 1)) + blk];
+#pragma unroll
+            for (int i = 0; i < NG; ++i) {
+#pragma unroll
+                for (int k = blk * 2 * d; k < blk * 2 * d + d; ++k) {
+                    bfly<kFwd>(x[i * GA + k], x[i * GA + k + d], w, p);
+                    if (kTwo) bfly<kFwd>(y[i * GA + k], y[i * GA + k + d], w, p);
+                }
+            }
+        }
+    }
+}
+
+template <bool kFwd, bool kTwo>
+__device__ __forceinline__ void phase_b(uint32_t (&x)[16], uint32_t (&y)[16], const uint2* __restrict__ tw,
+                                        uint32_t B, uint32_t t, uint32_t p) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int e = kFwd ? 3 - s : s;
+        const int d = 1 << e;
+#pragma unroll
+        for (int blk = 0; blk < 8 / d; ++blk) {
+            const uint2 w = tw[(B >> (e + 1)) + (t << (3 - e)) + blk];
+#pragma unroll
+            for (int k = blk * 2 * d; k < blk * 2 * d + d; ++k) {
+                bfly<kFwd>(x[k], x[k + d], w, p);
+                if (kTwo) bfly<kFwd>(y[k], y[k + d], w, p);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t padded(uint32_t u) { return u + (u >> 4); }
+
+// Column pass: C = 2^logC adjacent columns of one prime, R = M rows each.
+template <int LOGM, bool kFwd>
+__global__ void __launch_bounds__(512)
+k_ntt_col_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logC) {
+    extern __shared__ uint32_t lds[];
+    constexpr uint32_t M = 1u << LOGM, TPS = M / 16, GA = TPS, NG = 16 / GA;
+    const uint32_t C = 1u << logC;
+    const uint32_t c = threadIdx.x & (C - 1), t = threadIdx.x >> logC;
+    const int j = blockIdx.y;
+    const size_t S = size_t(1) << P.logS;
+    uint32_t* x = X + static_cast<size_t>(j) * P.N + (blockIdx.x << logC) + c;
+    const uint2* tw = (kFwd ? P.tw : P.itw) + static_cast<size_t>(j) * P.N;
+    const uint32_t p = P.pc[j].p;
+    uint32_t r[16];
+    if (kFwd) {
+#pragma unroll
+        for (uint32_t i = 0; i < NG; ++i)
+#pragma unroll
+            for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = x[(t + TPS * i + 16 * k) * S];
+        phase_a<LOGM, true, false>(r, r, tw, M, p);
+#pragma unroll
+        for (uint32_t i = 0; i < NG; ++i)
+#pragma unroll
+            for (uint32_t k = 0; k < GA; ++k) lds[padded(t + TPS * i + 16 * k) * C + c] = r[i * GA + k];
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) r[k] = lds[padded(16 * t + k) * C + c];
+        phase_b<true, false>(r, r, tw, M, t, p);
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) x[(16 * t + k) * S] = r[k];
+    } else {
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) r[k] = x[(16 * t + k) * S];
+        phase_b<false, false>(r, r, tw, M, t, p);
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) lds[padded(16 * t + k) * C + c] = r[k];
+        __syncthreads();
+#pragma unroll
+        for (uint32_t i = 0; i < NG; ++i)
+#pragma unroll
+            for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = lds[padded(t + TPS * i + 16 * k) * C + c];
+        phase_a<LOGM, false, false>(r, r, tw, M, p);
+#pragma unroll
+        for (uint32_t i = 0; i < NG; ++i)
+#pragma unroll
+            for (uint32_t k = 0; k < GA; ++k) x[(t + TPS * i + 16 * k) * S] = r[i * GA + k];
+    }
+}
+
+// Row pass alone (forward or inverse): G = 2^logG rows of M = S elements.
+template <int LOGM, bool kFwd>
+__global__ void __launch_bounds__(512)
+k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
+    extern __shared__ uint32_t lds[];
+    constexpr uint32_t M = 1u << LOGM, TPS = M / 16, GA = TPS, NG = 16 / GA, ROW = M + M / 16;
+    const uint32_t t = threadIdx.x & (TPS - 1), rl = threadIdx.x / TPS;
+    const uint32_t rho = (blockIdx.x << logG) + rl;
+    const int j = blockIdx.y;
+    uint32_t* x = X + static_cast<size_t>(j) * P.N + static_cast<size_t>(rho) * M;
+    const uint2* tw = (kFwd ? P.tw : P.itw) + static_cast<size_t>(j) * P.N;
+    const uint32_t p = P.pc[j].p;
+    const uint32_t B = P.N + rho * M;
+    uint32_t* l = lds + rl * ROW;
+    uint32_t r[16];
+    if (kFwd) {
+#pragma unroll
+        for (uint32_t i = 0; i < NG; ++i)
+#pragma unroll
+            for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = x[t + TPS * i + 16 * k];
+        phase_a<LOGM, true, false>(r, r, tw, B, p);
+#pragma unroll
+        for (uint32_t i = 0; i < NG; ++i)
+#pragma unroll
+            for (uint32_t k = 0; k < GA; ++k) l[padded(t + TPS * i + 16 * k)] = r[i * GA + k];
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) r[k] = l[padded(16 * t + k)];
+        phase_b<true, false>(r, r, tw, B, t, p);
+        // phase B layout: 16 consecutive words per thread
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) x[16 * t + k] = r[k];
+    } else {
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) r[k] = x[16 * t + k];
+        phase_b<false, false>(r, r, tw, B, t, p);
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) l[padded(16 * t + k)] = r[k];
+        __syncthreads();
+#pragma unroll
+        for (uint32_t i = 0; i < NG; ++i)
+#pragma unroll
+            for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = l[padded(t + TPS * i + 16 * k)];
+        phase_a<LOGM, false, false>(r, r, tw, B, p);
+#pragma unroll
+        for (uint32_t i = 0; i < NG; ++i)
+#pragma unroll
+            for (uint32_t k = 0; k < GA; ++k) x[t + TPS * i + 16 * k] = r[i * GA + k];
+    }
+}
+
+__device__ __forceinline__ uint32_t pointwise_mul(uint32_t a, uint32_t b, const PrimeConst& pc) {
+    return mul_shoup(mont_mul(a, b, pc.p, pc.pneg), pc.scale, pc.scale_s, pc.p);
+}
+
+// Row pass fused: forward rows of A and B (sharing twiddle loads), the pointwise
+// product with N^-1 (P/p_j)^-1 into A, inverse rows of A.
+template <int LOGM>
+__global__ void __launch_bounds__(512)
+k_ntt_rows_fused_reg(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, DevPlan P, uint32_t logG) {
+    extern __shared__ uint32_t lds[];
+    constexpr uint32_t M = 1u << LOGM, TPS = M / 16, GA = TPS, NG = 16 / GA, ROW = M + M / 16;
+    const uint32_t t = threadIdx.x & (TPS - 1), rl = threadIdx.x / TPS;
+    const uint32_t G = 1u << logG;
+    const uint32_t rho = (blockIdx.x << logG) + rl;
+    const int j = blockIdx.y;
+    const size_t row = static_cast<size_t>(j) * P.N + static_cast<size_t>(rho) * M;
+    const uint2* tw = P.tw + static_cast<size_t>(j) * P.N;
+    const uint2* itw = P.itw + static_cast<size_t>(j) * P.N;
+    const PrimeConst pc = P.pc[j];
+    const uint32_t B = P.N + rho * M;
+    uint32_t* la = lds + rl * ROW;
+    uint32_t* lb = lds + (G + rl) * ROW;
+    uint32_t a[16], b[16];
+#pragma unroll
+    for (uint32_t i = 0; i < NG; ++i)
+#pragma unroll
+        for (uint32_t k = 0; k < GA; ++k) {
+            a[i * GA + k] = A[row + t + TPS * i + 16 * k];
+            b[i * GA + k] = Bm[row + t + TPS * i + 16 * k];
+        }
+    phase_a<LOGM, true, true>(a, b, tw, B, pc.p);
+#pragma unroll
+    for (uint32_t i = 0; i < NG; ++i)
+#pragma unroll
+        for (uint32_t k = 0; k < GA; ++k) {
+            la[padded(t + TPS * i + 16 * k)] = a[i * GA + k];
+            lb[padded(t + TPS * i + 16 * k)] = b[i * GA + k];
+        }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) {
+        a[k] = la[padded(16 * t + k)];
+        b[k] = lb[padded(16 * t + k)];
+    }
+    phase_b<true, true>(a, b, tw, B, t, pc.p);
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) a[k] = pointwise_mul(a[k], b[k], pc);
+    phase_b<false, false>(a, a, itw, B, t, pc.p);
+    // Each thread rewrites only the LDS words it read above: no barrier needed first.
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) la[padded(16 * t + k)] = a[k];
+    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < NG; ++i)
+#pragma unroll
+        for (uint32_t k = 0; k < GA; ++k) a[i * GA + k] = la[padded(t + TPS * i + 16 * k)];
+    phase_a<LOGM, false, false>(a, a, itw, B, pc.p);
+#pragma unroll
+    for (uint32_t i = 0; i < NG; ++i)
+#pragma unroll
+        for (uint32_t k = 0; k < GA; ++k) A[row + t + TPS * i + 16 * k] = a[i * GA + k];
+}
+
+bool reg_ok(uint32_t logm) { return logm >= 4 && logm <= 8; }
+
+// Columns per column-pass workgroup: up to 512 threads, at most S columns.
+uint32_t col_logC(const DevPlan& P) {
+    const uint32_t want = 13 - P.logR;   // 2^(logC + logR - 4) <= 512 threads
+    return want < P.logS ? want : P.logS;
+}
+
+// Rows per row-pass workgroup: up to 256 threads, at most R rows.
+uint32_t row_logG(const DevPlan& P) {
+    const uint32_t want = 12 - P.logS;
+    return want < P.logR ? want : P.logR;
+}
+
+template <int LOGM>
+void col_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
+    const uint32_t logC = col_logC(P);
+    const dim3 grid(1u << (P.logS - logC), P.K);
+    const dim3 block(1u << (logC + LOGM - 4));
+    const size_t lds = sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logC;
+    if (forward)
+        hipLaunchKernelGGL((k_ntt_col_reg<LOGM, true>), grid, block, lds, st, X, P, logC);
+    else
+        hipLaunchKernelGGL((k_ntt_col_reg<LOGM, false>), grid, block, lds, st, X, P, logC);
+}
+
+template <int LOGM>
+void row_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
+    const uint32_t logG = row_logG(P);
+    const dim3 grid(1u << (P.logR - logG), P.K);
+    const dim3 block(1u << (logG + LOGM - 4));
+    const size_t lds = sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG;
+    if (forward)
+        hipLaunchKernelGGL((k_ntt_row_reg<LOGM, true>), grid, block, lds, st, X, P, logG);
+    else
+        hipLaunchKernelGGL((k_ntt_row_reg<LOGM, false>), grid, block, lds, st, X, P, logG);
+}
+
+template <int LOGM>
+void fused_launch(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st) {
+    const uint32_t logG = row_logG(P);
+    const dim3 grid(1u << (P.logR - logG), P.K);
+    const dim3 block(1u << (logG + LOGM - 4));
+    const size_t lds = 2 * (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG);
+    hipLaunchKernelGGL((k_ntt_rows_fused_reg<LOGM>), grid, block, lds, st, A, B, P, logG);
+}
+
+}  // namespace
+
+void launch_ntt_cols(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
+    if (P.logR == 0) return;
+    switch (P.logR) {
+        case 4: col_launch<4>(P, X, forward, st); return;
+        case 5: col_launch<5>(P, X, forward, st); return;
+        case 6: col_launch<6>(P, X, forward, st); return;
+        case 7: col_launch<7>(P, X, forward, st); return;
+        case 8: col_launch<8>(P, X, forward, st); return;
+        default: launch_ntt_cols_generic(P, X, forward, st);
+    }
+}
+
+void launch_ntt_rows(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
+    switch (reg_ok(P.logS) ? P.logS : 0) {
+        case 4: row_launch<4>(P, X, forward, st); return;
+        case 5: row_launch<5>(P, X, forward, st); return;
+        case 6: row_launch<6>(P, X, forward, st); return;
+        case 7: row_launch<7>(P, X, forward, st); return;
+        case 8: row_launch<8>(P, X, forward, st); return;
+        default: launch_ntt_rows_generic(P, X, forward, st);
+    }
+}
+
+void launch_ntt_rows_fused(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st) {
+    switch (reg_ok(P.logS) ? P.logS : 0) {
+        case 4: fused_launch<4>(P, A, B, st); return;
+        case 5: fused_launch<5>(P, A, B, st); return;
+        case 6: fused_launch<6>(P, A, B, st); return;
+        case 7: fused_launch<7>(P, A, B, st); return;
+        case 8: fused_launch<8>(P, A, B, st); return;
+        default: launch_ntt_rows_fused_generic(P, A, B, st);
+    }
+}
+
+}  // namespace pm

@@ -158,8 +158,9 @@ HostPlan make_plan(uint32_t N, uint32_t W, const uint32_t* q, int L) {
     pl.p = choose_primes(N, pl.qbits);
     const int K = pl.K = static_cast<int>(pl.p.size());
 
-    // Two-pass geometry: rows of S = 2^ceil(logN / 2) (one pass when N <= 4096).
-    pl.logS = pl.logN <= 12 ? pl.logN : (pl.logN + 1) / 2;
+    // Two-pass geometry: rows of S = 2^ceil(logN / 2), columns of R = N / S
+    // (one pass when N <= 128).
+    pl.logS = pl.logN <= 7 ? pl.logN : (pl.logN + 1) / 2;
     pl.logR = pl.logN - pl.logS;
 
     pl.pneg.resize(K);
