@@ -233,10 +233,24 @@ void run_device(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, 
     check(hipGetLastError(), "kernel launch");
 }
 
+// A captured batch: every product's kernel chain, forked over up to kBranches
+// streams (independent products run concurrently), replayed as one hipGraph.
+constexpr int kBranches = 4;
+
 const char* const kPhaseNames[PM_NPHASES] = {"decompose_a", "decompose_b", "ntt_cols_fwd_a", "ntt_cols_fwd_b",
                                              "ntt_rows_fused", "ntt_cols_inv", "crt"};
 
 }  // namespace
+
+struct pm_batch {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<pm_plan*> plans;
+    ~pm_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
 
 extern "C" {
 
@@ -331,6 +345,69 @@ int pm_batch_multiply_device(int count, pm_plan* const* plans, const uint32_t* c
         }
     });
 }
+
+int pm_batch_create(int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
+                    uint32_t* const* c, pm_batch** out) {
+    return guarded([&] {
+        need(out != nullptr && count > 0 && plans && a && b && c, "bad argument");
+        *out = nullptr;
+        std::unique_ptr<pm_batch> bt(new pm_batch());
+        for (int i = 0; i < count; ++i) {
+            need(plans[i] && a[i] && b[i] && c[i], "null entry in batch");
+            for (int k = 0; k < i; ++k) need(plans[k] != plans[i], "each product of a batch needs its own plan");
+            bt->plans.push_back(plans[i]);
+        }
+        hipStream_t origin = nullptr;
+        check(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking), "hipStreamCreate");
+        std::vector<hipStream_t> branch(std::min(count, kBranches), nullptr);
+        std::vector<hipEvent_t> done(branch.size(), nullptr);
+        hipEvent_t fork = nullptr;
+        auto cleanup = [&] {
+            for (auto e : done)
+                if (e) (void)hipEventDestroy(e);
+            if (fork) (void)hipEventDestroy(fork);
+            for (auto s : branch)
+                if (s) (void)hipStreamDestroy(s);
+            (void)hipStreamDestroy(origin);
+        };
+        try {
+            for (auto& s : branch) check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+            for (auto& e : done) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+            check(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "hipEventCreate");
+            check(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                check(hipEventRecord(fork, origin), "hipEventRecord");
+                for (size_t s = 0; s < branch.size(); ++s) check(hipStreamWaitEvent(branch[s], fork, 0), "wait");
+                for (int i = 0; i < count; ++i) run_device(plans[i], a[i], b[i], c[i], branch[i % branch.size()]);
+                for (size_t s = 0; s < branch.size(); ++s) {
+                    check(hipEventRecord(done[s], branch[s]), "hipEventRecord");
+                    check(hipStreamWaitEvent(origin, done[s], 0), "wait");
+                }
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(origin, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            check(hipStreamEndCapture(origin, &bt->graph), "hipStreamEndCapture");
+            check(hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+        *out = bt.release();
+    });
+}
+
+int pm_batch_launch(pm_batch* batch, void* stream) {
+    return guarded([&] {
+        need(batch != nullptr, "batch is null");
+        check(hipGraphLaunch(batch->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
+    });
+}
+
+void pm_batch_destroy(pm_batch* batch) { delete batch; }
 
 int pm_stage_decompose(pm_plan* plan, const uint32_t* limbs, uint32_t* res, void* stream) {
     return guarded([&] {

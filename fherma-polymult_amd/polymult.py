@@ -52,6 +52,10 @@ def _load() -> ctypes.CDLL:
         "pm_multiply_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "pm_batch_multiply_device": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                                     ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+        "pm_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                           ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+        "pm_batch_launch": (ctypes.c_int, [_vp, _vp]),
+        "pm_batch_destroy": (None, [_vp]),
         "pm_stage_decompose": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
         "pm_stage_ntt": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
         "pm_stage_pointwise": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
@@ -246,3 +250,27 @@ def batch_multiply_device(plans, a_bufs, b_bufs, c_bufs, stream=None) -> None:
     arr = lambda xs: (_vp * n)(*[_vp(x) for x in xs])  # noqa: E731
     _check(lib.pm_batch_multiply_device(n, arr([p._h for p in plans]), arr([x.ptr for x in a_bufs]),
                                         arr([x.ptr for x in b_bufs]), arr([x.ptr for x in c_bufs]), _vp(stream)))
+
+
+class Batch:
+    """A batch of products captured once as a hipGraph; launch() replays it."""
+
+    def __init__(self, plans, a_bufs, b_bufs, c_bufs):
+        n = len(plans)
+        arr = lambda xs: (_vp * n)(*[_vp(x) for x in xs])  # noqa: E731
+        h = _vp()
+        _check(lib.pm_batch_create(n, arr([p._h for p in plans]), arr([x.ptr for x in a_bufs]),
+                                   arr([x.ptr for x in b_bufs]), arr([x.ptr for x in c_bufs]), ctypes.byref(h)))
+        self._h = h.value
+        self._keep = (list(plans), list(a_bufs), list(b_bufs), list(c_bufs))
+
+    def launch(self, stream=None) -> None:
+        _check(lib.pm_batch_launch(_vp(self._h), _vp(stream)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib.pm_batch_destroy(_vp(self._h))
+            self._h = None
+
+    def __del__(self):
+        self.close()

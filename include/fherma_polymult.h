@@ -80,6 +80,16 @@ int pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint
 int pm_batch_multiply_device(int count, pm_plan* const* plans, const uint32_t* const* a,
                              const uint32_t* const* b, uint32_t* const* c, void* stream);
 
+/* The same batch captured once as a hipGraph (independent products forked over
+ * several streams) and replayed by pm_batch_launch: one launch per batch, no
+ * per-kernel host overhead. The device pointers are fixed at creation; each
+ * product needs its own plan, and the plans must outlive the batch. */
+typedef struct pm_batch pm_batch;
+int pm_batch_create(int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
+                    uint32_t* const* c, pm_batch** out);
+int pm_batch_launch(pm_batch* batch, void* stream);
+void pm_batch_destroy(pm_batch* batch);
+
 /* --- the five steps, one at a time (device pointers, enqueued on `stream`) --
  * Residues are K x N u32, prime-major. The forward NTT leaves the merged
  * Cooley-Tukey transform's bit-reversed order (rns_kernels.cuh:159-174);

@@ -131,3 +131,34 @@ def test_uneven_batch_matches_gmp(golden, form):
         c = dc.download((case["N"], case["L"]))
         assert oracle.digest(c) == case["c_sha"], case["name"]
         plan.close()
+
+
+@pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
+def test_graph_batch_replays_match_gmp(golden, form):
+    """The batch captured as a hipGraph, replayed several times with eager
+    products and synchronizations in between; every replay is checked."""
+    names = sorted(n for n in golden if n.startswith("config-uneven-"))[16:28]
+    plans, bufs, cases = [], [], []
+    for name in names:
+        case = golden[name]
+        q, a, b = pointgen.operands(case)
+        plans.append(polymult.Plan(case["N"], case["W"], q, form=form))
+        bufs.append((polymult.DeviceBuffer.from_array(a), polymult.DeviceBuffer.from_array(b),
+                     polymult.DeviceBuffer(a.nbytes)))
+        cases.append(case)
+    say("graph batch", len(plans), "form", form)
+    batch = polymult.Batch(plans, [x[0] for x in bufs], [x[1] for x in bufs], [x[2] for x in bufs])
+    eager = polymult.Plan(cases[0]["N"], cases[0]["W"], pointgen.operands(cases[0])[0], form=form)
+    for rep in range(3):
+        for _, (_, _, dc) in zip(cases, bufs):
+            dc.upload(np.zeros(dc.nbytes // 4, dtype=np.uint32))
+        batch.launch()
+        polymult.synchronize()
+        for case, (_, _, dc) in zip(cases, bufs):
+            assert oracle.digest(dc.download((case["N"], case["L"]))) == case["c_sha"], (rep, case["name"])
+        # eager work between replays
+        q0, a0, b0 = pointgen.operands(cases[0])
+        assert oracle.digest(eager.multiply(a0, b0)) == cases[0]["c_sha"]
+    batch.close()
+    for p in plans:
+        p.close()
