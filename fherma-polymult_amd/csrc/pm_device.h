@@ -61,5 +61,10 @@ void launch_ntt_rows_fused_generic(const DevPlan& P, uint32_t* A, const uint32_t
 // Matrix-core form (int8 MFMA, exact).
 void launch_decompose_mfma(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st);
 void launch_crt_mfma(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream_t st);
+// The same over coefficients [k0, k0 + count) only (chunked host pipeline).
+void launch_decompose_mfma_range(const DevPlan& P, const uint32_t* limbs, uint32_t* res, uint32_t k0,
+                                 uint32_t count, hipStream_t st);
+void launch_crt_mfma_range(const DevPlan& P, const uint32_t* Y, uint32_t* c, uint32_t k0, uint32_t count,
+                           hipStream_t st);
 
 }  // namespace pm

@@ -68,164 +68,187 @@ __device__ __forceinline__ uint32_t mod_p_f64(double r, uint32_t p, double pinv)
     return static_cast<uint32_t>(ri);
 }
 
+// Decompose, one workgroup per 64 coefficients (4 tiles of 16). Their limbs are
+// copied to LDS with coalesced loads (rows of 16 KT + 4 words, zero beyond L,
+// stored as int8 digits u ^ 0x80), every wave takes its B fragments from there,
+// and the four waves split the prime groups. A fragments of the next group are
+// loaded before the current group's MFMAs.
+template <int KT>
 __global__ void __launch_bounds__(64 * kWaves)
-k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res, DevPlan P) {
-    const int lane = threadIdx.x & 63;
-    const uint32_t nbase = (blockIdx.x * kWaves + (threadIdx.x >> 6)) * (16 * kTiles);
-    if (nbase >= P.N) return;   // no barriers below
-    const int g = lane >> 4, cl = lane & 15;
-    const int L = P.L, KT = P.dec_ksteps;
-
-    // B fragments: coefficient n = nbase + 16 t + cl, bytes 64 ks + 16 g + [0, 16),
-    // i.e. limb words 16 ks + 4 g + [0, 4). Words past L pair with zero digits.
-    v4i bf[kTiles][kDecMaxKSteps];
-#pragma unroll
-    for (int t = 0; t < kTiles; ++t) {
-        const uint32_t n = nbase + 16 * t + cl;
-        const uint32_t* src = limbs + static_cast<size_t>(n) * L;
-#pragma unroll
-        for (int ks = 0; ks < kDecMaxKSteps; ++ks) {
-            v4i v = {0, 0, 0, 0};
-            if (ks < KT && n < P.N) {
-                const int w0 = 16 * ks + 4 * g;
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    if (w0 + i < L) v[i] = static_cast<int>(src[w0 + i]);
-            }
-            bf[t][ks] = v ^ static_cast<int>(0x80808080u);
-        }
+k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res, DevPlan P, uint32_t k0,
+                 uint32_t count) {
+    extern __shared__ uint32_t xs[];
+    constexpr int Ls = 16 * KT + 4;
+    const int L = P.L;
+    const uint32_t nb = k0 + blockIdx.x * (16 * kTiles);
+    const uint32_t nk = min(static_cast<uint32_t>(16 * kTiles), k0 + count - nb);
+    const uint32_t* src = limbs + static_cast<size_t>(nb) * L;
+    for (uint32_t i = threadIdx.x; i < 16u * kTiles * Ls; i += blockDim.x) {
+        const uint32_t r = i / Ls, w = i - r * Ls;
+        const bool in = r < nk && w < static_cast<uint32_t>(L);
+        const uint32_t v = src[in ? r * L + w : 0];
+        xs[i] = (in ? v : 0u) ^ 0x80808080u;
     }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, cl = lane & 15;
+    v4i bf[kTiles][KT];
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t)
+#pragma unroll
+        for (int ks = 0; ks < KT; ++ks)
+            bf[t][ks] = *reinterpret_cast<const v4i*>(xs + (16 * t + cl) * Ls + 16 * ks + 4 * g);
 
     const v4i* dig = reinterpret_cast<const v4i*>(P.dec_digits);
     const int groups = (P.K + 3) >> 2;
-    for (int jg = 0; jg < groups; ++jg) {
+    v4i an[KT];
+#pragma unroll
+    for (int ks = 0; ks < KT; ++ks) an[ks] = dig[(min(wave, groups - 1) * KT + ks) * 64 + lane];
+    for (int jg = wave; jg < groups; jg += kWaves) {
+        v4i a[KT];
+#pragma unroll
+        for (int ks = 0; ks < KT; ++ks) a[ks] = an[ks];
+        const int jn = min(jg + kWaves, groups - 1);
+#pragma unroll
+        for (int ks = 0; ks < KT; ++ks) an[ks] = dig[(jn * KT + ks) * 64 + lane];
         v4i acc[kTiles];
 #pragma unroll
         for (int t = 0; t < kTiles; ++t) acc[t] = v4i{0, 0, 0, 0};
 #pragma unroll
-        for (int ks = 0; ks < kDecMaxKSteps; ++ks) {
-            if (ks < KT) {
-                const v4i a = dig[(jg * KT + ks) * 64 + lane];
+        for (int ks = 0; ks < KT; ++ks)
 #pragma unroll
-                for (int t = 0; t < kTiles; ++t) acc[t] = mfma16(a, bf[t][ks], acc[t]);
-            }
-        }
+            for (int t = 0; t < kTiles; ++t) acc[t] = mfma16(a[ks], bf[t][ks], acc[t]);
         // Lane (g, cl) holds rows 4g .. 4g + 3 = prime 4 jg + g, digits b = 0..3.
         const int j = 4 * jg + g;
         if (j < P.K) {
             const uint32_t p = P.pc[j].p;
             const double pinv = P.p_recip[j];
             const double corr = static_cast<double>(P.dec_corr[j]);
+            uint32_t* out = res + static_cast<size_t>(j) * P.N + nb;
 #pragma unroll
             for (int t = 0; t < kTiles; ++t) {
-                const uint32_t n = nbase + 16 * t + cl;
-                if (n < P.N) {
-                    const double r = fma(16777216.0, static_cast<double>(acc[t][3]),
-                                         fma(65536.0, static_cast<double>(acc[t][2]),
-                                             fma(256.0, static_cast<double>(acc[t][1]),
-                                                 static_cast<double>(acc[t][0]) + corr)));
-                    res[static_cast<size_t>(j) * P.N + n] = mod_p_f64(r, p, pinv);
-                }
+                const double r = fma(16777216.0, static_cast<double>(acc[t][3]),
+                                     fma(65536.0, static_cast<double>(acc[t][2]),
+                                         fma(256.0, static_cast<double>(acc[t][1]),
+                                             static_cast<double>(acc[t][0]) + corr)));
+                const uint32_t v = mod_p_f64(r, p, pinv);
+                if (static_cast<uint32_t>(16 * t + cl) < nk) out[16 * t + cl] = v;
             }
         }
     }
 }
 
+// CRT, one workgroup per 64 coefficients.
+//   1. all threads: y[j][nb .. nb + 64) for every prime into LDS as int8 digits
+//      (rows of 68 words, rows past K zero), and partial sums of y_j / p_j;
+//   2. the four waves split the limb tiles st: MFMA over all (prime, byte)
+//      K-steps, limb partials (signed 64-bit) into LDS;
+//   3. wave 0, one coefficient per lane: v, the carry chain with the offset
+//      correction and -vP, the reduction mod q, and a coalesced store via LDS.
+constexpr int kYRow = 68;   // 64 coefficients + 4: lane groups g land on alternate bank halves
+
+// LDS words: y digits (phases 1-2), reused for the output transpose (phase 3);
+// limb partials (int64); partial sums of y / p (double).
+__host__ __device__ constexpr size_t crt_region0_words(int KS, int LMAX) {
+    return static_cast<size_t>(16 * KS) * kYRow > static_cast<size_t>(64) * (LMAX + 1)
+               ? static_cast<size_t>(16 * KS) * kYRow
+               : static_cast<size_t>(64) * (LMAX + 1);
+}
+
+__host__ __device__ constexpr size_t crt_lds_words(int KS, int ST, int LMAX) {
+    return crt_region0_words(KS, LMAX) + static_cast<size_t>(4 * ST) * 64 * 2 + 4 * 64 * 2;
+}
+
 template <int LMAX>
 __global__ void __launch_bounds__(64 * kWaves)
-k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P) {
-    __shared__ long long part[kWaves][4][64];
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int g = lane >> 4, cl = lane & 15;
-    const uint32_t nbase = (blockIdx.x * kWaves + wave) * (16 * kTiles);
+k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, uint32_t k0, uint32_t count) {
+    extern __shared__ uint32_t smem[];
     const int K = P.K, L = P.L, KS = P.crt_ksteps, ST = P.crt_stiles;
+    const int KR = 16 * KS;
+    uint32_t* ydig = smem;
+    uint32_t* outt = smem;   // after phase 2
+    long long* part = reinterpret_cast<long long*>(smem + crt_region0_words(KS, LMAX));
+    double* fr = reinterpret_cast<double*>(part + static_cast<size_t>(4 * ST) * 64);
+    const uint32_t nb = k0 + blockIdx.x * 64;
+    const uint32_t nk = min(64u, k0 + count - nb);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
-    // B fragments: coefficient n = nbase + 16 t + cl, K-elements (prime, byte)
-    // for primes 16 ks + 4 g + [0, 4): the four words y[j][n] as they are.
-    v4i bf[kTiles][kCrtMaxKSteps];
-    double f[kTiles];
-#pragma unroll
-    for (int t = 0; t < kTiles; ++t) {
-        const uint32_t n = nbase + 16 * t + cl;
-        f[t] = 0;
-#pragma unroll
-        for (int ks = 0; ks < kCrtMaxKSteps; ++ks) {
-            v4i v = {0, 0, 0, 0};
-            if (ks < KS) {
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int j = 16 * ks + 4 * g + i;
-                    if (j < K && n < P.N) {
-                        const uint32_t y = Y[static_cast<size_t>(j) * P.N + n];
-                        f[t] = fma(static_cast<double>(y), P.p_recip[j], f[t]);
-                        v[i] = static_cast<int>(y);
-                    }
-                }
-            }
-            bf[t][ks] = v ^ static_cast<int>(0x80808080u);
+    // 1. y digits and the partial rounding sums; loads are clamped, not branched.
+    {
+        const uint32_t col = nb + min(static_cast<uint32_t>(lane), nk - 1);
+        double f = 0;
+#pragma unroll 8
+        for (int j = wave; j < KR; j += kWaves) {
+            const bool in = j < K && static_cast<uint32_t>(lane) < nk;
+            const uint32_t y = Y[static_cast<size_t>(min(j, K - 1)) * P.N + col];
+            const uint32_t yv = in ? y : 0u;
+            ydig[j * kYRow + lane] = yv ^ 0x80808080u;
+            f = fma(static_cast<double>(yv), P.p_recip[min(j, K - 1)], f);
         }
+        fr[wave * 64 + lane] = f;
     }
-    // sum_j y_j / p_j: add the four lane groups' partial sums.
-#pragma unroll
-    for (int t = 0; t < kTiles; ++t) {
-        f[t] += __shfl_xor(f[t], 16);
-        f[t] += __shfl_xor(f[t], 32);
-    }
-    // From here on lane l owns coefficient nbase + l, i.e. tile g, column cl.
-    const double frac = g == 0 ? f[0] : g == 1 ? f[1] : g == 2 ? f[2] : f[3];
-    const int v = crt_round(frac, K);
-    const uint32_t* nv = P.negvP + static_cast<size_t>(v) * L;
-    const v4i* dig = reinterpret_cast<const v4i*>(P.crt_digits);
+    __syncthreads();
 
-    uint32_t x[LMAX + 2];
-    long long carry = 0;
-    constexpr int kSTiles = LMAX / 4 + 1;   // covers limbs 0 .. LMAX + 3
-#pragma unroll
-    for (int st = 0; st < kSTiles; ++st) {
-        if (st < ST) {
+    // 2. limb partials. A lane's rows 16 st + 4 g + r are the bytes of limb 4 st + g.
+    {
+        const int g = lane >> 4, cl = lane & 15;
+        const v4i* dig = reinterpret_cast<const v4i*>(P.crt_digits);
+        for (int st = wave; st < ST; st += kWaves) {
             v4i acc[kTiles];
 #pragma unroll
             for (int t = 0; t < kTiles; ++t) acc[t] = v4i{0, 0, 0, 0};
+            v4i an = dig[(st * KS) * 64 + lane];
+            for (int ks = 0; ks < KS; ++ks) {
+                const v4i a = an;
+                an = dig[(st * KS + min(ks + 1, KS - 1)) * 64 + lane];
+                const uint32_t* yr = ydig + (16 * ks + 4 * g) * kYRow + cl;
 #pragma unroll
-            for (int ks = 0; ks < kCrtMaxKSteps; ++ks) {
-                if (ks < KS) {
-                    const v4i a = dig[(st * KS + ks) * 64 + lane];
-#pragma unroll
-                    for (int t = 0; t < kTiles; ++t) acc[t] = mfma16(a, bf[t][ks], acc[t]);
+                for (int t = 0; t < kTiles; ++t) {
+                    v4i b;
+                    b[0] = static_cast<int>(yr[16 * t]);
+                    b[1] = static_cast<int>(yr[kYRow + 16 * t]);
+                    b[2] = static_cast<int>(yr[2 * kYRow + 16 * t]);
+                    b[3] = static_cast<int>(yr[3 * kYRow + 16 * t]);
+                    acc[t] = mfma16(a, b, acc[t]);
                 }
             }
-            // rows 16 st + 4 g + r, r = 0..3: the bytes of limb 4 st + g.
 #pragma unroll
             for (int t = 0; t < kTiles; ++t) {
-                part[wave][g][16 * t + cl] = static_cast<long long>(acc[t][0]) +
-                                             static_cast<long long>(acc[t][1]) * 256 +
-                                             static_cast<long long>(acc[t][2]) * 65536 +
-                                             static_cast<long long>(acc[t][3]) * 16777216;
+                part[(4 * st + g) * 64 + 16 * t + cl] = static_cast<long long>(acc[t][0]) +
+                                                        static_cast<long long>(acc[t][1]) * 256 +
+                                                        static_cast<long long>(acc[t][2]) * 65536 +
+                                                        static_cast<long long>(acc[t][3]) * 16777216;
             }
         }
-        __syncthreads();
+    }
+    __syncthreads();
+    if (wave != 0) return;   // no barrier follows
+
+    // 3. one coefficient per lane.
+    const double frac = fr[lane] + fr[64 + lane] + fr[128 + lane] + fr[192 + lane];
+    const int v = crt_round(frac, K);
+    const uint32_t* nv = P.negvP + static_cast<size_t>(v) * L;
+    uint32_t x[LMAX + 2];
+    long long carry = 0;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int lam = 4 * st + i;
-            if (lam < LMAX + 2) {
-                long long s = carry;
-                if (st < ST) s += part[wave][i][lane];
-                if (lam < L + 2) s += P.crt_corr[lam];
-                if (lam < L) s += nv[lam];
-                x[lam] = static_cast<uint32_t>(s);
-                carry = s >> 32;   // arithmetic: partials may be negative, the total is not
-            }
-        }
-        __syncthreads();
+    for (int lam = 0; lam < LMAX + 2; ++lam) {
+        long long s = carry;
+        if (lam < 4 * ST) s += part[lam * 64 + lane];
+        if (lam < L + 2) s += P.crt_corr[lam];
+        if (lam < L) s += nv[lam];
+        x[lam] = static_cast<uint32_t>(s);
+        carry = s >> 32;   // arithmetic: partials may be negative, the total is not
     }
     reduce_mod_q<LMAX>(x, P);
-    const uint32_t n = nbase + lane;
-    if (n < P.N) {
-        uint32_t* out = c + static_cast<size_t>(n) * L;
 #pragma unroll
-        for (int l = 0; l < LMAX; ++l)
-            if (l < L) out[l] = x[l];
+    for (int l = 0; l < LMAX; ++l)
+        if (l < L) outt[lane * (LMAX + 1) + l] = x[l];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    uint32_t* dst = c + static_cast<size_t>(nb) * L;
+    for (uint32_t i = lane; i < nk * static_cast<uint32_t>(L); i += 64) {
+        const uint32_t r = i / L;
+        dst[i] = outt[r * (LMAX + 1) + (i - r * L)];
     }
 }
 
@@ -317,20 +340,38 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
 }
 
 void launch_decompose_mfma(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st) {
+    launch_decompose_mfma_range(P, limbs, res, 0, P.N, st);
+}
+
+void launch_decompose_mfma_range(const DevPlan& P, const uint32_t* limbs, uint32_t* res, uint32_t k0,
+                                 uint32_t count, hipStream_t st) {
     if (!P.dec_digits) throw std::runtime_error("matrix-core tables not prepared");
-    const uint32_t per_block = 16 * kTiles * kWaves;
-    const dim3 grid((P.N + per_block - 1) / per_block);
-    hipLaunchKernelGGL(k_decompose_mfma, grid, dim3(64 * kWaves), 0, st, limbs, res, P);
+    if (count == 0) return;
+    const dim3 grid((count + 16 * kTiles - 1) / (16 * kTiles));
+    const size_t lds = sizeof(uint32_t) * 16 * kTiles * (16 * P.dec_ksteps + 4);
+    switch (P.dec_ksteps) {
+        case 1: hipLaunchKernelGGL(k_decompose_mfma<1>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
+        case 2: hipLaunchKernelGGL(k_decompose_mfma<2>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
+        case 3: hipLaunchKernelGGL(k_decompose_mfma<3>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
+        case 4: hipLaunchKernelGGL(k_decompose_mfma<4>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
+        default: throw std::runtime_error("L out of range");
+    }
 }
 
 void launch_crt_mfma(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream_t st) {
+    launch_crt_mfma_range(P, Y, c, 0, P.N, st);
+}
+
+void launch_crt_mfma_range(const DevPlan& P, const uint32_t* Y, uint32_t* c, uint32_t k0, uint32_t count,
+                           hipStream_t st) {
     if (!P.crt_digits) throw std::runtime_error("matrix-core tables not prepared");
-    const uint32_t per_block = 16 * kTiles * kWaves;
-    const dim3 grid((P.N + per_block - 1) / per_block);
+    if (count == 0) return;
+    const dim3 grid((count + 63) / 64);
     const int lmax = (P.L + 7) / 8 * 8;
+    const size_t lds = sizeof(uint32_t) * crt_lds_words(P.crt_ksteps, P.crt_stiles, lmax);
     switch (lmax) {
 #define PM_CRT_CASE(LM) \
-    case LM: hipLaunchKernelGGL(k_crt_mfma<LM>, grid, dim3(64 * kWaves), 0, st, Y, c, P); break;
+    case LM: hipLaunchKernelGGL(k_crt_mfma<LM>, grid, dim3(64 * kWaves), lds, st, Y, c, P, k0, count); break;
         PM_CRT_CASE(8)
         PM_CRT_CASE(16)
         PM_CRT_CASE(24)
