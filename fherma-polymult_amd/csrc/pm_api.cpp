@@ -13,9 +13,11 @@
 #include "pm_device.h"
 #include "pm_plan.h"
 #include "pm_mfma_host.h"
+#include "pm_state.h"
 
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -27,14 +29,9 @@
 namespace {
 
 thread_local std::string g_err;
-
-struct HipError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-void check(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
-}
+using pm::check;
+using pm::dev_alloc;
+using pm::HipError;
 
 template <class F>
 int guarded(F&& f) {
@@ -59,12 +56,9 @@ int guarded(F&& f) {
     }
 }
 
-void* dev_alloc(size_t bytes) {
-    void* p = nullptr;
-    if (bytes == 0) bytes = 4;
-    check(hipMalloc(&p, bytes), "hipMalloc");
-    return p;
-}
+}  // namespace
+
+namespace pm {
 
 // Device twiddle tables for (N, prime list), shared between plans.
 struct Twiddles {
@@ -75,6 +69,12 @@ struct Twiddles {
         if (itw) (void)hipFree(itw);
     }
 };
+
+}  // namespace pm
+
+namespace {
+
+using pm::Twiddles;
 
 std::mutex g_tw_mutex;
 std::map<std::vector<uint32_t>, std::weak_ptr<Twiddles>> g_tw_cache;
@@ -101,52 +101,6 @@ std::shared_ptr<Twiddles> twiddles_for(const pm::HostPlan& pl) {
 }
 
 }  // namespace
-
-struct pm_plan {
-    pm::HostPlan host;
-    pm::DevPlan dev{};
-    int form = PM_FORM_PLAIN;
-    std::shared_ptr<Twiddles> tw;
-    std::vector<void*> owned;            // device allocations
-    uint32_t* A = nullptr;               // K x N residues of a, then y
-    uint32_t* B = nullptr;               // K x N residues of b
-    uint32_t* d_limbs = nullptr;         // 3 x N x L: a, b, c for host-pointer calls
-    uint32_t* h_stage = nullptr;         // pinned host staging, 3 x N x L
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[PM_NPHASES + 1] = {};
-    bool mfma_ready = false;
-
-    ~pm_plan() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (auto e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-        for (void* p : owned) (void)hipFree(p);
-        if (h_stage) (void)hipHostFree(h_stage);
-    }
-
-    template <class T>
-    T* upload(const std::vector<T>& v) {
-        T* d = static_cast<T*>(dev_alloc(v.size() * sizeof(T)));
-        owned.push_back(d);
-        if (!v.empty()) check(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
-        return d;
-    }
-
-    size_t words() const { return static_cast<size_t>(host.N) * host.L; }
-    size_t residues() const { return static_cast<size_t>(host.K) * host.N; }
-
-    void ensure_mfma() {
-        if (mfma_ready) return;
-        pm::prepare_mfma(host, dev, [this](const void* src, size_t bytes) {
-            void* d = dev_alloc(bytes);
-            owned.push_back(d);
-            check(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice), "upload mfma tables");
-            return d;
-        });
-        mfma_ready = true;
-    }
-};
 
 namespace {
 
@@ -206,10 +160,20 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q)
     return pl.release();
 }
 
+// A captured batch: every product's kernel chain, forked over up to kBranches
+// streams (independent products run concurrently), replayed as one hipGraph.
+constexpr int kBranches = 4;
+
+const char* const kPhaseNames[PM_NPHASES] = {"decompose_a", "decompose_b", "ntt_cols_fwd_a", "ntt_cols_fwd_b",
+                                             "ntt_rows_fused", "ntt_cols_inv", "crt"};
+
+}  // namespace
+
+namespace pm {
+
 // The five phases of one product on device pointers, enqueued on st. When ev
 // is given, ev[i] is recorded before phase i and ev[PM_NPHASES] after the last.
-void run_device(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st,
-                hipEvent_t* ev = nullptr) {
+void run_device(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st, hipEvent_t* ev) {
     const pm::DevPlan& P = pl->dev;
     const bool mfma = pl->form == PM_FORM_MFMA;
     auto mark = [&](int i) {
@@ -233,14 +197,9 @@ void run_device(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, 
     check(hipGetLastError(), "kernel launch");
 }
 
-// A captured batch: every product's kernel chain, forked over up to kBranches
-// streams (independent products run concurrently), replayed as one hipGraph.
-constexpr int kBranches = 4;
+}  // namespace pm
 
-const char* const kPhaseNames[PM_NPHASES] = {"decompose_a", "decompose_b", "ntt_cols_fwd_a", "ntt_cols_fwd_b",
-                                             "ntt_rows_fused", "ntt_cols_inv", "crt"};
-
-}  // namespace
+using pm::run_device;
 
 struct pm_batch {
     hipGraph_t graph = nullptr;
@@ -313,25 +272,24 @@ int pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint
 int pm_multiply(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c) {
     return guarded([&] {
         need(plan && a && b && c, "null argument");
-        const size_t words = plan->words();
-        const size_t bytes = words * sizeof(uint32_t);
-        if (!plan->d_limbs) {
-            plan->d_limbs = static_cast<uint32_t*>(dev_alloc(3 * bytes));
-            plan->owned.push_back(plan->d_limbs);
-        }
-        if (!plan->h_stage) check(hipHostMalloc(reinterpret_cast<void**>(&plan->h_stage), 3 * bytes, 0), "hipHostMalloc");
-        uint32_t* da = plan->d_limbs;
-        uint32_t* db = da + words;
-        uint32_t* dc = db + words;
-        hipStream_t st = plan->stream;
-        std::memcpy(plan->h_stage, a, bytes);
-        check(hipMemcpyAsync(da, plan->h_stage, bytes, hipMemcpyHostToDevice, st), "copy a");
-        std::memcpy(plan->h_stage + words, b, bytes);
-        check(hipMemcpyAsync(db, plan->h_stage + words, bytes, hipMemcpyHostToDevice, st), "copy b");
-        run_device(plan, da, db, dc, st);
-        check(hipMemcpyAsync(plan->h_stage + 2 * words, dc, bytes, hipMemcpyDeviceToHost, st), "copy c");
-        check(hipStreamSynchronize(st), "synchronize");
-        std::memcpy(c, plan->h_stage + 2 * words, bytes);
+        pm::host_multiply(plan, a, b, c);
+    });
+}
+
+int pm_plan_set_host_mode(pm_plan* plan, int mode) {
+    return guarded([&] {
+        need(plan != nullptr, "plan is null");
+        need(mode >= PM_HOST_STAGED && mode <= PM_HOST_DIRECT, "unknown host mode");
+        pm::host_set_mode(plan, mode);
+    });
+}
+
+int pm_plan_tune_host(pm_plan* plan, int* mode, double* ms_per_product) {
+    return guarded([&] {
+        need(plan != nullptr, "plan is null");
+        const double ms = pm::host_autotune(plan);
+        if (mode) *mode = pm::host_get_mode(plan);
+        if (ms_per_product) *ms_per_product = ms;
     });
 }
 
@@ -557,6 +515,25 @@ void* fherma_init(const fherma::Point& p) {
     if (p.q.data.size() != p.L) throw std::runtime_error("q must have L limbs");
     pm_plan* plan = nullptr;
     if (pm_plan_create(p.N, p.W, p.L, p.q.data.data(), &plan) != PM_OK) throw std::runtime_error(g_err);
+    // Like the reference's init (solve.cu:817-862): time the host modes on dummy
+    // operands and keep the fastest, unless PM_HOST_MODE or PM_AUTOTUNE=0 says otherwise.
+    const char* mode = std::getenv("PM_HOST_MODE");
+    const char* tune = std::getenv("PM_AUTOTUNE");
+    int rc = PM_OK;
+    if (mode) {
+        const int m = !std::strcmp(mode, "staged") ? PM_HOST_STAGED
+                      : !std::strcmp(mode, "mapped") ? PM_HOST_MAPPED
+                      : !std::strcmp(mode, "direct") ? PM_HOST_DIRECT : -1;
+        rc = m < 0 ? (g_err = "PM_HOST_MODE must be staged, mapped or direct", PM_EINVAL)
+                   : pm_plan_set_host_mode(plan, m);
+    } else if (!(tune && !std::strcmp(tune, "0"))) {
+        rc = pm_plan_tune_host(plan, nullptr, nullptr);
+    }
+    if (rc != PM_OK) {
+        const std::string msg = g_err;
+        pm_plan_destroy(plan);
+        throw std::runtime_error(msg);
+    }
     return plan;
 }
 

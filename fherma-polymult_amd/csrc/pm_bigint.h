@@ -52,7 +52,7 @@ __device__ __forceinline__ void sub_mul_q(uint32_t (&x)[LMAX + 2], const uint32_
 }
 
 template <int LMAX>
-__device__ __forceinline__ void reduce_mod_q(uint32_t (&x)[LMAX + 2], const DevPlan& P) {
+__device__ __forceinline__ void reduce_mod_q(uint32_t (&x)[LMAX + 2], const DevPlan& P, const uint32_t* q) {
     double xd = 0;
 #pragma unroll
     for (int i = LMAX + 1; i >= 0; --i) {
@@ -60,9 +60,15 @@ __device__ __forceinline__ void reduce_mod_q(uint32_t (&x)[LMAX + 2], const DevP
     }
     const double f = floor(xd / P.q_top);
     const uint64_t e = f >= 1.0 ? static_cast<uint64_t>(f) - 1 : 0;
-    if (e >> 32) sub_mul_q<LMAX>(x, P.q, P.L, static_cast<uint32_t>(e >> 32), 1);
-    if (static_cast<uint32_t>(e)) sub_mul_q<LMAX>(x, P.q, P.L, static_cast<uint32_t>(e), 0);
-    for (int r = 0; r < 3 && geq_q<LMAX>(x, P.q, P.L); ++r) sub_q<LMAX>(x, P.q, P.L);
+    if (e >> 32) sub_mul_q<LMAX>(x, q, P.L, static_cast<uint32_t>(e >> 32), 1);
+    if (static_cast<uint32_t>(e)) sub_mul_q<LMAX>(x, q, P.L, static_cast<uint32_t>(e), 0);
+    for (int r = 0; r < 3 && geq_q<LMAX>(x, q, P.L); ++r) sub_q<LMAX>(x, q, P.L);
+}
+
+// The same with q read from the plan's device copy.
+template <int LMAX>
+__device__ __forceinline__ void reduce_mod_q(uint32_t (&x)[LMAX + 2], const DevPlan& P) {
+    reduce_mod_q<LMAX>(x, P, P.q);
 }
 
 __device__ __forceinline__ int crt_round(double frac, int K) {

@@ -50,6 +50,10 @@ void launch_ntt_rows(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st
 void launch_ntt_rows_fused(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st);
 void launch_pointwise(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st);
 void launch_crt_valu(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream_t st);
+void launch_decompose_valu_range(const DevPlan& P, const uint32_t* limbs, uint32_t* res, uint32_t k0,
+                                 uint32_t count, hipStream_t st);
+void launch_crt_valu_range(const DevPlan& P, const uint32_t* Y, uint32_t* c, uint32_t k0, uint32_t count,
+                           hipStream_t st);
 
 // The same NTT passes with one LDS barrier per stage, for any sub-transform
 // length up to 4096 (pm_device.hip); the launchers above use them when the

@@ -32,12 +32,13 @@ constexpr int kDecCoef = 64;      // coefficients per workgroup, one per lane
 constexpr int kDecThreads = 256;  // 4 waves; wave w takes primes w, w + 4, ...
 
 __global__ void __launch_bounds__(kDecThreads)
-k_decompose_valu(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res, DevPlan P) {
+k_decompose_valu(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res, DevPlan P, uint32_t k0,
+                 uint32_t count) {
     extern __shared__ uint32_t dec_tile[];   // kDecCoef rows of L + 1 words (odd stride)
     const int L = P.L;
     const int stride = L + 1;
-    const uint32_t n0 = blockIdx.x * kDecCoef;
-    const uint32_t nk = min(static_cast<uint32_t>(kDecCoef), P.N - n0);
+    const uint32_t n0 = k0 + blockIdx.x * kDecCoef;
+    const uint32_t nk = min(static_cast<uint32_t>(kDecCoef), k0 + count - n0);
     const uint32_t* src = limbs + static_cast<size_t>(n0) * L;
     for (uint32_t i = threadIdx.x; i < nk * L; i += blockDim.x) {
         const uint32_t r = i / L;
@@ -190,9 +191,9 @@ __global__ void k_pointwise(uint32_t* __restrict__ A, const uint32_t* __restrict
 
 template <int LMAX>
 __global__ void __launch_bounds__(128)
-k_crt_valu(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P) {
-    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= P.N) return;
+k_crt_valu(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, uint32_t k0, uint32_t count) {
+    const uint32_t n = k0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= k0 + count) return;
     const int L = P.L, K = P.K;
     uint64_t col[LMAX + 2];
 #pragma unroll
@@ -254,9 +255,15 @@ TileGeom row_geom(const DevPlan& P, uint32_t max_log_tile) {
 }  // namespace
 
 void launch_decompose_valu(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st) {
-    const dim3 grid((P.N + kDecCoef - 1) / kDecCoef);
+    launch_decompose_valu_range(P, limbs, res, 0, P.N, st);
+}
+
+void launch_decompose_valu_range(const DevPlan& P, const uint32_t* limbs, uint32_t* res, uint32_t k0,
+                                 uint32_t count, hipStream_t st) {
+    if (count == 0) return;
+    const dim3 grid((count + kDecCoef - 1) / kDecCoef);
     const size_t lds = sizeof(uint32_t) * kDecCoef * (P.L + 1);
-    hipLaunchKernelGGL(k_decompose_valu, grid, dim3(kDecThreads), lds, st, limbs, res, P);
+    hipLaunchKernelGGL(k_decompose_valu, grid, dim3(kDecThreads), lds, st, limbs, res, P, k0, count);
 }
 
 void launch_ntt_cols_generic(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
@@ -293,11 +300,17 @@ void launch_pointwise(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStrea
 }
 
 void launch_crt_valu(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream_t st) {
-    const dim3 grid((P.N + 127) / 128);
+    launch_crt_valu_range(P, Y, c, 0, P.N, st);
+}
+
+void launch_crt_valu_range(const DevPlan& P, const uint32_t* Y, uint32_t* c, uint32_t k0, uint32_t count,
+                           hipStream_t st) {
+    if (count == 0) return;
+    const dim3 grid((count + 127) / 128);
     const int lmax = (P.L + 7) / 8 * 8;
     switch (lmax) {
 #define PM_CRT_CASE(LM) \
-    case LM: hipLaunchKernelGGL(k_crt_valu<LM>, grid, dim3(128), 0, st, Y, c, P); break;
+    case LM: hipLaunchKernelGGL(k_crt_valu<LM>, grid, dim3(128), 0, st, Y, c, P, k0, count); break;
         PM_CRT_CASE(8)
         PM_CRT_CASE(16)
         PM_CRT_CASE(24)

@@ -80,6 +80,15 @@ k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
     extern __shared__ uint32_t xs[];
     constexpr int Ls = 16 * KT + 4;
     const int L = P.L;
+    // per-prime constants for the epilogue: p, corr, 1/p
+    double* kpinv = reinterpret_cast<double*>(xs + 16 * kTiles * Ls);
+    uint32_t* kp = reinterpret_cast<uint32_t*>(kpinv + P.K);
+    uint32_t* kcorr = kp + P.K;
+    for (int j = threadIdx.x; j < P.K; j += blockDim.x) {
+        kpinv[j] = P.p_recip[j];
+        kp[j] = P.pc[j].p;
+        kcorr[j] = P.dec_corr[j];
+    }
     const uint32_t nb = k0 + blockIdx.x * (16 * kTiles);
     const uint32_t nk = min(static_cast<uint32_t>(16 * kTiles), k0 + count - nb);
     const uint32_t* src = limbs + static_cast<size_t>(nb) * L;
@@ -121,9 +130,9 @@ k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
         // Lane (g, cl) holds rows 4g .. 4g + 3 = prime 4 jg + g, digits b = 0..3.
         const int j = 4 * jg + g;
         if (j < P.K) {
-            const uint32_t p = P.pc[j].p;
-            const double pinv = P.p_recip[j];
-            const double corr = static_cast<double>(P.dec_corr[j]);
+            const uint32_t p = kp[j];
+            const double pinv = kpinv[j];
+            const double corr = static_cast<double>(kcorr[j]);
             uint32_t* out = res + static_cast<size_t>(j) * P.N + nb;
 #pragma unroll
             for (int t = 0; t < kTiles; ++t) {
@@ -156,7 +165,7 @@ __host__ __device__ constexpr size_t crt_region0_words(int KS, int LMAX) {
 }
 
 __host__ __device__ constexpr size_t crt_lds_words(int KS, int ST, int LMAX) {
-    return crt_region0_words(KS, LMAX) + static_cast<size_t>(4 * ST) * 64 * 2 + 4 * 64 * 2;
+    return crt_region0_words(KS, LMAX) + static_cast<size_t>(4 * ST) * 64 * 2 + 4 * 64 * 2 + 2 * (LMAX + 2);
 }
 
 template <int LMAX>
@@ -169,6 +178,12 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
     uint32_t* outt = smem;   // after phase 2
     long long* part = reinterpret_cast<long long*>(smem + crt_region0_words(KS, LMAX));
     double* fr = reinterpret_cast<double*>(part + static_cast<size_t>(4 * ST) * 64);
+    uint32_t* qs = reinterpret_cast<uint32_t*>(fr + 4 * 64);   // q, zero above L (LMAX + 2 words)
+    uint32_t* corr = qs + LMAX + 2;                            // offset correction (LMAX + 2 words)
+    for (int i = threadIdx.x; i < LMAX + 2; i += blockDim.x) {
+        qs[i] = i < L ? P.q[i] : 0u;
+        corr[i] = i < L + 2 ? P.crt_corr[i] : 0u;
+    }
     const uint32_t nb = k0 + blockIdx.x * 64;
     const uint32_t nk = min(64u, k0 + count - nb);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -176,14 +191,26 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
     // 1. y digits and the partial rounding sums; loads are clamped, not branched.
     {
         const uint32_t col = nb + min(static_cast<uint32_t>(lane), nk - 1);
+        const bool lane_in = static_cast<uint32_t>(lane) < nk;
         double f = 0;
-#pragma unroll 8
-        for (int j = wave; j < KR; j += kWaves) {
-            const bool in = j < K && static_cast<uint32_t>(lane) < nk;
-            const uint32_t y = Y[static_cast<size_t>(min(j, K - 1)) * P.N + col];
-            const uint32_t yv = in ? y : 0u;
-            ydig[j * kYRow + lane] = yv ^ 0x80808080u;
-            f = fma(static_cast<double>(yv), P.p_recip[min(j, K - 1)], f);
+        // rows j = wave + 4 i, eight loads in flight per batch
+        for (int i0 = 0; i0 < 4 * kCrtMaxKSteps; i0 += 8) {
+            if (wave + 4 * i0 >= KR) break;
+            uint32_t y[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int j = min(wave + 4 * (i0 + u), K - 1);
+                y[u] = Y[static_cast<size_t>(j) * P.N + col];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int j = wave + 4 * (i0 + u);
+                if (j < KR) {
+                    const uint32_t yv = (j < K && lane_in) ? y[u] : 0u;
+                    ydig[j * kYRow + lane] = yv ^ 0x80808080u;
+                    f = fma(static_cast<double>(yv), P.p_recip[min(j, K - 1)], f);
+                }
+            }
         }
         fr[wave * 64 + lane] = f;
     }
@@ -197,10 +224,14 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
             v4i acc[kTiles];
 #pragma unroll
             for (int t = 0; t < kTiles; ++t) acc[t] = v4i{0, 0, 0, 0};
-            v4i an = dig[(st * KS) * 64 + lane];
-            for (int ks = 0; ks < KS; ++ks) {
-                const v4i a = an;
-                an = dig[(st * KS + min(ks + 1, KS - 1)) * 64 + lane];
+            // every A fragment of this limb tile at once (clamped, not branched)
+            v4i af[kCrtMaxKSteps];
+#pragma unroll
+            for (int ks = 0; ks < kCrtMaxKSteps; ++ks) af[ks] = dig[(st * KS + min(ks, KS - 1)) * 64 + lane];
+#pragma unroll
+            for (int ks = 0; ks < kCrtMaxKSteps; ++ks) {
+                if (ks >= KS) continue;
+                const v4i a = af[ks];
                 const uint32_t* yr = ydig + (16 * ks + 4 * g) * kYRow + cl;
 #pragma unroll
                 for (int t = 0; t < kTiles; ++t) {
@@ -229,17 +260,19 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
     const int v = crt_round(frac, K);
     const uint32_t* nv = P.negvP + static_cast<size_t>(v) * L;
     uint32_t x[LMAX + 2];
+    // -vP mod q first: all of its loads in flight together (clamped, not branched)
+#pragma unroll
+    for (int lam = 0; lam < LMAX; ++lam) x[lam] = nv[min(lam, L - 1)];
+    x[LMAX] = x[LMAX + 1] = 0;
     long long carry = 0;
 #pragma unroll
     for (int lam = 0; lam < LMAX + 2; ++lam) {
-        long long s = carry;
+        long long s = carry + corr[lam] + (lam < L ? x[lam] : 0u);
         if (lam < 4 * ST) s += part[lam * 64 + lane];
-        if (lam < L + 2) s += P.crt_corr[lam];
-        if (lam < L) s += nv[lam];
         x[lam] = static_cast<uint32_t>(s);
         carry = s >> 32;   // arithmetic: partials may be negative, the total is not
     }
-    reduce_mod_q<LMAX>(x, P);
+    reduce_mod_q<LMAX>(x, P, qs);
 #pragma unroll
     for (int l = 0; l < LMAX; ++l)
         if (l < L) outt[lane * (LMAX + 1) + l] = x[l];
@@ -348,7 +381,7 @@ void launch_decompose_mfma_range(const DevPlan& P, const uint32_t* limbs, uint32
     if (!P.dec_digits) throw std::runtime_error("matrix-core tables not prepared");
     if (count == 0) return;
     const dim3 grid((count + 16 * kTiles - 1) / (16 * kTiles));
-    const size_t lds = sizeof(uint32_t) * 16 * kTiles * (16 * P.dec_ksteps + 4);
+    const size_t lds = sizeof(uint32_t) * 16 * kTiles * (16 * P.dec_ksteps + 4) + 16 * static_cast<size_t>(P.K);
     switch (P.dec_ksteps) {
         case 1: hipLaunchKernelGGL(k_decompose_mfma<1>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
         case 2: hipLaunchKernelGGL(k_decompose_mfma<2>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;

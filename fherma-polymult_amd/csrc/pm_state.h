@@ -1,0 +1,101 @@
+// Internal state shared by the library's host-side files: the plan object
+// behind the opaque pm_plan handle, and the HIP error helpers.
+#pragma once
+
+#include "../../include/fherma_polymult.h"
+#include "pm_device.h"
+#include "pm_mfma_host.h"
+#include "pm_plan.h"
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace pm {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+inline void* dev_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (bytes == 0) bytes = 4;
+    check(hipMalloc(&p, bytes), "hipMalloc");
+    return p;
+}
+
+struct Twiddles;        // device twiddle tables shared between plans (pm_api.cpp)
+struct HostPipeline;    // host transfer pipeline of a plan (pm_host.cpp)
+void destroy_pipeline(HostPipeline* hp);
+
+}  // namespace pm
+
+// The object behind a pm_plan handle.
+struct pm_plan {
+    pm::HostPlan host;
+    pm::DevPlan dev{};
+    int form = PM_FORM_PLAIN;
+    std::shared_ptr<pm::Twiddles> tw;
+    std::vector<void*> owned;            // device allocations
+    uint32_t* A = nullptr;               // K x N residues of a, then y
+    uint32_t* B = nullptr;               // K x N residues of b
+    pm::HostPipeline* pipe = nullptr;    // host-pointer calls (created on first use)
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[PM_NPHASES + 1] = {};
+    bool mfma_ready = false;
+
+    ~pm_plan() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        pm::destroy_pipeline(pipe);
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+        for (void* p : owned) (void)hipFree(p);
+    }
+
+    template <class T>
+    T* upload(const std::vector<T>& v) {
+        T* d = static_cast<T*>(pm::dev_alloc(v.size() * sizeof(T)));
+        owned.push_back(d);
+        if (!v.empty()) pm::check(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+        return d;
+    }
+
+    size_t words() const { return static_cast<size_t>(host.N) * host.L; }
+    size_t residues() const { return static_cast<size_t>(host.K) * host.N; }
+
+    void ensure_mfma() {
+        if (mfma_ready) return;
+        pm::prepare_mfma(host, dev, [this](const void* src, size_t bytes) {
+            void* d = pm::dev_alloc(bytes);
+            owned.push_back(d);
+            pm::check(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice), "upload mfma tables");
+            return d;
+        });
+        mfma_ready = true;
+    }
+};
+
+
+namespace pm {
+
+// The five phases of one product on device pointers, enqueued on st (pm_api.cpp).
+void run_device(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st,
+                hipEvent_t* ev = nullptr);
+
+// fherma_run on host pointers through the plan's pipeline (pm_host.cpp).
+void host_multiply(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c);
+// Times every host mode on dummy operands and keeps the fastest; returns its
+// median milliseconds per product (pm_host.cpp).
+double host_autotune(pm_plan* pl);
+void host_set_mode(pm_plan* pl, int mode);
+int host_get_mode(const pm_plan* pl);
+
+}  // namespace pm

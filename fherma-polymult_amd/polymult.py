@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(HERE, "libfherma_polymult.so")
 
 PM_OK, PM_EINVAL, PM_EHIP, PM_ENOMEM, PM_EINTERNAL = 0, 1, 2, 3, 4
 FORM_PLAIN, FORM_MFMA = 0, 1
+HOST_STAGED, HOST_MAPPED, HOST_DIRECT = 0, 1, 2
 NPHASES = 7
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -49,6 +50,8 @@ def _load() -> ctypes.CDLL:
         "pm_plan_set_form": (ctypes.c_int, [_vp, ctypes.c_int]),
         "pm_choose_primes": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, _u32p, ctypes.c_uint32, _u32p]),
         "pm_multiply": (ctypes.c_int, [_vp, _u32p, _u32p, _u32p]),
+        "pm_plan_set_host_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "pm_plan_tune_host": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]),
         "pm_multiply_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "pm_batch_multiply_device": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                                     ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
@@ -200,6 +203,15 @@ class Plan:
         c = np.empty((self.N, self.L), dtype=np.uint32)
         _check(lib.pm_multiply(self.handle, _ptr(a), _ptr(b), _ptr(c)))
         return c
+
+    def set_host_mode(self, mode: int) -> None:
+        _check(lib.pm_plan_set_host_mode(self.handle, int(mode)))
+
+    def tune_host(self):
+        """(chosen host mode, its median ms per product) from the timed choice."""
+        m, ms = ctypes.c_int(0), ctypes.c_double(0)
+        _check(lib.pm_plan_tune_host(self.handle, ctypes.byref(m), ctypes.byref(ms)))
+        return m.value, ms.value
 
     def multiply_device(self, a: DeviceBuffer, b: DeviceBuffer, c: DeviceBuffer, stream=None) -> None:
         _check(lib.pm_multiply_device(self.handle, _vp(a.ptr), _vp(b.ptr), _vp(c.ptr), _vp(stream)))

@@ -74,6 +74,22 @@ int pm_plan_set_form(pm_plan* plan, int form);
 int pm_multiply(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c);
 int pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, void* stream);
 
+/* How pm_multiply moves host operands (reference solve.cu:14-34, 282-296):
+ *   PM_HOST_STAGED  chunked copies through pinned staging (parallel host memcpy,
+ *                   hipMemcpyAsync), overlapped with the decompose and CRT chunks
+ *   PM_HOST_MAPPED  the decompose reads, and the CRT writes, pinned host memory
+ *                   mapped into the device, with no DMA copies
+ *   PM_HOST_DIRECT  hipMemcpyAsync from and to the caller's pageable memory
+ * The reference's XNACK/HMM mode (kernels on pageable memory) needs xnack+ code
+ * objects and HSA_XNACK=1, which this build does not assume. */
+enum pm_host_mode { PM_HOST_STAGED = 0, PM_HOST_MAPPED = 1, PM_HOST_DIRECT = 2 };
+int pm_plan_set_host_mode(pm_plan* plan, int mode);
+/* Times each host mode on dummy operands (one warm-up, median of five) and keeps
+ * the fastest; staged stays unless another is at least 5 % faster
+ * (reference solve.cu:817-862). fherma_init calls it unless PM_HOST_MODE names a
+ * mode or PM_AUTOTUNE=0. Either output may be null. */
+int pm_plan_tune_host(pm_plan* plan, int* mode, double* ms_per_product);
+
 /* A batch of independent products, each with its own plan (N and q may differ),
  * all device pointers, enqueued on `stream`. No counterpart in the reference:
  * it is for callers with many products per call (uneven batches). */

@@ -72,3 +72,18 @@ def test_not_enough_primes_is_refused():
     assert e.value.code == polymult.PM_EINVAL
     with pytest.raises(ValueError):
         oracle.choose_primes(1 << 22, 2048)
+
+
+def test_driver_refuses_out_of_range_points(tmp_path):
+    """N = 2^23 and W = 2049 (L = 65) are refused by fherma_init before any
+    device work: non-zero exit, no answers (reference fuzz.py:149-160)."""
+    import json
+    solution = os.path.join(ROOT, "fherma-polymult_amd", "solution")
+    for N, W, L in ((1 << 23, 32, 1), (64, 2049, 65)):
+        d = tmp_path / ("p%d_%d" % (N, W))
+        (d / "point").mkdir(parents=True)
+        np.ones(L, dtype="<u4").tofile(str(d / "point" / "q.bin"))
+        (d / "manifest.json").write_text(json.dumps({"N": N, "W": W, "L": L, "cases": 1}))
+        r = subprocess.run([solution, str(d)], capture_output=True, text=True, timeout=60)
+        assert r.returncode != 0
+        assert not (d / "out" / "000000").exists()
