@@ -64,9 +64,14 @@ namespace pm {
 struct Twiddles {
     uint2* tw = nullptr;
     uint2* itw = nullptr;
+    uint2* rz = nullptr;
+    uint2* rzi = nullptr;
+    uint32_t* rom = nullptr;
+    uint32_t* romi = nullptr;
     ~Twiddles() {
-        if (tw) (void)hipFree(tw);
-        if (itw) (void)hipFree(itw);
+        for (void* p : {static_cast<void*>(tw), static_cast<void*>(itw), static_cast<void*>(rz),
+                        static_cast<void*>(rzi), static_cast<void*>(rom), static_cast<void*>(romi)})
+            if (p) (void)hipFree(p);
     }
 };
 
@@ -96,6 +101,18 @@ std::shared_ptr<Twiddles> twiddles_for(const pm::HostPlan& pl) {
     check(hipMemcpy(t->tw, host.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice), "upload twiddles");
     pm::build_twiddles(pl, false, 0, pl.K, host.data());
     check(hipMemcpy(t->itw, host.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice), "upload twiddles");
+    const size_t zw = 2 * static_cast<size_t>(pl.K) * (size_t(1) << pl.logR) * pl.logS;
+    const size_t ow = static_cast<size_t>(pl.K) * ((size_t(1) << pl.logS) / 2);
+    std::vector<uint32_t> z(zw), o(ow);
+    for (int fwd = 1; fwd >= 0; --fwd) {
+        pm::build_row_twiddles(pl, fwd != 0, 0, pl.K, z.data(), o.data());
+        uint2* dz = static_cast<uint2*>(dev_alloc(zw * sizeof(uint32_t)));
+        uint32_t* dom = static_cast<uint32_t*>(dev_alloc(ow * sizeof(uint32_t)));
+        (fwd ? t->rz : t->rzi) = dz;
+        (fwd ? t->rom : t->romi) = dom;
+        check(hipMemcpy(dz, z.data(), zw * sizeof(uint32_t), hipMemcpyHostToDevice), "upload row twiddles");
+        check(hipMemcpy(dom, o.data(), ow * sizeof(uint32_t), hipMemcpyHostToDevice), "upload row twiddles");
+    }
     g_tw_cache[key] = t;
     return t;
 }
@@ -128,6 +145,9 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q)
         pc[j].one_s = pm::shoup_companion(1, p);
         pc[j].scale = h.scale[j];
         pc[j].scale_s = h.scale_s[j];
+        const unsigned __int128 mu = (static_cast<unsigned __int128>(1) << 64) / p;
+        pc[j].mu_k = static_cast<uint32_t>(mu >> 32);
+        pc[j].mu_lo = static_cast<uint32_t>(mu);
     }
     pl->tw = twiddles_for(h);
 
@@ -144,6 +164,10 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q)
     d.pow2 = pl->upload(h.pow2);
     d.tw = pl->tw->tw;
     d.itw = pl->tw->itw;
+    d.rz = pl->tw->rz;
+    d.rzi = pl->tw->rzi;
+    d.rom = pl->tw->rom;
+    d.romi = pl->tw->romi;
     d.p_recip = pl->upload(h.p_recip);
     d.M = pl->upload(h.M);
     d.negvP = pl->upload(h.negvP);

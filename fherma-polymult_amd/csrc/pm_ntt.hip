@@ -48,10 +48,53 @@ __device__ __forceinline__ void bfly(uint32_t& x, uint32_t& y, const uint2 w, ui
         gs(x, y, w, p);
 }
 
+// Twiddle sources: tw(e, b) is the twiddle of block b (counted from the start
+// of the sub-transform) of the stage with distance 2^e.
+struct TableTw {   // the full K x N table: index (B >> (e + 1)) + b
+    const uint2* __restrict__ tab;
+    uint32_t B;
+    __device__ __forceinline__ uint2 operator()(int e, uint32_t b) const { return tab[(B >> (e + 1)) + b]; }
+};
+
+// w = Z * Om mod p and its Shoup companion floor(w 2^32 / p), from
+// floor(2^64 / p) = mu_k 2^32 + mu_lo: the estimate mu_k w + mulhi(w, mu_lo) is
+// exact or one short, and the remainder -q p mod 2^32 tells which.
+__device__ __forceinline__ uint2 make_twiddle(uint32_t om, uint2 z, const PrimeConst& pc) {
+    const uint32_t w = mul_shoup(om, z.x, z.y, pc.p);
+    uint32_t q = pc.mu_k * w + mulhi32(w, pc.mu_lo);
+    q += (0u - q * pc.p) >= pc.p;
+    return make_uint2(w, q);
+}
+
+struct GenTw {     // row rho: Z[rho][e] * Om[b]
+    const uint2* __restrict__ z;
+    const uint32_t* __restrict__ om;
+    PrimeConst pc;
+    __device__ __forceinline__ uint2 operator()(int e, uint32_t b) const { return make_twiddle(om[b], z[e], pc); }
+};
+
+// Phase A twiddles of one row, precomputed in LDS: stage e, block b at
+// 2^(m-1-e) - 1 + b (2^(m-4) - 1 entries).
+template <int LOGM>
+struct LdsTwA {
+    const uint2* tab;
+    __device__ __forceinline__ uint2 operator()(int e, uint32_t b) const {
+        return tab[(1u << (LOGM - 1 - e)) - 1 + b];
+    }
+};
+
+template <int LOGM>
+__device__ __forceinline__ void fill_row_phase_a(uint2* tab, const GenTw& g, uint32_t t) {
+    constexpr uint32_t n = (1u << (LOGM - 4)) - 1;
+    if (t < n) {
+        const uint32_t m = 31 - __clz(t + 1);   // floor(log2(t + 1))
+        tab[t] = g(LOGM - 1 - m, t + 1 - (1u << m));
+    }
+}
+
 // Phase A on one or two register sets (kTwo: x and y share the twiddles).
-template <int LOGM, bool kFwd, bool kTwo>
-__device__ __forceinline__ void phase_a(uint32_t (&x)[16], uint32_t (&y)[16], const uint2* __restrict__ tw,
-                                        uint32_t B, uint32_t p) {
+template <int LOGM, bool kFwd, bool kTwo, class TW>
+__device__ __forceinline__ void phase_a(uint32_t (&x)[16], uint32_t (&y)[16], const TW& tw, uint32_t p) {
     constexpr int GA = 1 << (LOGM - 4), NG = 16 / GA;
 #pragma unroll
     for (int s = 0; s < LOGM - 4; ++s) {
@@ -59,7 +102,7 @@ __device__ __forceinline__ void phase_a(uint32_t (&x)[16], uint32_t (&y)[16], co
         const int d = 1 << (e - 4);
 #pragma unroll
         for (int blk = 0; blk < GA / (2 * d); ++blk) {
-            const uint2 w = tw[(B >> (e + 1)) + blk];
+            const uint2 w = tw(e, blk);
 #pragma unroll
             for (int i = 0; i < NG; ++i) {
 #pragma unroll
@@ -72,16 +115,16 @@ __device__ __forceinline__ void phase_a(uint32_t (&x)[16], uint32_t (&y)[16], co
     }
 }
 
-template <bool kFwd, bool kTwo>
-__device__ __forceinline__ void phase_b(uint32_t (&x)[16], uint32_t (&y)[16], const uint2* __restrict__ tw,
-                                        uint32_t B, uint32_t t, uint32_t p) {
+template <bool kFwd, bool kTwo, class TW>
+__device__ __forceinline__ void phase_b(uint32_t (&x)[16], uint32_t (&y)[16], const TW& tw, uint32_t t,
+                                        uint32_t p) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const int e = kFwd ? 3 - s : s;
         const int d = 1 << e;
 #pragma unroll
         for (int blk = 0; blk < 8 / d; ++blk) {
-            const uint2 w = tw[(B >> (e + 1)) + (t << (3 - e)) + blk];
+            const uint2 w = tw(e, (t << (3 - e)) + blk);
 #pragma unroll
             for (int k = blk * 2 * d; k < blk * 2 * d + d; ++k) {
                 bfly<kFwd>(x[k], x[k + d], w, p);
@@ -104,7 +147,7 @@ k_ntt_col_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logC) {
     const int j = blockIdx.y;
     const size_t S = size_t(1) << P.logS;
     uint32_t* x = X + static_cast<size_t>(j) * P.N + (blockIdx.x << logC) + c;
-    const uint2* tw = (kFwd ? P.tw : P.itw) + static_cast<size_t>(j) * P.N;
+    const TableTw tw{(kFwd ? P.tw : P.itw) + static_cast<size_t>(j) * P.N, M};
     const uint32_t p = P.pc[j].p;
     uint32_t r[16];
     if (kFwd) {
@@ -112,7 +155,7 @@ k_ntt_col_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logC) {
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
             for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = x[(t + TPS * i + 16 * k) * S];
-        phase_a<LOGM, true, false>(r, r, tw, M, p);
+        phase_a<LOGM, true, false>(r, r, tw, p);
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -120,13 +163,13 @@ k_ntt_col_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logC) {
         __syncthreads();
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) r[k] = lds[padded(16 * t + k) * C + c];
-        phase_b<true, false>(r, r, tw, M, t, p);
+        phase_b<true, false>(r, r, tw, t, p);
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) x[(16 * t + k) * S] = r[k];
     } else {
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) r[k] = x[(16 * t + k) * S];
-        phase_b<false, false>(r, r, tw, M, t, p);
+        phase_b<false, false>(r, r, tw, t, p);
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) lds[padded(16 * t + k) * C + c] = r[k];
         __syncthreads();
@@ -134,7 +177,7 @@ k_ntt_col_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logC) {
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
             for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = lds[padded(t + TPS * i + 16 * k) * C + c];
-        phase_a<LOGM, false, false>(r, r, tw, M, p);
+        phase_a<LOGM, false, false>(r, r, tw, p);
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -143,26 +186,34 @@ k_ntt_col_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logC) {
 }
 
 // Row pass alone (forward or inverse): G = 2^logG rows of M = S elements.
+// Twiddles are generated per row (GenTw); phase A's are computed once per row
+// into LDS by the row's first threads.
 template <int LOGM, bool kFwd>
 __global__ void __launch_bounds__(512)
 k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
     extern __shared__ uint32_t lds[];
     constexpr uint32_t M = 1u << LOGM, TPS = M / 16, GA = TPS, NG = 16 / GA, ROW = M + M / 16;
     const uint32_t t = threadIdx.x & (TPS - 1), rl = threadIdx.x / TPS;
+    const uint32_t G = 1u << logG;
     const uint32_t rho = (blockIdx.x << logG) + rl;
     const int j = blockIdx.y;
     uint32_t* x = X + static_cast<size_t>(j) * P.N + static_cast<size_t>(rho) * M;
-    const uint2* tw = (kFwd ? P.tw : P.itw) + static_cast<size_t>(j) * P.N;
-    const uint32_t p = P.pc[j].p;
-    const uint32_t B = P.N + rho * M;
+    const PrimeConst pc = P.pc[j];
+    const uint32_t R = 1u << P.logR;
+    const GenTw gen{(kFwd ? P.rz : P.rzi) + (static_cast<size_t>(j) * R + rho) * LOGM,
+                    (kFwd ? P.rom : P.romi) + static_cast<size_t>(j) * (M / 2), pc};
     uint32_t* l = lds + rl * ROW;
+    uint2* twa = reinterpret_cast<uint2*>(lds + G * ROW) + rl * GA;
+    fill_row_phase_a<LOGM>(twa, gen, t);
+    const LdsTwA<LOGM> ta{twa};
     uint32_t r[16];
     if (kFwd) {
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
             for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = x[t + TPS * i + 16 * k];
-        phase_a<LOGM, true, false>(r, r, tw, B, p);
+        __syncthreads();
+        phase_a<LOGM, true, false>(r, r, ta, pc.p);
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -170,14 +221,14 @@ k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
         __syncthreads();
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) r[k] = l[padded(16 * t + k)];
-        phase_b<true, false>(r, r, tw, B, t, p);
+        phase_b<true, false>(r, r, gen, t, pc.p);
         // phase B layout: 16 consecutive words per thread
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) x[16 * t + k] = r[k];
     } else {
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) r[k] = x[16 * t + k];
-        phase_b<false, false>(r, r, tw, B, t, p);
+        phase_b<false, false>(r, r, gen, t, pc.p);
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) l[padded(16 * t + k)] = r[k];
         __syncthreads();
@@ -185,7 +236,7 @@ k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
             for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = l[padded(t + TPS * i + 16 * k)];
-        phase_a<LOGM, false, false>(r, r, tw, B, p);
+        phase_a<LOGM, false, false>(r, r, ta, pc.p);
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -209,12 +260,17 @@ k_ntt_rows_fused_reg(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, 
     const uint32_t rho = (blockIdx.x << logG) + rl;
     const int j = blockIdx.y;
     const size_t row = static_cast<size_t>(j) * P.N + static_cast<size_t>(rho) * M;
-    const uint2* tw = P.tw + static_cast<size_t>(j) * P.N;
-    const uint2* itw = P.itw + static_cast<size_t>(j) * P.N;
     const PrimeConst pc = P.pc[j];
-    const uint32_t B = P.N + rho * M;
+    const uint32_t R = 1u << P.logR;
+    const size_t zoff = (static_cast<size_t>(j) * R + rho) * LOGM, ooff = static_cast<size_t>(j) * (M / 2);
+    const GenTw fw{P.rz + zoff, P.rom + ooff, pc};
+    const GenTw iw{P.rzi + zoff, P.romi + ooff, pc};
     uint32_t* la = lds + rl * ROW;
     uint32_t* lb = lds + (G + rl) * ROW;
+    uint2* twa = reinterpret_cast<uint2*>(lds + 2 * G * ROW) + rl * 2 * GA;
+    fill_row_phase_a<LOGM>(twa, fw, t);
+    fill_row_phase_a<LOGM>(twa + GA, iw, t);
+    const LdsTwA<LOGM> fa{twa}, ia{twa + GA};
     uint32_t a[16], b[16];
 #pragma unroll
     for (uint32_t i = 0; i < NG; ++i)
@@ -223,7 +279,8 @@ k_ntt_rows_fused_reg(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, 
             a[i * GA + k] = A[row + t + TPS * i + 16 * k];
             b[i * GA + k] = Bm[row + t + TPS * i + 16 * k];
         }
-    phase_a<LOGM, true, true>(a, b, tw, B, pc.p);
+    __syncthreads();
+    phase_a<LOGM, true, true>(a, b, fa, pc.p);
 #pragma unroll
     for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -237,10 +294,10 @@ k_ntt_rows_fused_reg(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, 
         a[k] = la[padded(16 * t + k)];
         b[k] = lb[padded(16 * t + k)];
     }
-    phase_b<true, true>(a, b, tw, B, t, pc.p);
+    phase_b<true, true>(a, b, fw, t, pc.p);
 #pragma unroll
     for (uint32_t k = 0; k < 16; ++k) a[k] = pointwise_mul(a[k], b[k], pc);
-    phase_b<false, false>(a, a, itw, B, t, pc.p);
+    phase_b<false, false>(a, a, iw, t, pc.p);
     // Each thread rewrites only the LDS words it read above: no barrier needed first.
 #pragma unroll
     for (uint32_t k = 0; k < 16; ++k) la[padded(16 * t + k)] = a[k];
@@ -249,7 +306,7 @@ k_ntt_rows_fused_reg(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, 
     for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
         for (uint32_t k = 0; k < GA; ++k) a[i * GA + k] = la[padded(t + TPS * i + 16 * k)];
-    phase_a<LOGM, false, false>(a, a, itw, B, pc.p);
+    phase_a<LOGM, false, false>(a, a, ia, pc.p);
 #pragma unroll
     for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -287,7 +344,8 @@ void row_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     const uint32_t logG = row_logG(P);
     const dim3 grid(1u << (P.logR - logG), P.K);
     const dim3 block(1u << (logG + LOGM - 4));
-    const size_t lds = sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG;
+    const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
+                       (sizeof(uint2) << (LOGM - 4 + logG));
     if (forward)
         hipLaunchKernelGGL((k_ntt_row_reg<LOGM, true>), grid, block, lds, st, X, P, logG);
     else
@@ -299,7 +357,8 @@ void fused_launch(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t 
     const uint32_t logG = row_logG(P);
     const dim3 grid(1u << (P.logR - logG), P.K);
     const dim3 block(1u << (logG + LOGM - 4));
-    const size_t lds = 2 * (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG);
+    const size_t lds = 2 * (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
+                       (2 * sizeof(uint2) << (LOGM - 4 + logG));
     hipLaunchKernelGGL((k_ntt_rows_fused_reg<LOGM>), grid, block, lds, st, A, B, P, logG);
 }
 

@@ -246,6 +246,28 @@ void build_twiddles(const HostPlan& pl, bool forward, int j0, int j1, uint32_t* 
     }
 }
 
+void build_row_twiddles(const HostPlan& pl, bool forward, int j0, int j1, uint32_t* z_pairs, uint32_t* om) {
+    const uint32_t R = 1u << pl.logR, S = 1u << pl.logS;
+    for (int j = j0; j < j1; ++j) {
+        const uint32_t p = pl.p[j];
+        const uint32_t r = forward ? pl.psi[j] : pl.ipsi[j];
+        uint32_t* z = z_pairs + 2 * static_cast<size_t>(j - j0) * R * pl.logS;
+        for (uint32_t rho = 0; rho < R; ++rho) {
+            const uint64_t base = 2ull * (pl.logR ? bitrev32(rho, pl.logR) : 0) + 1;
+            for (uint32_t e = 0; e < pl.logS; ++e) {
+                const uint32_t w = pow_mod(r, (base << e) % (2ull * pl.N), p);
+                z[2 * (rho * pl.logS + e)] = w;
+                z[2 * (rho * pl.logS + e) + 1] = shoup_companion(w, p);
+            }
+        }
+        uint32_t* o = om + static_cast<size_t>(j - j0) * (S / 2);
+        for (uint32_t b = 0; b < S / 2; ++b) {
+            const uint64_t e = pl.logS > 1 ? 2ull * R * bitrev32(b, pl.logS - 1) : 0;
+            o[b] = pow_mod(r, e % (2ull * pl.N), p);
+        }
+    }
+}
+
 void balanced_digits(const uint32_t* x, int limbs, int8_t* out, int ndigits) {
     int carry = 0;
     for (int i = 0; i < ndigits; ++i) {

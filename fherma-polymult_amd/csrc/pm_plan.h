@@ -49,6 +49,14 @@ std::vector<uint32_t> choose_primes(uint32_t N, int qbits);
 // as (w, floor(w 2^32 / p)) pairs, for primes [j0, j1) of the plan.
 void build_twiddles(const HostPlan& pl, bool forward, int j0, int j1, uint32_t* out_pairs);
 
+// Row-pass twiddle factors for primes [j0, j1): the twiddle psi^bitrev_N(idx)
+// of row rho (S contiguous elements), stage distance 2^e, block b is
+//     Z[rho][e] * Om[b],  Z[rho][e] = psi^(2^e (2 bitrev_R(rho) + 1)),
+//                         Om[b]     = psi^(2 R bitrev_{logS-1}(b)),  b < S/2,
+// (psi^-1 for the inverse). z_pairs: (R x logS) (Z, Shoup companion) per
+// prime; om: S/2 values per prime.
+void build_row_twiddles(const HostPlan& pl, bool forward, int j0, int j1, uint32_t* z_pairs, uint32_t* om);
+
 // Balanced base-256 digits: x = sum_i d_i 256^i with d_i in [-128, 127].
 // Used for the int8 matrix-core forms.
 void balanced_digits(const uint32_t* x, int limbs, int8_t* out, int ndigits);

@@ -29,31 +29,56 @@ def gs(x, y, w, p):
     return (x + y) % p, (x - y) % p * w % p
 
 
-def phase_a(regs, LOGM, fwd, tw, B, p):
+def table_tw(tab, B):
+    return lambda e, b: int(tab[(B >> (e + 1)) + b])
+
+
+def gen_tw(psi, p, rho, logR, logS, N, fwd):
+    """Row rho's twiddles as the kernels generate them: Z[rho][e] * Om[b]."""
+    r = psi if fwd else pow(psi, p - 2, p)
+    R = 1 << logR
+    base = 2 * (int(format(rho, "0%db" % logR)[::-1], 2) if logR else 0) + 1
+    Z = [pow(r, (base << e) % (2 * N), p) for e in range(logS)]
+    Om = [pow(r, (2 * R * (int(format(b, "0%db" % (logS - 1))[::-1], 2) if logS > 1 else 0)) % (2 * N), p)
+          for b in range(1 << (logS - 1))]
+    return lambda e, b: Z[e] * Om[b] % p
+
+
+def lds_phase_a(gen, LOGM):
+    """fill_row_phase_a + LdsTwA: slot t holds stage LOGM-1-m, block t+1-2^m."""
+    n = (1 << (LOGM - 4)) - 1
+    tab = [None] * n
+    for t in range(n):
+        m = (t + 1).bit_length() - 1
+        tab[t] = gen(LOGM - 1 - m, t + 1 - (1 << m))
+    return lambda e, b: tab[(1 << (LOGM - 1 - e)) - 1 + b]
+
+
+def phase_a(regs, LOGM, fwd, tw, p):
     GA = 1 << (LOGM - 4)
     NG = 16 // GA
     for s in range(LOGM - 4):
         e = LOGM - 1 - s if fwd else 4 + s
         d = 1 << (e - 4)
         for blk in range(GA // (2 * d)):
-            w = int(tw[(B >> (e + 1)) + blk])
+            w = tw(e, blk)
             for i in range(NG):
                 for k in range(blk * 2 * d, blk * 2 * d + d):
                     a, b = i * GA + k, i * GA + k + d
                     regs[a], regs[b] = (ct if fwd else gs)(regs[a], regs[b], w, p)
 
 
-def phase_b(regs, fwd, tw, B, t, p):
+def phase_b(regs, fwd, tw, t, p):
     for s in range(4):
         e = 3 - s if fwd else s
         d = 1 << e
         for blk in range(8 // d):
-            w = int(tw[(B >> (e + 1)) + (t << (3 - e)) + blk])
+            w = tw(e, (t << (3 - e)) + blk)
             for k in range(blk * 2 * d, blk * 2 * d + d):
                 regs[k], regs[k + d] = (ct if fwd else gs)(regs[k], regs[k + d], w, p)
 
 
-def sub_transform(vals, LOGM, fwd, tw, B, p):
+def sub_transform(vals, LOGM, fwd, tw_a, tw_b, p):
     """One column or row through phase A / LDS / phase B exactly as a workgroup
     of M/16 threads runs it; returns the new values in element order."""
     M = 1 << LOGM
@@ -67,9 +92,9 @@ def sub_transform(vals, LOGM, fwd, tw, B, p):
     regs = [[int(vals[u]) for u in first[t]] for t in range(TPS)]
     for t in range(TPS):
         if fwd:
-            phase_a(regs[t], LOGM, True, tw, B, p)
+            phase_a(regs[t], LOGM, True, tw_a, p)
         else:
-            phase_b(regs[t], False, tw, B, t, p)
+            phase_b(regs[t], False, tw_b, t, p)
     lds = [None] * (M + M // 16)
     for t in range(TPS):
         for r, u in enumerate(first[t]):
@@ -78,9 +103,9 @@ def sub_transform(vals, LOGM, fwd, tw, B, p):
     regs = [[lds[padded(u)] for u in second[t]] for t in range(TPS)]
     for t in range(TPS):
         if fwd:
-            phase_b(regs[t], True, tw, B, t, p)
+            phase_b(regs[t], True, tw_b, t, p)
         else:
-            phase_a(regs[t], LOGM, False, tw, B, p)
+            phase_a(regs[t], LOGM, False, tw_a, p)
     out = [0] * M
     for t in range(TPS):
         for r, u in enumerate(second[t]):
@@ -88,7 +113,7 @@ def sub_transform(vals, LOGM, fwd, tw, B, p):
     return out
 
 
-def emulate(x, tw, itw, N, p, fwd):
+def emulate(x, tw, itw, N, p, fwd, psi):
     logN = N.bit_length() - 1
     logS, logR = geometry(logN)
     S, R = 1 << logS, 1 << logR
@@ -97,13 +122,17 @@ def emulate(x, tw, itw, N, p, fwd):
     def cols(table, direction):
         for c in range(S):
             col = [x[c + u * S] for u in range(R)]
-            col = sub_transform(col, logR, direction, table, R, p)
+            t = table_tw(table, R)
+            col = sub_transform(col, logR, direction, t, t, p)
             for u in range(R):
                 x[c + u * S] = col[u]
     def rows(table, direction):
         for rho in range(R):
             row = x[rho * S:(rho + 1) * S]
-            x[rho * S:(rho + 1) * S] = sub_transform(row, logS, direction, table, N + rho * S, p)
+            g = gen_tw(psi, p, rho, logR, logS, N, direction)
+            assert all(g(e, b) == int(table[((N + rho * S) >> (e + 1)) + b])
+                       for e in range(logS) for b in range(S >> (e + 1)))
+            x[rho * S:(rho + 1) * S] = sub_transform(row, logS, direction, lds_phase_a(g, logS), g, p)
     if fwd:
         cols(tw, True)
         rows(tw, True)
@@ -128,5 +157,20 @@ def test_register_schedule_matches_oracle_ntt(N):
         brv = [int(format(i, "0%db" % logN)[::-1], 2) for i in range(N)]
         tw = np.array([pow(psi, e, p) for e in brv], dtype=object)
         itw = np.array([pow(ipsi, e, p) for e in brv], dtype=object)
-        assert emulate(res[j], tw, itw, N, p, True) == fwd[j].tolist()
-        assert emulate(fwd[j], tw, itw, N, p, False) == inv[j].tolist()
+        assert emulate(res[j], tw, itw, N, p, True, psi) == fwd[j].tolist()
+        assert emulate(fwd[j], tw, itw, N, p, False, psi) == inv[j].tolist()
+
+
+def test_shoup_companion_on_the_fly():
+    """make_twiddle's companion: mu_k w + mulhi(w, mu_lo), plus one if the
+    remainder -q p mod 2^32 is at least p, equals floor(w 2^32 / p)."""
+    import random
+    rng = random.Random(1)
+    for p in list(oracle.choose_primes(65536, 2048)[:8]) + list(oracle.choose_primes(4096, 256)[-4:]):
+        p = int(p)
+        mu = (1 << 64) // p
+        k, lo = mu >> 32, mu & 0xFFFFFFFF
+        for w in [1, 2, p - 1, p - 2] + [rng.randrange(1, p) for _ in range(20000)]:
+            q = (k * w + ((w * lo) >> 32)) & 0xFFFFFFFF
+            q += 1 if ((-q * p) & 0xFFFFFFFF) >= p else 0
+            assert q == (w << 32) // p
