@@ -172,6 +172,8 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q)
     d.M = pl->upload(h.M);
     d.negvP = pl->upload(h.negvP);
     d.q = pl->upload(h.q);
+    d.mq = pl->upload(h.mq);
+    d.nq = pl->upload(h.nq);
 
     pl->A = static_cast<uint32_t*>(dev_alloc(pl->residues() * sizeof(uint32_t)));
     pl->owned.push_back(pl->A);

@@ -40,6 +40,11 @@ struct DevPlan {
     const uint32_t* M;        // K x L: P/p_j mod q
     const uint32_t* negvP;    // (K + 1) x L: -vP mod q
     const uint32_t* q;        // L
+    // Quotient estimate for the final reduction: floor(T / q) with
+    // T = sum_j y_j M_j + negvP_v is sum_j y_j mq[j] + nq[v] in FP64, exact or
+    // one off (the accumulated error stays below 2^-6).
+    const double* mq;         // K: M_j / q
+    const double* nq;         // K + 1: negvP_v / q
     // int8 matrix-core forms (pm_mfma.hip); null until prepared.
     const int8_t* dec_digits; // [ceil(K/4)][dec_ksteps][64 lanes][16]: MFMA A fragments
     const uint32_t* dec_corr; // K: 128 sum_t (256^t mod p_j) mod p_j

@@ -198,13 +198,17 @@ k_crt_valu(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
     uint64_t col[LMAX + 2];
 #pragma unroll
     for (int l = 0; l < LMAX + 2; ++l) col[l] = 0;
-    double frac = 0;
+    double frac = 0, qf = 0;
     for (int j = 0; j < K; j += 2) {
         const bool two = j + 1 < K;
         const uint32_t y0 = Y[static_cast<size_t>(j) * P.N + n];
         const uint32_t y1 = two ? Y[static_cast<size_t>(j + 1) * P.N + n] : 0u;
         frac = fma(static_cast<double>(y0), P.p_recip[j], frac);
-        if (two) frac = fma(static_cast<double>(y1), P.p_recip[j + 1], frac);
+        qf = fma(static_cast<double>(y0), P.mq[j], qf);
+        if (two) {
+            frac = fma(static_cast<double>(y1), P.p_recip[j + 1], frac);
+            qf = fma(static_cast<double>(y1), P.mq[j + 1], qf);
+        }
         const uint32_t* m0 = P.M + static_cast<size_t>(j) * L;
         const uint32_t* m1 = two ? m0 + L : m0;
 #pragma unroll
@@ -227,7 +231,18 @@ k_crt_valu(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
         x[l] = static_cast<uint32_t>(s);
         carry = s >> 32;
     }
-    reduce_mod_q<LMAX>(x, P);
+    // minus Q q, Q = floor(T / q) from its FP64 estimate (exact or one off)
+    const uint64_t Q = quotient_estimate(qf + P.nq[v]);
+    const uint32_t Ql = static_cast<uint32_t>(Q), Qh = static_cast<uint32_t>(Q >> 32);
+    uint64_t a1 = 0, b1 = 0, b2 = 0;
+    int64_t borrow = 0;
+#pragma unroll
+    for (int l = 0; l < LMAX + 2; ++l) {
+        const int64_t s = static_cast<int64_t>(x[l]) - q_times_limb(Ql, Qh, l < L ? P.q[l] : 0u, a1, b1, b2) + borrow;
+        x[l] = static_cast<uint32_t>(s);
+        borrow = s >> 32;
+    }
+    fix_range<LMAX>(x, P.q, L);
     uint32_t* out = c + static_cast<size_t>(n) * L;
 #pragma unroll
     for (int l = 0; l < LMAX; ++l)

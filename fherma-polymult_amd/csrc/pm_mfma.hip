@@ -165,7 +165,7 @@ __host__ __device__ constexpr size_t crt_region0_words(int KS, int LMAX) {
 }
 
 __host__ __device__ constexpr size_t crt_lds_words(int KS, int ST, int LMAX) {
-    return crt_region0_words(KS, LMAX) + static_cast<size_t>(4 * ST) * 64 * 2 + 4 * 64 * 2 + 2 * (LMAX + 2);
+    return crt_region0_words(KS, LMAX) + static_cast<size_t>(4 * ST) * 64 * 2 + 2 * 4 * 64 * 2 + 2 * (LMAX + 2);
 }
 
 template <int LMAX>
@@ -178,7 +178,8 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
     uint32_t* outt = smem;   // after phase 2
     long long* part = reinterpret_cast<long long*>(smem + crt_region0_words(KS, LMAX));
     double* fr = reinterpret_cast<double*>(part + static_cast<size_t>(4 * ST) * 64);
-    uint32_t* qs = reinterpret_cast<uint32_t*>(fr + 4 * 64);   // q, zero above L (LMAX + 2 words)
+    double* fq = fr + 4 * 64;                                  // partial sums of y M_j / q
+    uint32_t* qs = reinterpret_cast<uint32_t*>(fq + 4 * 64);   // q, zero above L (LMAX + 2 words)
     uint32_t* corr = qs + LMAX + 2;                            // offset correction (LMAX + 2 words)
     for (int i = threadIdx.x; i < LMAX + 2; i += blockDim.x) {
         qs[i] = i < L ? P.q[i] : 0u;
@@ -192,7 +193,7 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
     {
         const uint32_t col = nb + min(static_cast<uint32_t>(lane), nk - 1);
         const bool lane_in = static_cast<uint32_t>(lane) < nk;
-        double f = 0;
+        double f = 0, f2 = 0;
         // rows j = wave + 4 i, eight loads in flight per batch
         for (int i0 = 0; i0 < 4 * kCrtMaxKSteps; i0 += 8) {
             if (wave + 4 * i0 >= KR) break;
@@ -209,10 +210,12 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
                     const uint32_t yv = (j < K && lane_in) ? y[u] : 0u;
                     ydig[j * kYRow + lane] = yv ^ 0x80808080u;
                     f = fma(static_cast<double>(yv), P.p_recip[min(j, K - 1)], f);
+                    f2 = fma(static_cast<double>(yv), P.mq[min(j, K - 1)], f2);
                 }
             }
         }
         fr[wave * 64 + lane] = f;
+        fq[wave * 64 + lane] = f2;
     }
     __syncthreads();
 
@@ -255,9 +258,12 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
     __syncthreads();
     if (wave != 0) return;   // no barrier follows
 
-    // 3. one coefficient per lane.
+    // 3. one coefficient per lane: T = sum_j y_j M_j + negvP_v minus Q q, with the
+    // quotient Q estimated in FP64, all in one carry chain; then one correction.
     const double frac = fr[lane] + fr[64 + lane] + fr[128 + lane] + fr[192 + lane];
     const int v = crt_round(frac, K);
+    const uint64_t Q = quotient_estimate(fq[lane] + fq[64 + lane] + fq[128 + lane] + fq[192 + lane] + P.nq[v]);
+    const uint32_t Ql = static_cast<uint32_t>(Q), Qh = static_cast<uint32_t>(Q >> 32);
     const uint32_t* nv = P.negvP + static_cast<size_t>(v) * L;
     uint32_t x[LMAX + 2];
     // -vP mod q first: all of its loads in flight together (clamped, not branched)
@@ -265,14 +271,16 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
     for (int lam = 0; lam < LMAX; ++lam) x[lam] = nv[min(lam, L - 1)];
     x[LMAX] = x[LMAX + 1] = 0;
     long long carry = 0;
+    uint64_t a1 = 0, b1 = 0, b2 = 0;
 #pragma unroll
     for (int lam = 0; lam < LMAX + 2; ++lam) {
-        long long s = carry + corr[lam] + (lam < L ? x[lam] : 0u);
+        const int64_t qpiece = q_times_limb(Ql, Qh, qs[lam], a1, b1, b2);
+        long long s = carry + corr[lam] + (lam < L ? x[lam] : 0u) - qpiece;
         if (lam < 4 * ST) s += part[lam * 64 + lane];
         x[lam] = static_cast<uint32_t>(s);
-        carry = s >> 32;   // arithmetic: partials may be negative, the total is not
+        carry = s >> 32;   // arithmetic: partials may be negative
     }
-    reduce_mod_q<LMAX>(x, P, qs);
+    fix_range<LMAX>(x, qs, L);
 #pragma unroll
     for (int l = 0; l < LMAX; ++l)
         if (l < L) outt[lane * (LMAX + 1) + l] = x[l];

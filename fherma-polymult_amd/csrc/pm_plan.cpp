@@ -223,6 +223,16 @@ HostPlan make_plan(uint32_t N, uint32_t W, const uint32_t* q, int L) {
         sub_q(neg.data(), L, t.data(), L);
         std::copy(neg.begin(), neg.end(), pl.negvP.begin() + static_cast<size_t>(v) * L);
     }
+    // x / q in double for x < q: the top three limbs of both, aligned on q's top.
+    auto ratio = [&](const uint32_t* x) {
+        long double xd = 0;
+        for (int i = L - 1; i >= pl.q_drop; --i) xd = xd * 4294967296.0L + x[i];
+        return static_cast<double>(xd / static_cast<long double>(pl.q_top));
+    };
+    pl.mq.resize(K);
+    for (int j = 0; j < K; ++j) pl.mq[j] = ratio(pl.M.data() + static_cast<size_t>(j) * L);
+    pl.nq.resize(K + 1);
+    for (int v = 0; v <= K; ++v) pl.nq[v] = ratio(pl.negvP.data() + static_cast<size_t>(v) * L);
     return pl;
 }
 

@@ -33,6 +33,8 @@ struct HostPlan {
     std::vector<double> p_recip;      // K: 1 / p_j
     std::vector<uint32_t> M;          // K x L: P / p_j mod q
     std::vector<uint32_t> negvP;      // (K + 1) x L: -v P mod q
+    std::vector<double> mq;           // K: M_j / q
+    std::vector<double> nq;           // K + 1: negvP_v / q
 
     // Geometry of the two-pass NTT: N = R x S, rows of S contiguous elements.
     uint32_t logS = 0, logR = 0;
