@@ -92,11 +92,43 @@ k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
     const uint32_t nb = k0 + blockIdx.x * (16 * kTiles);
     const uint32_t nk = min(static_cast<uint32_t>(16 * kTiles), k0 + count - nb);
     const uint32_t* src = limbs + static_cast<size_t>(nb) * L;
-    for (uint32_t i = threadIdx.x; i < 16u * kTiles * Ls; i += blockDim.x) {
-        const uint32_t r = i / Ls, w = i - r * Ls;
-        const bool in = r < nk && w < static_cast<uint32_t>(L);
-        const uint32_t v = src[in ? r * L + w : 0];
-        xs[i] = (in ? v : 0u) ^ 0x80808080u;
+    constexpr uint32_t kSlots = 16u * kTiles * Ls;
+    if (nk == 16u * kTiles) {
+        // A full tile: its 64 L words are contiguous and 16-byte aligned (nb is a
+        // multiple of 64). At most KT 16-byte loads per thread, all in flight.
+        const uint32_t words = 16u * kTiles * static_cast<uint32_t>(L);
+        const uint4* src4 = reinterpret_cast<const uint4*>(src);
+        uint4 v[KT];
+#pragma unroll
+        for (int r = 0; r < KT; ++r) {
+            const uint32_t i4 = threadIdx.x + r * blockDim.x;
+            v[r] = src4[min(i4, words / 4 - 1)];
+        }
+#pragma unroll
+        for (int r = 0; r < KT; ++r) {
+            const uint32_t i4 = threadIdx.x + r * blockDim.x;
+            if (4 * i4 < words) {
+                const uint32_t w4[4] = {v[r].x, v[r].y, v[r].z, v[r].w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const uint32_t i = 4 * i4 + e, row = i / L;
+                    xs[row * Ls + (i - row * L)] = w4[e] ^ 0x80808080u;
+                }
+            }
+        }
+        // digits of the padding words past L: byte 0
+        const uint32_t pad = Ls - L;
+        for (uint32_t i = threadIdx.x; i < 16u * kTiles * pad; i += blockDim.x) {
+            const uint32_t row = i / pad;
+            xs[row * Ls + L + (i - row * pad)] = 0x80808080u;
+        }
+    } else {
+        for (uint32_t i = threadIdx.x; i < kSlots; i += blockDim.x) {
+            const uint32_t r = i / Ls, w = i - r * Ls;
+            const bool in = r < nk && w < static_cast<uint32_t>(L);
+            const uint32_t v = src[in ? r * L + w : 0];
+            xs[i] = (in ? v : 0u) ^ 0x80808080u;
+        }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -194,24 +226,19 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
         const uint32_t col = nb + min(static_cast<uint32_t>(lane), nk - 1);
         const bool lane_in = static_cast<uint32_t>(lane) < nk;
         double f = 0, f2 = 0;
-        // rows j = wave + 4 i, eight loads in flight per batch
-        for (int i0 = 0; i0 < 4 * kCrtMaxKSteps; i0 += 8) {
-            if (wave + 4 * i0 >= KR) break;
-            uint32_t y[8];
+        // rows j = wave + 4 i: every load of the thread in flight at once
+        constexpr int kRows = 4 * kCrtMaxKSteps;   // per wave, for KR <= 160
+        uint32_t y[kRows];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int j = min(wave + 4 * (i0 + u), K - 1);
-                y[u] = Y[static_cast<size_t>(j) * P.N + col];
-            }
+        for (int i = 0; i < kRows; ++i) y[i] = Y[static_cast<size_t>(min(wave + 4 * i, K - 1)) * P.N + col];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int j = wave + 4 * (i0 + u);
-                if (j < KR) {
-                    const uint32_t yv = (j < K && lane_in) ? y[u] : 0u;
-                    ydig[j * kYRow + lane] = yv ^ 0x80808080u;
-                    f = fma(static_cast<double>(yv), P.p_recip[min(j, K - 1)], f);
-                    f2 = fma(static_cast<double>(yv), P.mq[min(j, K - 1)], f2);
-                }
+        for (int i = 0; i < kRows; ++i) {
+            const int j = wave + 4 * i;
+            if (j < KR) {
+                const uint32_t yv = (j < K && lane_in) ? y[i] : 0u;
+                ydig[j * kYRow + lane] = yv ^ 0x80808080u;
+                f = fma(static_cast<double>(yv), P.p_recip[min(j, K - 1)], f);
+                f2 = fma(static_cast<double>(yv), P.mq[min(j, K - 1)], f2);
             }
         }
         fr[wave * 64 + lane] = f;
@@ -258,32 +285,46 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
     __syncthreads();
     if (wave != 0) return;   // no barrier follows
 
-    // 3. one coefficient per lane: T = sum_j y_j M_j + negvP_v minus Q q, with the
-    // quotient Q estimated in FP64, all in one carry chain; then one correction.
+    // 3. one coefficient per lane: R = sum_j y_j M_j + negvP_v - Q q, with the
+    // quotient Q estimated in FP64 (exact or one off, so R is in [-q, 2q)), in
+    // one carry chain over the lane's LDS row, which first receives -vP mod q.
+    // The borrow of R - q is tracked alongside; one pass then adds q (R < 0),
+    // subtracts q (R >= q) or keeps R.
     const double frac = fr[lane] + fr[64 + lane] + fr[128 + lane] + fr[192 + lane];
     const int v = crt_round(frac, K);
     const uint64_t Q = quotient_estimate(fq[lane] + fq[64 + lane] + fq[128 + lane] + fq[192 + lane] + P.nq[v]);
     const uint32_t Ql = static_cast<uint32_t>(Q), Qh = static_cast<uint32_t>(Q >> 32);
     const uint32_t* nv = P.negvP + static_cast<size_t>(v) * L;
-    uint32_t x[LMAX + 2];
-    // -vP mod q first: all of its loads in flight together (clamped, not branched)
+    uint32_t* row = outt + lane * (LMAX + 1);
+    for (int l0 = 0; l0 < L; l0 += 16) {
+        uint32_t t[16];
 #pragma unroll
-    for (int lam = 0; lam < LMAX; ++lam) x[lam] = nv[min(lam, L - 1)];
-    x[LMAX] = x[LMAX + 1] = 0;
-    long long carry = 0;
-    uint64_t a1 = 0, b1 = 0, b2 = 0;
+        for (int u = 0; u < 16; ++u) t[u] = nv[min(l0 + u, L - 1)];
 #pragma unroll
-    for (int lam = 0; lam < LMAX + 2; ++lam) {
-        const int64_t qpiece = q_times_limb(Ql, Qh, qs[lam], a1, b1, b2);
-        long long s = carry + corr[lam] + (lam < L ? x[lam] : 0u) - qpiece;
-        if (lam < 4 * ST) s += part[lam * 64 + lane];
-        x[lam] = static_cast<uint32_t>(s);
-        carry = s >> 32;   // arithmetic: partials may be negative
+        for (int u = 0; u < 16; ++u)
+            if (l0 + u < L) row[l0 + u] = t[u];
     }
-    fix_range<LMAX>(x, qs, L);
-#pragma unroll
-    for (int l = 0; l < LMAX; ++l)
-        if (l < L) outt[lane * (LMAX + 1) + l] = x[l];
+    long long carry = 0;
+    int64_t bq = 0;
+    uint64_t a1 = 0, b1 = 0, b2 = 0;
+    for (int lam = 0; lam < L + 2; ++lam) {
+        const uint32_t qv = qs[lam];
+        long long s = carry + corr[lam] - q_times_limb(Ql, Qh, qv, a1, b1, b2);
+        if (lam < L) s += row[lam];
+        if (lam < 4 * ST) s += part[lam * 64 + lane];
+        const uint32_t r = static_cast<uint32_t>(s);
+        carry = s >> 32;   // arithmetic: partials may be negative
+        if (lam < L) row[lam] = r;
+        bq = (static_cast<int64_t>(r) - qv + bq) >> 32;
+    }
+    // |R| < 2^(32 L + 33): the chain's final carry is R's sign.
+    const int64_t sign = carry < 0 ? 1 : (bq >= 0 ? -1 : 0);
+    int64_t c2 = 0;
+    for (int lam = 0; lam < L; ++lam) {
+        const int64_t s = static_cast<int64_t>(row[lam]) + sign * static_cast<int64_t>(qs[lam]) + c2;
+        row[lam] = static_cast<uint32_t>(s);
+        c2 = s >> 32;
+    }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     uint32_t* dst = c + static_cast<size_t>(nb) * L;
