@@ -50,6 +50,8 @@ struct DevPlan {
     const uint32_t* dec_corr; // K: 128 sum_t (256^t mod p_j) mod p_j
     const int8_t* crt_digits; // [crt_stiles][crt_ksteps][64 lanes][16]: MFMA A fragments
     const uint32_t* crt_corr; // L + 2 limbs: 128 (2^32 - 1)/255 sum_j M_j, exact
+    long long* crt_part;      // workspace: 4 crt_stiles x N signed limb partials
+    double2* crt_fq;          // workspace: N x (sum y_j / p_j, sum y_j M_j / q)
     int dec_ksteps;           // ceil(4L / 64)
     int crt_ksteps;           // ceil(4K / 64)
     int crt_stiles;           // ceil((4L + 4) / 16)

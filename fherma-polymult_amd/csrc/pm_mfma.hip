@@ -179,49 +179,32 @@ k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
     }
 }
 
-// CRT, one workgroup per 64 coefficients.
+// CRT in two kernels.
+//
+// k_crt_mfma_partials, one workgroup per 64 coefficients:
 //   1. all threads: y[j][nb .. nb + 64) for every prime into LDS as int8 digits
-//      (rows of 68 words, rows past K zero), and partial sums of y_j / p_j;
+//      (rows of 68 words, rows past K zero), with the partial sums of y_j / p_j
+//      and of y_j M_j / q, which wave 0 folds and writes out per coefficient;
 //   2. the four waves split the limb tiles st: MFMA over all (prime, byte)
-//      K-steps, limb partials (signed 64-bit) into LDS;
-//   3. wave 0, one coefficient per lane: v, the carry chain with the offset
-//      correction and -vP, the reduction mod q, and a coalesced store via LDS.
+//      K-steps; each lane folds its four rows (the bytes of one 32-bit limb)
+//      into a signed 64-bit limb partial, stored limb-major [limb][n].
+// k_crt_finish, one thread per coefficient: v and the quotient estimate Q, then
+//   R = sum of partials + offset correction + negvP_v - Q q in one carry chain
+//   (R in [-q, 2q) since Q is exact or one off), then one correction by +-q.
 constexpr int kYRow = 68;   // 64 coefficients + 4: lane groups g land on alternate bank halves
 
-// LDS words: y digits (phases 1-2), reused for the output transpose (phase 3);
-// limb partials (int64); partial sums of y / p (double).
-__host__ __device__ constexpr size_t crt_region0_words(int KS, int LMAX) {
-    return static_cast<size_t>(16 * KS) * kYRow > static_cast<size_t>(64) * (LMAX + 1)
-               ? static_cast<size_t>(16 * KS) * kYRow
-               : static_cast<size_t>(64) * (LMAX + 1);
-}
-
-__host__ __device__ constexpr size_t crt_lds_words(int KS, int ST, int LMAX) {
-    return crt_region0_words(KS, LMAX) + static_cast<size_t>(4 * ST) * 64 * 2 + 2 * 4 * 64 * 2 + 2 * (LMAX + 2);
-}
-
-template <int LMAX>
 __global__ void __launch_bounds__(64 * kWaves)
-k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, uint32_t k0, uint32_t count) {
+k_crt_mfma_partials(const uint32_t* __restrict__ Y, long long* __restrict__ part, double2* __restrict__ fq_out,
+                    DevPlan P, uint32_t k0, uint32_t count) {
     extern __shared__ uint32_t smem[];
-    const int K = P.K, L = P.L, KS = P.crt_ksteps, ST = P.crt_stiles;
+    const int K = P.K, KS = P.crt_ksteps, ST = P.crt_stiles;
     const int KR = 16 * KS;
     uint32_t* ydig = smem;
-    uint32_t* outt = smem;   // after phase 2
-    long long* part = reinterpret_cast<long long*>(smem + crt_region0_words(KS, LMAX));
-    double* fr = reinterpret_cast<double*>(part + static_cast<size_t>(4 * ST) * 64);
-    double* fq = fr + 4 * 64;                                  // partial sums of y M_j / q
-    uint32_t* qs = reinterpret_cast<uint32_t*>(fq + 4 * 64);   // q, zero above L (LMAX + 2 words)
-    uint32_t* corr = qs + LMAX + 2;                            // offset correction (LMAX + 2 words)
-    for (int i = threadIdx.x; i < LMAX + 2; i += blockDim.x) {
-        qs[i] = i < L ? P.q[i] : 0u;
-        corr[i] = i < L + 2 ? P.crt_corr[i] : 0u;
-    }
+    double* fr = reinterpret_cast<double*>(smem + static_cast<size_t>(KR) * kYRow);   // 4 x 64
+    double* fq = fr + 4 * 64;                                                          // 4 x 64
     const uint32_t nb = k0 + blockIdx.x * 64;
     const uint32_t nk = min(64u, k0 + count - nb);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-
-    // 1. y digits and the partial rounding sums; loads are clamped, not branched.
     {
         const uint32_t col = nb + min(static_cast<uint32_t>(lane), nk - 1);
         const bool lane_in = static_cast<uint32_t>(lane) < nk;
@@ -245,92 +228,96 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
         fq[wave * 64 + lane] = f2;
     }
     __syncthreads();
-
-    // 2. limb partials. A lane's rows 16 st + 4 g + r are the bytes of limb 4 st + g.
-    {
-        const int g = lane >> 4, cl = lane & 15;
-        const v4i* dig = reinterpret_cast<const v4i*>(P.crt_digits);
-        for (int st = wave; st < ST; st += kWaves) {
-            v4i acc[kTiles];
+    if (wave == 0 && static_cast<uint32_t>(lane) < nk) {
+        fq_out[nb + lane] = make_double2(fr[lane] + fr[64 + lane] + fr[128 + lane] + fr[192 + lane],
+                                         fq[lane] + fq[64 + lane] + fq[128 + lane] + fq[192 + lane]);
+    }
+    const int g = lane >> 4, cl = lane & 15;
+    const v4i* dig = reinterpret_cast<const v4i*>(P.crt_digits);
+    for (int st = wave; st < ST; st += kWaves) {
+        v4i acc[kTiles];
 #pragma unroll
-            for (int t = 0; t < kTiles; ++t) acc[t] = v4i{0, 0, 0, 0};
-            // every A fragment of this limb tile at once (clamped, not branched)
-            v4i af[kCrtMaxKSteps];
+        for (int t = 0; t < kTiles; ++t) acc[t] = v4i{0, 0, 0, 0};
+        // every A fragment of this limb tile at once (clamped, not branched)
+        v4i af[kCrtMaxKSteps];
 #pragma unroll
-            for (int ks = 0; ks < kCrtMaxKSteps; ++ks) af[ks] = dig[(st * KS + min(ks, KS - 1)) * 64 + lane];
+        for (int ks = 0; ks < kCrtMaxKSteps; ++ks) af[ks] = dig[(st * KS + min(ks, KS - 1)) * 64 + lane];
 #pragma unroll
-            for (int ks = 0; ks < kCrtMaxKSteps; ++ks) {
-                if (ks >= KS) continue;
-                const v4i a = af[ks];
-                const uint32_t* yr = ydig + (16 * ks + 4 * g) * kYRow + cl;
-#pragma unroll
-                for (int t = 0; t < kTiles; ++t) {
-                    v4i b;
-                    b[0] = static_cast<int>(yr[16 * t]);
-                    b[1] = static_cast<int>(yr[kYRow + 16 * t]);
-                    b[2] = static_cast<int>(yr[2 * kYRow + 16 * t]);
-                    b[3] = static_cast<int>(yr[3 * kYRow + 16 * t]);
-                    acc[t] = mfma16(a, b, acc[t]);
-                }
-            }
+        for (int ks = 0; ks < kCrtMaxKSteps; ++ks) {
+            if (ks >= KS) continue;
+            const uint32_t* yr = ydig + (16 * ks + 4 * g) * kYRow + cl;
 #pragma unroll
             for (int t = 0; t < kTiles; ++t) {
-                part[(4 * st + g) * 64 + 16 * t + cl] = static_cast<long long>(acc[t][0]) +
-                                                        static_cast<long long>(acc[t][1]) * 256 +
-                                                        static_cast<long long>(acc[t][2]) * 65536 +
-                                                        static_cast<long long>(acc[t][3]) * 16777216;
+                v4i b;
+                b[0] = static_cast<int>(yr[16 * t]);
+                b[1] = static_cast<int>(yr[kYRow + 16 * t]);
+                b[2] = static_cast<int>(yr[2 * kYRow + 16 * t]);
+                b[3] = static_cast<int>(yr[3 * kYRow + 16 * t]);
+                acc[t] = mfma16(af[ks], b, acc[t]);
+            }
+        }
+        // rows 16 st + 4 g + r, r = 0..3: the bytes of limb 4 st + g
+        long long* out = part + static_cast<size_t>(4 * st + g) * P.N + nb + cl;
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) {
+            if (static_cast<uint32_t>(16 * t + cl) < nk) {
+                out[16 * t] = static_cast<long long>(acc[t][0]) + static_cast<long long>(acc[t][1]) * 256 +
+                              static_cast<long long>(acc[t][2]) * 65536 +
+                              static_cast<long long>(acc[t][3]) * 16777216;
             }
         }
     }
-    __syncthreads();
-    if (wave != 0) return;   // no barrier follows
+}
 
-    // 3. one coefficient per lane: R = sum_j y_j M_j + negvP_v - Q q, with the
-    // quotient Q estimated in FP64 (exact or one off, so R is in [-q, 2q)), in
-    // one carry chain over the lane's LDS row, which first receives -vP mod q.
-    // The borrow of R - q is tracked alongside; one pass then adds q (R < 0),
-    // subtracts q (R >= q) or keeps R.
-    const double frac = fr[lane] + fr[64 + lane] + fr[128 + lane] + fr[192 + lane];
-    const int v = crt_round(frac, K);
-    const uint64_t Q = quotient_estimate(fq[lane] + fq[64 + lane] + fq[128 + lane] + fq[192 + lane] + P.nq[v]);
+template <int LMAX>
+__global__ void __launch_bounds__(256)
+k_crt_finish(const long long* __restrict__ part, const double2* __restrict__ fq_in, uint32_t* __restrict__ c,
+             DevPlan P, uint32_t k0, uint32_t count) {
+    const uint32_t n = k0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= k0 + count) return;
+    const int K = P.K, L = P.L, NL = 4 * P.crt_stiles;   // limbs with partials
+    const double2 fv = fq_in[n];
+    const int v = crt_round(fv.x, K);
+    const uint64_t Q = quotient_estimate(fv.y + P.nq[v]);
     const uint32_t Ql = static_cast<uint32_t>(Q), Qh = static_cast<uint32_t>(Q >> 32);
     const uint32_t* nv = P.negvP + static_cast<size_t>(v) * L;
-    uint32_t* row = outt + lane * (LMAX + 1);
-    for (int l0 = 0; l0 < L; l0 += 16) {
-        uint32_t t[16];
+    uint32_t x[LMAX + 2];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) t[u] = nv[min(l0 + u, L - 1)];
-#pragma unroll
-        for (int u = 0; u < 16; ++u)
-            if (l0 + u < L) row[l0 + u] = t[u];
-    }
+    for (int l = 0; l < LMAX; ++l) x[l] = nv[min(l, L - 1)];   // all in flight, clamped
     long long carry = 0;
-    int64_t bq = 0;
     uint64_t a1 = 0, b1 = 0, b2 = 0;
-    for (int lam = 0; lam < L + 2; ++lam) {
-        const uint32_t qv = qs[lam];
-        long long s = carry + corr[lam] - q_times_limb(Ql, Qh, qv, a1, b1, b2);
-        if (lam < L) s += row[lam];
-        if (lam < 4 * ST) s += part[lam * 64 + lane];
-        const uint32_t r = static_cast<uint32_t>(s);
-        carry = s >> 32;   // arithmetic: partials may be negative
-        if (lam < L) row[lam] = r;
-        bq = (static_cast<int64_t>(r) - qv + bq) >> 32;
+    const long long* pp = part + n;
+#pragma unroll
+    for (int l0 = 0; l0 < LMAX + 2; l0 += 8) {
+        long long pv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) pv[u] = pp[static_cast<size_t>(min(l0 + u, NL - 1)) * P.N];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int l = l0 + u;
+            if (l < LMAX + 2) {
+                const uint32_t qv = l < L ? P.q[l] : 0u;
+                long long s = carry + (l < L + 2 ? static_cast<long long>(P.crt_corr[l]) : 0) -
+                              q_times_limb(Ql, Qh, qv, a1, b1, b2);
+                if (l < L) s += x[l];
+                if (l < NL) s += pv[u];
+                x[l] = static_cast<uint32_t>(s);
+                carry = s >> 32;   // arithmetic: partials may be negative
+            }
+        }
     }
-    // |R| < 2^(32 L + 33): the chain's final carry is R's sign.
-    const int64_t sign = carry < 0 ? 1 : (bq >= 0 ? -1 : 0);
-    int64_t c2 = 0;
-    for (int lam = 0; lam < L; ++lam) {
-        const int64_t s = static_cast<int64_t>(row[lam]) + sign * static_cast<int64_t>(qs[lam]) + c2;
-        row[lam] = static_cast<uint32_t>(s);
-        c2 = s >> 32;
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    uint32_t* dst = c + static_cast<size_t>(nb) * L;
-    for (uint32_t i = lane; i < nk * static_cast<uint32_t>(L); i += 64) {
-        const uint32_t r = i / L;
-        dst[i] = outt[r * (LMAX + 1) + (i - r * L)];
+    fix_range<LMAX>(x, P.q, L);
+    uint32_t* out = c + static_cast<size_t>(n) * L;
+    if ((L & 3) == 0 && (reinterpret_cast<uintptr_t>(c) & 15) == 0) {
+        // 16-byte stores: a quarter of the cache-line accesses of word stores
+        uint4* o4 = reinterpret_cast<uint4*>(out);
+#pragma unroll
+        for (int l = 0; l < LMAX; l += 4)
+            if (l < L) o4[l / 4] = make_uint4(x[l], x[l + 1], x[l + 2], x[l + 3]);
+    } else {
+#pragma unroll
+        for (int l = 0; l < LMAX; ++l)
+            if (l < L) out[l] = x[l];
     }
 }
 
@@ -446,14 +433,17 @@ void launch_crt_mfma(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream
 
 void launch_crt_mfma_range(const DevPlan& P, const uint32_t* Y, uint32_t* c, uint32_t k0, uint32_t count,
                            hipStream_t st) {
-    if (!P.crt_digits) throw std::runtime_error("matrix-core tables not prepared");
+    if (!P.crt_digits || !P.crt_part) throw std::runtime_error("matrix-core tables not prepared");
     if (count == 0) return;
     const dim3 grid((count + 63) / 64);
+    const size_t lds = sizeof(uint32_t) * (static_cast<size_t>(16 * P.crt_ksteps) * kYRow + 4 * 64 * 2 * 2);
+    hipLaunchKernelGGL(k_crt_mfma_partials, grid, dim3(64 * kWaves), lds, st, Y, P.crt_part, P.crt_fq, P, k0,
+                       count);
+    const dim3 grid2((count + 255) / 256);
     const int lmax = (P.L + 7) / 8 * 8;
-    const size_t lds = sizeof(uint32_t) * crt_lds_words(P.crt_ksteps, P.crt_stiles, lmax);
     switch (lmax) {
 #define PM_CRT_CASE(LM) \
-    case LM: hipLaunchKernelGGL(k_crt_mfma<LM>, grid, dim3(64 * kWaves), lds, st, Y, c, P, k0, count); break;
+    case LM: hipLaunchKernelGGL(k_crt_finish<LM>, grid2, dim3(256), 0, st, P.crt_part, P.crt_fq, c, P, k0, count); break;
         PM_CRT_CASE(8)
         PM_CRT_CASE(16)
         PM_CRT_CASE(24)
