@@ -90,11 +90,34 @@ def golden_digests():
     return {c["seed"]: c["c_sha"] for c in cases if c["name"].startswith("config-n65536-w2048-")}
 
 
-# --- algorithmic bytes per launch of each phase (K x N residues, N x L limbs)
+# --- bytes each kernel must read and write per launch (K x N residues, N x L limbs)
 def phase_bytes(K, L):
     kn, nl = 4 * K * N, 4 * N * L
     return {"decompose_a": nl + kn, "decompose_b": nl + kn, "ntt_cols_fwd_a": 2 * kn, "ntt_cols_fwd_b": 2 * kn,
             "ntt_rows_fused": 3 * kn, "ntt_cols_inv": 2 * kn, "crt": kn + nl}
+
+
+# Kernel names (as rocprofv3 reports them) of each phase, for the traffic table.
+PHASE_KERNELS = {"decompose_a": "k_decompose_mfma", "decompose_b": "k_decompose_mfma",
+                 "ntt_cols_fwd_a": "k_ntt_col_reg", "ntt_cols_fwd_b": "k_ntt_col_reg",
+                 "ntt_rows_fused": "k_ntt_rows_fused_reg", "ntt_cols_inv": "k_ntt_col_reg",
+                 "crt": "k_crt_mfma"}
+
+
+def measured_traffic(phase):
+    """HBM bytes per launch of a phase's kernel from the committed counter table
+    (profiles/hbm_traffic_n65536_w2048.txt: kernel, scaled FETCH_SIZE + WRITE_SIZE
+    bytes per dispatch), or None when the table or the kernel is absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_n65536_w2048.txt")
+    if not os.path.exists(path):
+        return None
+    want = PHASE_KERNELS.get(phase)
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 2 and parts[0] == want and parts[1].isdigit():
+                return int(parts[1])
+    return None
 
 
 def cpu_baseline(points, budget_s=12.0):
@@ -129,6 +152,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-yardstick", action="store_true",
+                    help="first run three 256 MiB device copies (the known-size kernel that scales the "
+                         "HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -139,6 +165,14 @@ def main():
 
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
+
+    if args.copy_yardstick:
+        src, dst = polymult.DeviceBuffer(256 << 20), polymult.DeviceBuffer(256 << 20)
+        for _ in range(3):
+            polymult.device_copy(dst, src)
+        polymult.synchronize()
+        src.close()
+        dst.close()
 
     log("generating %d points" % BATCH)
     points = [make_point(SEED0 + i, N, W) for i in range(BATCH)]
@@ -202,8 +236,9 @@ def main():
     dom = max(phases, key=phases.get)
     achieved = nbytes[dom] / (phases[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "kernel_ms": round(phases[dom], 5), "algorithmic_bytes": nbytes[dom]}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom), "kernel": dom,
+                "kernel_name": PHASE_KERNELS[dom], "kernel_ms": round(phases[dom], 5),
+                "algorithmic_bytes": nbytes[dom]}
     prod_bytes = 12 * N * L + 56 * K * N
     cpu = None if args.no_cpu else cpu_baseline(points[:1])
     name, cus = polymult.device_info()

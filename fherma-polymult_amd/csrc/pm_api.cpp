@@ -197,8 +197,8 @@ const char* const kPhaseNames[PM_NPHASES] = {"decompose_a", "decompose_b", "ntt_
 
 namespace pm {
 
-// The five phases of one product on device pointers, enqueued on st. When ev
-// is given, ev[i] is recorded before phase i and ev[PM_NPHASES] after the last.
+// The phases of one product on device pointers, enqueued on st. When ev is
+// given, ev[i] is recorded before phase i and ev[PM_NPHASES] after the last.
 void run_device(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st, hipEvent_t* ev) {
     const pm::DevPlan& P = pl->dev;
     const bool mfma = pl->form == PM_FORM_MFMA;
@@ -494,6 +494,17 @@ int pm_copy_to_device(void* dst, const void* src, size_t bytes) {
 
 int pm_copy_to_host(void* dst, const void* src, size_t bytes) {
     return guarded([&] { check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H"); });
+}
+
+int pm_device_copy(void* dst, const void* src, size_t bytes, void* stream) {
+    return guarded([&] {
+        need(dst && src, "null pointer");
+        need(bytes % 16 == 0 && reinterpret_cast<uintptr_t>(dst) % 16 == 0 &&
+                 reinterpret_cast<uintptr_t>(src) % 16 == 0,
+             "pm_device_copy needs 16-byte aligned pointers and sizes");
+        pm::launch_copy16(dst, src, bytes, static_cast<hipStream_t>(stream));
+        check(hipGetLastError(), "kernel launch");
+    });
 }
 
 int pm_stream_create(void** out) {

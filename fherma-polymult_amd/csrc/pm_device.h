@@ -48,14 +48,15 @@ struct DevPlan {
     // int8 matrix-core forms (pm_mfma.hip); null until prepared.
     const int8_t* dec_digits; // [ceil(K/4)][dec_ksteps][64 lanes][16]: MFMA A fragments
     const uint32_t* dec_corr; // K: 128 sum_t (256^t mod p_j) mod p_j
-    const int8_t* crt_digits; // [crt_stiles][crt_ksteps][64 lanes][16]: MFMA A fragments
+    // CRT digit table: row s (0..4L) holds byte s of the balanced digits of
+    // M_j for primes j = 4J .. 4J + 3 in word J (zero for j >= K); rows of
+    // crt_kjs words, crt_kjs = 2 (mod 4) and >= ceil(K / 4).
+    const uint32_t* crt_tab;
     const uint32_t* crt_corr; // L + 2 limbs: 128 (2^32 - 1)/255 sum_j M_j, exact
-    long long* crt_part;      // workspace: 4 crt_stiles x N signed limb partials
-    double2* crt_fq;          // workspace: N x (sum y_j / p_j, sum y_j M_j / q)
     int dec_ksteps;           // ceil(4L / 64)
     int crt_ksteps;           // ceil(4K / 64)
     int crt_stiles;           // ceil((4L + 4) / 16)
-    int pad2;
+    int crt_kjs;
 };
 
 // Plain form (VALU): every launcher enqueues on `st` and returns.
@@ -79,6 +80,10 @@ void launch_ntt_rows_fused_generic(const DevPlan& P, uint32_t* A, const uint32_t
 
 // Matrix-core form (int8 MFMA, exact).
 void launch_decompose_mfma(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st);
+// mid, when given, is recorded between the partials and the reduction kernels.
+// Plain grid-stride copy of `bytes` (a multiple of 16) with 16-byte accesses.
+void launch_copy16(void* dst, const void* src, size_t bytes, hipStream_t st);
+
 void launch_crt_mfma(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream_t st);
 // The same over coefficients [k0, k0 + count) only (chunked host pipeline).
 void launch_decompose_mfma_range(const DevPlan& P, const uint32_t* limbs, uint32_t* res, uint32_t k0,

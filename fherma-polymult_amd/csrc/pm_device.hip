@@ -19,6 +19,8 @@
 #include "pm_arith.h"
 #include "pm_bigint.h"
 
+#include <algorithm>
+
 namespace pm {
 namespace {
 
@@ -268,6 +270,19 @@ TileGeom row_geom(const DevPlan& P, uint32_t max_log_tile) {
 }
 
 }  // namespace
+
+__global__ void __launch_bounds__(256) k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+void launch_copy16(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    const size_t n = bytes / 16;
+    if (n == 0) return;
+    const unsigned blocks = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, st, static_cast<uint4*>(dst),
+                       static_cast<const uint4*>(src), n);
+}
 
 void launch_decompose_valu(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st) {
     launch_decompose_valu_range(P, limbs, res, 0, P.N, st);

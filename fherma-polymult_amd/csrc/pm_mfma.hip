@@ -27,9 +27,10 @@
 //     sum_j y_j M_j = sum_s 256^s G[s][n] + 0x80808080 sum_j M_j   (exactly)
 //   MFMA rows are byte positions s, columns coefficients, K-dim (prime, byte).
 //   A lane's four rows are the four bytes of one 32-bit limb, so it folds them
-//   into one signed 64-bit limb partial; the carry chain, + negvP_v and the
-//   reduction mod q run one coefficient per lane after an LDS transpose.
-//   v = round(sum_j y_j / p_j) is summed in FP64 while the y bytes are loaded.
+//   into one signed 64-bit limb partial, written out limb-major; a second kernel
+//   runs the carry chain, + negvP_v and the reduction mod q, one coefficient per
+//   thread. v = round(sum_j y_j / p_j) and the FP64 quotient estimate's sum are
+//   accumulated while the y bytes are loaded.
 //
 // MFMA operand layout (16x16x64 i8): lane l supplies row (A) / column (B)
 // l % 16 and the 16 K-elements 16 (l / 16) + [0, 16) of each 64-wide K step;
@@ -179,145 +180,344 @@ k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
     }
 }
 
-// CRT in two kernels.
+// CRT, one workgroup (four waves) per 64 coefficients, in one kernel:
+//   1. all threads: y[j][nb .. nb + 64) for every prime, byte-transposed per
+//      group of four primes (word a of group J holds byte a of y_4J .. y_4J+3),
+//      into LDS as int8 digits, with partial sums of y_j / p_j and y_j M_j / q;
+//      the M digit table is copied to LDS alongside. Wave 0 then forms
+//      v = round(sum y_j / p_j) and the quotient estimate
+//      Q = floor(sum y_j M_j / q + v (-P mod q) / q) per coefficient.
+//   2. MFMA: wave w owns the limb tiles st = w, w + 4, w + 8, w + 12 (wave 0
+//      also st = 16) for all four 16-coefficient tiles, a 4 x 4 block of
+//      accumulators, so every A and B fragment read from LDS feeds four MFMAs.
+//      Each lane folds its four rows (the bytes of one limb) into a signed
+//      64-bit limb partial; after a barrier they go to LDS as [limb][coefficient].
+//   3. R = sum_l partial_l 2^(32 l) + offset correction + v (-P mod q) - Q q is
+//      congruent to the product mod q and lies in [-q, 2q) (Q is exact or one
+//      off). The L + 2 limbs are cut into four segments, one per wave, lane =
+//      coefficient: each wave runs its segment's carry chain with carry-in 0,
+//      then the carries between segments are resolved from each segment's
+//      carry-out, low 64 bits and all-zero / all-ones flags (a carry-in can
+//      move a segment's carry-out by at most one). The sign of R and R >= q
+//      (compared segment-wise from the top) pick R + q, R - q or R; that
+//      addition is resolved across segments the same way.
+//   4. the wave's limbs pass through LDS so the stores are whole lines.
 //
-// k_crt_mfma_partials, one workgroup per 64 coefficients:
-//   1. all threads: y[j][nb .. nb + 64) for every prime into LDS as int8 digits
-//      (rows of 68 words, rows past K zero), with the partial sums of y_j / p_j
-//      and of y_j M_j / q, which wave 0 folds and writes out per coefficient;
-//   2. the four waves split the limb tiles st: MFMA over all (prime, byte)
-//      K-steps; each lane folds its four rows (the bytes of one 32-bit limb)
-//      into a signed 64-bit limb partial, stored limb-major [limb][n].
-// k_crt_finish, one thread per coefficient: v and the quotient estimate Q, then
-//   R = sum of partials + offset correction + negvP_v - Q q in one carry chain
-//   (R in [-q, 2q) since Q is exact or one off), then one correction by +-q.
-constexpr int kYRow = 68;   // 64 coefficients + 4: lane groups g land on alternate bank halves
+// K-dim order: K-step ks, lane group g, fragment word a, byte r is prime
+// 4 (4 ks + g) + r, byte a of y; the A word for row s is then word J = 4 ks + g
+// of digit-table row s - a.
+constexpr int kYRow = 68;         // 64 coefficients + 4: lane groups g land on alternate bank halves
+constexpr int kTabLoads = 11;     // 16-byte loads per thread for the digit table: (4 64 + 1) 42 words
+constexpr int kSegMax = 17;       // limbs per segment: 4 x 17 >= L + 2 for L <= 64
+
+__host__ __device__ constexpr uint32_t crt_tab_words(int L, int kjs) {
+    return (static_cast<uint32_t>(4 * L + 1) * kjs + 3u) & ~3u;
+}
+
+// LDS words: region 0 holds the y digits (phases 1-2), then the limb partials
+// (int64, 4 ST x 64) and the segment exports (1280 words), then the output
+// stage; the digit table; 1024 words of FP64 partial sums, reused for the
+// segment comparisons; 192 words of (v, Ql, Qh).
+__host__ __device__ constexpr uint32_t crt_region0_words(int KS, int ST) {
+    return static_cast<uint32_t>(16 * KS * kYRow) > static_cast<uint32_t>(512 * ST + 1280)
+               ? static_cast<uint32_t>(16 * KS * kYRow)
+               : static_cast<uint32_t>(512 * ST + 1280);
+}
+
+__host__ __device__ constexpr uint32_t crt_lds_words(int KS, int ST, int L, int kjs) {
+    return crt_region0_words(KS, ST) + crt_tab_words(L, kjs) + 1024 + 192;
+}
+
+__device__ __forceinline__ void transpose_bytes(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t (&d)[4]) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        const int sh = 8 * a;
+        d[a] = ((w0 >> sh) & 0xffu) | (((w1 >> sh) & 0xffu) << 8) | (((w2 >> sh) & 0xffu) << 16) |
+               (((w3 >> sh) & 0xffu) << 24);
+    }
+}
+
+__device__ __forceinline__ long long fold_limb(const v4i& a) {
+    return static_cast<long long>(a[0]) + static_cast<long long>(a[1]) * 256 + static_cast<long long>(a[2]) * 65536 +
+           static_cast<long long>(a[3]) * 16777216;
+}
+
+// Carry out of a segment of kSegMax limbs whose own chain (carry-in 0) ended
+// with carry-out cout, low two limbs lo and upper limbs all zero (uz) / all
+// ones (uo), when carry-in C (|C| < 2^63) is added instead:
+// cout + floor((X + C) / 2^(32 kSegMax)).
+__device__ __forceinline__ long long seg_carry(long long C, long long cout, uint64_t lo, bool uz, bool uo) {
+    long long adj = 0;
+    if (C > 0 && uo && lo + static_cast<uint64_t>(C) < lo) adj = 1;
+    if (C < 0 && uz && lo < static_cast<uint64_t>(-C)) adj = -1;
+    return cout + adj;
+}
 
 __global__ void __launch_bounds__(64 * kWaves)
-k_crt_mfma_partials(const uint32_t* __restrict__ Y, long long* __restrict__ part, double2* __restrict__ fq_out,
-                    DevPlan P, uint32_t k0, uint32_t count) {
+k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, uint32_t k0, uint32_t count) {
     extern __shared__ uint32_t smem[];
-    const int K = P.K, KS = P.crt_ksteps, ST = P.crt_stiles;
-    const int KR = 16 * KS;
-    uint32_t* ydig = smem;
-    double* fr = reinterpret_cast<double*>(smem + static_cast<size_t>(KR) * kYRow);   // 4 x 64
-    double* fq = fr + 4 * 64;                                                          // 4 x 64
+    const int K = P.K, L = P.L, KS = P.crt_ksteps, ST = P.crt_stiles, KJS = P.crt_kjs;
+    const int KJ = (K + 3) >> 2, nd = 4 * L + 1;
+    const uint32_t tab_words = crt_tab_words(L, KJS);
+    uint32_t* ydig = smem;                                                     // 16 KS rows of kYRow words
+    long long* part = reinterpret_cast<long long*>(smem);                      // after phase 2: [limb][64]
+    long long* ex64 = part + 256 * ST;                                         // segment exports
+    uint32_t* ex32 = reinterpret_cast<uint32_t*>(ex64 + 512);
+    uint32_t* stage = smem;                                                    // after phase 3: 64 x (L + 1)
+    uint32_t* tab = smem + crt_region0_words(KS, ST);                          // nd rows of KJS words
+    double* fr = reinterpret_cast<double*>(tab + tab_words);                   // 4 x 64
+    double* fq = fr + 4 * 64;                                                  // 4 x 64
+    int* cmpw = reinterpret_cast<int*>(fr);                                    // phase 3: 4 x 64
+    uint32_t* vq = reinterpret_cast<uint32_t*>(fq + 4 * 64);                   // 3 x 64: v, Ql, Qh
     const uint32_t nb = k0 + blockIdx.x * 64;
     const uint32_t nk = min(64u, k0 + count - nb);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: scalar limb indices
+
+    // 1. digit table and y words: every load of the thread in flight at once
     {
+        const uint4* tab4 = reinterpret_cast<const uint4*>(P.crt_tab);
+        const uint32_t tab4n = tab_words / 4;
+        uint4 tv[kTabLoads];
+#pragma unroll
+        for (int r = 0; r < kTabLoads; ++r) tv[r] = tab4[min(threadIdx.x + r * blockDim.x, tab4n - 1)];
         const uint32_t col = nb + min(static_cast<uint32_t>(lane), nk - 1);
         const bool lane_in = static_cast<uint32_t>(lane) < nk;
-        double f = 0, f2 = 0;
-        // rows j = wave + 4 i: every load of the thread in flight at once
-        constexpr int kRows = 4 * kCrtMaxKSteps;   // per wave, for KR <= 160
+        // prime groups J = wave + 4 i, i < KS: primes 4 J + r
+        constexpr int kRows = 4 * kCrtMaxKSteps;
         uint32_t y[kRows];
 #pragma unroll
-        for (int i = 0; i < kRows; ++i) y[i] = Y[static_cast<size_t>(min(wave + 4 * i, K - 1)) * P.N + col];
-#pragma unroll
         for (int i = 0; i < kRows; ++i) {
-            const int j = wave + 4 * i;
-            if (j < KR) {
-                const uint32_t yv = (j < K && lane_in) ? y[i] : 0u;
-                ydig[j * kYRow + lane] = yv ^ 0x80808080u;
-                f = fma(static_cast<double>(yv), P.p_recip[min(j, K - 1)], f);
-                f2 = fma(static_cast<double>(yv), P.mq[min(j, K - 1)], f2);
+            const int j = 4 * (wave + 4 * (i / 4)) + (i % 4);
+            y[i] = Y[static_cast<size_t>(min(j, K - 1)) * P.N + col];
+        }
+        uint4* tab4s = reinterpret_cast<uint4*>(tab);
+#pragma unroll
+        for (int r = 0; r < kTabLoads; ++r)
+            if (threadIdx.x + r * blockDim.x < tab4n) tab4s[threadIdx.x + r * blockDim.x] = tv[r];
+        double f = 0, f2 = 0;
+#pragma unroll
+        for (int i = 0; i < kCrtMaxKSteps; ++i) {
+            if (i >= KS) continue;
+            const int J = wave + 4 * i;
+            uint32_t w[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int j = 4 * J + r;
+                w[r] = (j < K && lane_in) ? y[4 * i + r] : 0u;
+                f = fma(static_cast<double>(w[r]), P.p_recip[min(j, K - 1)], f);
+                f2 = fma(static_cast<double>(w[r]), P.mq[min(j, K - 1)], f2);
             }
+            uint32_t d[4];
+            transpose_bytes(w[0], w[1], w[2], w[3], d);
+#pragma unroll
+            for (int a = 0; a < 4; ++a) ydig[(4 * J + a) * kYRow + lane] = d[a] ^ 0x80808080u;
         }
         fr[wave * 64 + lane] = f;
         fq[wave * 64 + lane] = f2;
     }
     __syncthreads();
-    if (wave == 0 && static_cast<uint32_t>(lane) < nk) {
-        fq_out[nb + lane] = make_double2(fr[lane] + fr[64 + lane] + fr[128 + lane] + fr[192 + lane],
-                                         fq[lane] + fq[64 + lane] + fq[128 + lane] + fq[192 + lane]);
+    if (wave == 0) {
+        const double frac = fr[lane] + fr[64 + lane] + fr[128 + lane] + fr[192 + lane];
+        const int v = crt_round(frac, K);
+        // -P mod q is negvP row 1; v (-P mod q) / q is v nq[1]
+        const uint64_t Q = quotient_estimate(fq[lane] + fq[64 + lane] + fq[128 + lane] + fq[192 + lane] +
+                                             static_cast<double>(v) * P.nq[1]);
+        vq[lane] = static_cast<uint32_t>(v);
+        vq[64 + lane] = static_cast<uint32_t>(Q);
+        vq[128 + lane] = static_cast<uint32_t>(Q >> 32);
     }
+
+    // 2. limb partials
     const int g = lane >> 4, cl = lane & 15;
-    const v4i* dig = reinterpret_cast<const v4i*>(P.crt_digits);
-    for (int st = wave; st < ST; st += kWaves) {
-        v4i acc[kTiles];
+    long long p0[4][kTiles], p1[kTiles];
+    {
+        v4i acc[4][kTiles];
 #pragma unroll
-        for (int t = 0; t < kTiles; ++t) acc[t] = v4i{0, 0, 0, 0};
-        // every A fragment of this limb tile at once (clamped, not branched)
-        v4i af[kCrtMaxKSteps];
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int ks = 0; ks < kCrtMaxKSteps; ++ks) af[ks] = dig[(st * KS + min(ks, KS - 1)) * 64 + lane];
+            for (int t = 0; t < kTiles; ++t) acc[i][t] = v4i{0, 0, 0, 0};
+        v4i acc1[kTiles];
 #pragma unroll
-        for (int ks = 0; ks < kCrtMaxKSteps; ++ks) {
-            if (ks >= KS) continue;
+        for (int t = 0; t < kTiles; ++t) acc1[t] = v4i{0, 0, 0, 0};
+        const bool has1 = wave == 0 && ST > 16;
+#pragma unroll 1
+        for (int ks = 0; ks < KS; ++ks) {
+            const int J = 4 * ks + g;
+            v4i bf[kTiles];
             const uint32_t* yr = ydig + (16 * ks + 4 * g) * kYRow + cl;
 #pragma unroll
             for (int t = 0; t < kTiles; ++t) {
-                v4i b;
-                b[0] = static_cast<int>(yr[16 * t]);
-                b[1] = static_cast<int>(yr[kYRow + 16 * t]);
-                b[2] = static_cast<int>(yr[2 * kYRow + 16 * t]);
-                b[3] = static_cast<int>(yr[3 * kYRow + 16 * t]);
-                acc[t] = mfma16(af[ks], b, acc[t]);
+                bf[t][0] = static_cast<int>(yr[16 * t]);
+                bf[t][1] = static_cast<int>(yr[kYRow + 16 * t]);
+                bf[t][2] = static_cast<int>(yr[2 * kYRow + 16 * t]);
+                bf[t][3] = static_cast<int>(yr[3 * kYRow + 16 * t]);
+            }
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                const int st = i < 4 ? wave + 4 * i : 16;
+                if (i < 4 ? st >= ST : !has1) continue;
+                const int s = 16 * st + cl;
+                v4i af;
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int row = s - a;
+                    const bool ok = J < KJ && row >= 0 && row < nd;
+                    const uint32_t v = tab[min(max(row, 0), nd - 1) * KJS + min(J, KJS - 1)];
+                    af[a] = static_cast<int>(ok ? v : 0u);
+                }
+#pragma unroll
+                for (int t = 0; t < kTiles; ++t) {
+                    if (i < 4) acc[i < 4 ? i : 0][t] = mfma16(af, bf[t], acc[i < 4 ? i : 0][t]);
+                    else acc1[t] = mfma16(af, bf[t], acc1[t]);
+                }
             }
         }
-        // rows 16 st + 4 g + r, r = 0..3: the bytes of limb 4 st + g
-        long long* out = part + static_cast<size_t>(4 * st + g) * P.N + nb + cl;
 #pragma unroll
-        for (int t = 0; t < kTiles; ++t) {
-            if (static_cast<uint32_t>(16 * t + cl) < nk) {
-                out[16 * t] = static_cast<long long>(acc[t][0]) + static_cast<long long>(acc[t][1]) * 256 +
-                              static_cast<long long>(acc[t][2]) * 65536 +
-                              static_cast<long long>(acc[t][3]) * 16777216;
-            }
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int t = 0; t < kTiles; ++t) p0[i][t] = fold_limb(acc[i][t]);
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) p1[t] = fold_limb(acc1[t]);
+    }
+    __syncthreads();   // y digits and the digit table are dead from here
+    // lane (g, cl), tile st, column tile t: limb 4 st + g of coefficient 16 t + cl
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int st = wave + 4 * i;
+        if (st < ST) {
+#pragma unroll
+            for (int t = 0; t < kTiles; ++t) part[(4 * st + g) * 64 + 16 * t + cl] = p0[i][t];
         }
     }
-}
+    if (wave == 0 && ST > 16) {
+#pragma unroll
+        for (int t = 0; t < kTiles; ++t) part[(64 + g) * 64 + 16 * t + cl] = p1[t];
+    }
+    __syncthreads();
 
-template <int LMAX>
-__global__ void __launch_bounds__(256)
-k_crt_finish(const long long* __restrict__ part, const double2* __restrict__ fq_in, uint32_t* __restrict__ c,
-             DevPlan P, uint32_t k0, uint32_t count) {
-    const uint32_t n = k0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= k0 + count) return;
-    const int K = P.K, L = P.L, NL = 4 * P.crt_stiles;   // limbs with partials
-    const double2 fv = fq_in[n];
-    const int v = crt_round(fv.x, K);
-    const uint64_t Q = quotient_estimate(fv.y + P.nq[v]);
-    const uint32_t Ql = static_cast<uint32_t>(Q), Qh = static_cast<uint32_t>(Q >> 32);
-    const uint32_t* nv = P.negvP + static_cast<size_t>(v) * L;
-    uint32_t x[LMAX + 2];
-#pragma unroll
-    for (int l = 0; l < LMAX; ++l) x[l] = nv[min(l, L - 1)];   // all in flight, clamped
+    // 3. one segment of 17 limbs per wave, one coefficient per lane; limbs past
+    // L + 2 are zero and only carry R's sign along
+    const int NL = 4 * ST;   // limbs with partials
+    const int l0 = kSegMax * wave;
+    const uint32_t v = vq[lane], Ql = vq[64 + lane], Qh = vq[128 + lane];
+    const uint32_t* pn = P.negvP + L;   // -P mod q
+    auto qlimb = [&](int l) -> uint32_t { return (l >= 0 && l < L) ? P.q[min(max(l, 0), L - 1)] : 0u; };
+    auto pnlimb = [&](int l) -> uint32_t { return (l >= 0 && l < L) ? pn[min(max(l, 0), L - 1)] : 0u; };
+    uint32_t x[kSegMax];
     long long carry = 0;
-    uint64_t a1 = 0, b1 = 0, b2 = 0;
-    const long long* pp = part + n;
+    {
+        uint64_t a1 = static_cast<uint64_t>(Ql) * qlimb(l0 - 1);
+        uint64_t b1 = static_cast<uint64_t>(Qh) * qlimb(l0 - 1);
+        uint64_t b2 = static_cast<uint64_t>(Qh) * qlimb(l0 - 2);
+        uint64_t c1 = static_cast<uint64_t>(v) * pnlimb(l0 - 1);
 #pragma unroll
-    for (int l0 = 0; l0 < LMAX + 2; l0 += 8) {
-        long long pv[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) pv[u] = pp[static_cast<size_t>(min(l0 + u, NL - 1)) * P.N];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < kSegMax; ++u) {
             const int l = l0 + u;
-            if (l < LMAX + 2) {
-                const uint32_t qv = l < L ? P.q[l] : 0u;
-                long long s = carry + (l < L + 2 ? static_cast<long long>(P.crt_corr[l]) : 0) -
-                              q_times_limb(Ql, Qh, qv, a1, b1, b2);
-                if (l < L) s += x[l];
-                if (l < NL) s += pv[u];
-                x[l] = static_cast<uint32_t>(s);
-                carry = s >> 32;   // arithmetic: partials may be negative
-            }
+            const long long pv = part[min(l, NL - 1) * 64 + lane];
+            const uint32_t corr = P.crt_corr[min(l, L + 1)];
+            const uint64_t vp = static_cast<uint64_t>(v) * pnlimb(l);
+            long long s = carry + (l < NL ? pv : 0) + (l < L + 2 ? static_cast<long long>(corr) : 0) +
+                          static_cast<long long>(static_cast<uint32_t>(vp)) + static_cast<long long>(c1 >> 32) -
+                          q_times_limb(Ql, Qh, qlimb(l), a1, b1, b2);
+            c1 = vp;
+            x[u] = static_cast<uint32_t>(s);
+            carry = s >> 32;
         }
     }
-    fix_range<LMAX>(x, P.q, L);
-    uint32_t* out = c + static_cast<size_t>(n) * L;
-    if ((L & 3) == 0 && (reinterpret_cast<uintptr_t>(c) & 15) == 0) {
-        // 16-byte stores: a quarter of the cache-line accesses of word stores
-        uint4* o4 = reinterpret_cast<uint4*>(out);
+    {
+        bool uz = true, uo = true;
 #pragma unroll
-        for (int l = 0; l < LMAX; l += 4)
-            if (l < L) o4[l / 4] = make_uint4(x[l], x[l + 1], x[l + 2], x[l + 3]);
-    } else {
+        for (int u = 2; u < kSegMax; ++u) {
+            uz = uz && x[u] == 0u;
+            uo = uo && x[u] == 0xffffffffu;
+        }
+        ex64[wave * 64 + lane] = carry;
+        ex64[256 + wave * 64 + lane] = static_cast<long long>(static_cast<uint64_t>(x[0]) | (static_cast<uint64_t>(x[1]) << 32));
+        ex32[wave * 64 + lane] = (uz ? 1u : 0u) | (uo ? 2u : 0u);
+    }
+    __syncthreads();
+    long long Cin = 0, Ctop = 0;
+    {
+        long long C = 0;
 #pragma unroll
-        for (int l = 0; l < LMAX; ++l)
-            if (l < L) out[l] = x[l];
+        for (int w = 0; w < 4; ++w) {
+            Cin = w == wave ? C : Cin;
+            const uint32_t fl = ex32[w * 64 + lane];
+            C = seg_carry(C, ex64[w * 64 + lane], static_cast<uint64_t>(ex64[256 + w * 64 + lane]), fl & 1u,
+                          (fl & 2u) != 0u);
+        }
+        Ctop = C;   // R's sign: |R| < 2^(32 L + 1)
+    }
+    {
+        long long C = Cin;
+#pragma unroll
+        for (int u = 0; u < kSegMax; ++u) {
+            const long long t = static_cast<long long>(x[u]) + C;
+            x[u] = static_cast<uint32_t>(t);
+            C = t >> 32;
+        }
+    }
+    // R >= q, segment by segment from the top
+    {
+        int cmp = 0;
+#pragma unroll
+        for (int u = kSegMax - 1; u >= 0; --u) {
+            const uint32_t ql = qlimb(l0 + u);
+            const int d = x[u] > ql ? 1 : (x[u] < ql ? -1 : 0);
+            cmp = cmp != 0 ? cmp : d;
+        }
+        cmpw[wave * 64 + lane] = cmp;
+    }
+    __syncthreads();
+    long long sign;
+    {
+        int cmp = 0;
+#pragma unroll
+        for (int w = 3; w >= 0; --w) cmp = cmp != 0 ? cmp : cmpw[w * 64 + lane];
+        sign = Ctop < 0 ? 1 : (cmp >= 0 ? -1 : 0);   // add q, subtract q, or keep
+    }
+    {
+        long long cc = 0;
+        bool az = true, ao = true;
+#pragma unroll
+        for (int u = 0; u < kSegMax; ++u) {
+            const long long t = static_cast<long long>(x[u]) + sign * static_cast<long long>(qlimb(l0 + u)) + cc;
+            x[u] = static_cast<uint32_t>(t);
+            cc = t >> 32;
+            az = az && x[u] == 0u;
+            ao = ao && x[u] == 0xffffffffu;
+        }
+        // exports of the first resolution are dead (every wave read them before the last barrier)
+        ex64[wave * 64 + lane] = cc;
+        ex32[wave * 64 + lane] = (az ? 1u : 0u) | (ao ? 2u : 0u);
+    }
+    __syncthreads();
+    {
+        long long C = 0;
+#pragma unroll
+        for (int w = 0; w < 3; ++w) {
+            const uint32_t fl = ex32[w * 64 + lane];
+            const long long over = (C == 1 && (fl & 2u)) ? 1 : ((C == -1 && (fl & 1u)) ? -1 : 0);
+            C = w < wave ? ex64[w * 64 + lane] + over : C;
+        }
+#pragma unroll
+        for (int u = 0; u < kSegMax; ++u) {
+            const long long t = static_cast<long long>(x[u]) + C;
+            x[u] = static_cast<uint32_t>(t);
+            C = t >> 32;
+        }
+    }
+    // 4. limbs below L to the stage (the partials are dead), then whole-line stores
+#pragma unroll
+    for (int u = 0; u < kSegMax; ++u)
+        if (l0 + u < L) stage[lane * (L + 1) + l0 + u] = x[u];
+    __syncthreads();
+    uint32_t* dst = c + static_cast<size_t>(nb) * L;
+    const uint32_t total = nk * static_cast<uint32_t>(L);
+    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+        const uint32_t r = i / L;
+        dst[i] = stage[r * (L + 1) + (i - r * L)];
     }
 }
 
@@ -371,18 +571,15 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
     const int nd = 4 * L + 1;
     std::vector<int8_t> m(static_cast<size_t>(K) * nd);
     for (int j = 0; j < K; ++j) balanced_digits(h.M.data() + static_cast<size_t>(j) * L, L, m.data() + static_cast<size_t>(j) * nd, nd);
-    std::vector<int8_t> crt(static_cast<size_t>(ST) * KS * 64 * 16, 0);
-    for (int st = 0; st < ST; ++st)
-        for (int ks = 0; ks < KS; ++ks)
-            for (int lane = 0; lane < 64; ++lane)
-                for (int e = 0; e < 16; ++e) {
-                    const int s = 16 * st + lane % 16;
-                    const int j = 16 * ks + 4 * (lane / 16) + e / 4;
-                    const int a = e % 4;
-                    const int idx = s - a;
-                    if (j < K && idx >= 0 && idx < nd)
-                        crt[((static_cast<size_t>(st) * KS + ks) * 64 + lane) * 16 + e] = m[static_cast<size_t>(j) * nd + idx];
-                }
+    if (ST > 17) throw std::logic_error("CRT limb tiles exceed the kernel's 17");
+    const int KJ = (K + 3) / 4;
+    const int kjs = KJ + ((2 - KJ % 4) + 4) % 4;   // = 2 (mod 4): conflict-free A-fragment reads
+    if (crt_tab_words(L, kjs) > 4u * kTabLoads * 64 * kWaves) throw std::logic_error("CRT digit table too large");
+    std::vector<uint32_t> tab(crt_tab_words(L, kjs), 0);
+    for (int sr = 0; sr < nd; ++sr)
+        for (int j = 0; j < K; ++j)
+            tab[static_cast<size_t>(sr) * kjs + j / 4] |=
+                static_cast<uint32_t>(static_cast<uint8_t>(m[static_cast<size_t>(j) * nd + sr])) << (8 * (j % 4));
     // 0x80808080 * sum_j M_j, exactly (L + 2 limbs).
     std::vector<uint64_t> acc(L + 2, 0);
     for (int j = 0; j < K; ++j)
@@ -402,10 +599,11 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
         carry = s >> 32;
     }
     if (carry) throw std::logic_error("CRT correction overflows L + 2 limbs");
-    d.crt_digits = static_cast<const int8_t*>(upload(crt.data(), crt.size()));
+    d.crt_tab = static_cast<const uint32_t*>(upload(tab.data(), tab.size() * sizeof(uint32_t)));
     d.crt_corr = static_cast<const uint32_t*>(upload(ccorr.data(), ccorr.size() * sizeof(uint32_t)));
     d.crt_ksteps = KS;
     d.crt_stiles = ST;
+    d.crt_kjs = kjs;
 }
 
 void launch_decompose_mfma(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st) {
@@ -433,28 +631,11 @@ void launch_crt_mfma(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream
 
 void launch_crt_mfma_range(const DevPlan& P, const uint32_t* Y, uint32_t* c, uint32_t k0, uint32_t count,
                            hipStream_t st) {
-    if (!P.crt_digits || !P.crt_part) throw std::runtime_error("matrix-core tables not prepared");
+    if (!P.crt_tab) throw std::runtime_error("matrix-core tables not prepared");
     if (count == 0) return;
     const dim3 grid((count + 63) / 64);
-    const size_t lds = sizeof(uint32_t) * (static_cast<size_t>(16 * P.crt_ksteps) * kYRow + 4 * 64 * 2 * 2);
-    hipLaunchKernelGGL(k_crt_mfma_partials, grid, dim3(64 * kWaves), lds, st, Y, P.crt_part, P.crt_fq, P, k0,
-                       count);
-    const dim3 grid2((count + 255) / 256);
-    const int lmax = (P.L + 7) / 8 * 8;
-    switch (lmax) {
-#define PM_CRT_CASE(LM) \
-    case LM: hipLaunchKernelGGL(k_crt_finish<LM>, grid2, dim3(256), 0, st, P.crt_part, P.crt_fq, c, P, k0, count); break;
-        PM_CRT_CASE(8)
-        PM_CRT_CASE(16)
-        PM_CRT_CASE(24)
-        PM_CRT_CASE(32)
-        PM_CRT_CASE(40)
-        PM_CRT_CASE(48)
-        PM_CRT_CASE(56)
-        PM_CRT_CASE(64)
-#undef PM_CRT_CASE
-        default: throw std::runtime_error("L out of range");
-    }
+    const size_t lds = sizeof(uint32_t) * crt_lds_words(P.crt_ksteps, P.crt_stiles, P.L, P.crt_kjs);
+    hipLaunchKernelGGL(k_crt_mfma, grid, dim3(64 * kWaves), lds, st, Y, c, P, k0, count);
 }
 
 }  // namespace pm

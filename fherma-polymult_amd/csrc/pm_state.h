@@ -79,12 +79,6 @@ struct pm_plan {
             pm::check(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice), "upload mfma tables");
             return d;
         });
-        // CRT workspace: signed limb partials and the two rounding sums
-        const size_t np = static_cast<size_t>(4 * dev.crt_stiles) * host.N;
-        dev.crt_part = static_cast<long long*>(pm::dev_alloc(np * sizeof(long long)));
-        owned.push_back(dev.crt_part);
-        dev.crt_fq = static_cast<double2*>(pm::dev_alloc(static_cast<size_t>(host.N) * sizeof(double2)));
-        owned.push_back(dev.crt_fq);
         mfma_ready = true;
     }
 };

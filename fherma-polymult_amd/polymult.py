@@ -72,6 +72,7 @@ def _load() -> ctypes.CDLL:
         "pm_device_free": (ctypes.c_int, [_vp]),
         "pm_copy_to_device": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
         "pm_copy_to_host": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+        "pm_device_copy": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp]),
         "pm_stream_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
         "pm_stream_destroy": (ctypes.c_int, [_vp]),
         "pm_stream_synchronize": (ctypes.c_int, [_vp]),
@@ -151,6 +152,14 @@ class DeviceBuffer:
 
 def synchronize() -> None:
     _check(lib.pm_device_synchronize())
+
+
+def device_copy(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int = None, stream=None) -> None:
+    """Device-to-device copy by the library's plain streaming kernel."""
+    n = min(dst.nbytes, src.nbytes) if nbytes is None else int(nbytes)
+    if n > dst.nbytes or n > src.nbytes:
+        raise ValueError("copy larger than its buffers")
+    _check(lib.pm_device_copy(_vp(dst.ptr), _vp(src.ptr), n, _vp(stream)))
 
 
 def device_info():

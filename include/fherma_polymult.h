@@ -133,6 +133,12 @@ int pm_device_alloc(size_t bytes, void** out);
 int pm_device_free(void* ptr);
 int pm_copy_to_device(void* dst, const void* src, size_t bytes);
 int pm_copy_to_host(void* dst, const void* src, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (16-byte loads and stores),
+ * enqueued on `stream`; pointers 16-byte aligned, bytes a multiple of 16. No
+ * counterpart in the reference: it is the known-size yardstick against which
+ * the HBM traffic counters (FETCH_SIZE, WRITE_SIZE) of the product's kernels
+ * are scaled. */
+int pm_device_copy(void* dst, const void* src, size_t bytes, void* stream);
 int pm_stream_create(void** out);
 int pm_stream_destroy(void* stream);
 int pm_stream_synchronize(void* stream);
