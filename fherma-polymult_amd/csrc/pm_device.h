@@ -52,6 +52,11 @@ struct DevPlan {
     // M_j for primes j = 4J .. 4J + 3 in word J (zero for j >= K); rows of
     // crt_kjs words, crt_kjs = 2 (mod 4) and >= ceil(K / 4).
     const uint32_t* crt_tab;
+    // CRT constants, zero-padded so that every index the kernel forms is in
+    // range: crt_fc = 1 / p_j (160 entries), then M_j / q (160); crt_lc = q,
+    // -P mod q and the offset correction, 72 words each, limb l at l + 2.
+    const double* crt_fc;
+    const uint32_t* crt_lc;
     const uint32_t* crt_corr; // L + 2 limbs: 128 (2^32 - 1)/255 sum_j M_j, exact
     int dec_ksteps;           // ceil(4L / 64)
     int crt_ksteps;           // ceil(4K / 64)

@@ -291,10 +291,10 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
             const int j = 4 * (wave + 4 * (i / 4)) + (i % 4);
             y[i] = Y[static_cast<size_t>(min(j, K - 1)) * P.N + col];
         }
+        // past the table's end every thread rewrites its last entry with the same value
         uint4* tab4s = reinterpret_cast<uint4*>(tab);
 #pragma unroll
-        for (int r = 0; r < kTabLoads; ++r)
-            if (threadIdx.x + r * blockDim.x < tab4n) tab4s[threadIdx.x + r * blockDim.x] = tv[r];
+        for (int r = 0; r < kTabLoads; ++r) tab4s[min(threadIdx.x + r * blockDim.x, tab4n - 1)] = tv[r];
         double f = 0, f2 = 0;
 #pragma unroll
         for (int i = 0; i < kCrtMaxKSteps; ++i) {
@@ -303,10 +303,10 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
             uint32_t w[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int j = 4 * J + r;
+                const int j = 4 * J + r;   // < 160: the padded constants are zero past K
                 w[r] = (j < K && lane_in) ? y[4 * i + r] : 0u;
-                f = fma(static_cast<double>(w[r]), P.p_recip[min(j, K - 1)], f);
-                f2 = fma(static_cast<double>(w[r]), P.mq[min(j, K - 1)], f2);
+                f = fma(static_cast<double>(w[r]), P.crt_fc[j], f);
+                f2 = fma(static_cast<double>(w[r]), P.crt_fc[4 * 4 * kCrtMaxKSteps + j], f2);
             }
             uint32_t d[4];
             transpose_bytes(w[0], w[1], w[2], w[3], d);
@@ -353,24 +353,27 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
                 bf[t][2] = static_cast<int>(yr[2 * kYRow + 16 * t]);
                 bf[t][3] = static_cast<int>(yr[3 * kYRow + 16 * t]);
             }
+            // A fragments of the wave's (up to) five limb tiles, clamped and masked,
+            // all read before the MFMAs; tiles past ST multiply zeros
+            v4i af[5];
 #pragma unroll
             for (int i = 0; i < 5; ++i) {
-                const int st = i < 4 ? wave + 4 * i : 16;
-                if (i < 4 ? st >= ST : !has1) continue;
-                const int s = 16 * st + cl;
-                v4i af;
+                const int s = 16 * (i < 4 ? wave + 4 * i : 16) + cl;
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
                     const int row = s - a;
                     const bool ok = J < KJ && row >= 0 && row < nd;
                     const uint32_t v = tab[min(max(row, 0), nd - 1) * KJS + min(J, KJS - 1)];
-                    af[a] = static_cast<int>(ok ? v : 0u);
+                    af[i][a] = static_cast<int>(ok ? v : 0u);
                 }
+            }
 #pragma unroll
-                for (int t = 0; t < kTiles; ++t) {
-                    if (i < 4) acc[i < 4 ? i : 0][t] = mfma16(af, bf[t], acc[i < 4 ? i : 0][t]);
-                    else acc1[t] = mfma16(af, bf[t], acc1[t]);
-                }
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int t = 0; t < kTiles; ++t) acc[i][t] = mfma16(af[i], bf[t], acc[i][t]);
+            if (has1) {
+#pragma unroll
+                for (int t = 0; t < kTiles; ++t) acc1[t] = mfma16(af[4], bf[t], acc1[t]);
             }
         }
 #pragma unroll
@@ -401,9 +404,12 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
     const int NL = 4 * ST;   // limbs with partials
     const int l0 = kSegMax * wave;
     const uint32_t v = vq[lane], Ql = vq[64 + lane], Qh = vq[128 + lane];
-    const uint32_t* pn = P.negvP + L;   // -P mod q
-    auto qlimb = [&](int l) -> uint32_t { return (l >= 0 && l < L) ? P.q[min(max(l, 0), L - 1)] : 0u; };
-    auto pnlimb = [&](int l) -> uint32_t { return (l >= 0 && l < L) ? pn[min(max(l, 0), L - 1)] : 0u; };
+    // limb l of q, -P mod q and the offset correction at l + 2 (l >= -2), zero outside
+    const uint32_t* lq = P.crt_lc + 2;
+    const uint32_t* lpn = P.crt_lc + 72 + 2;
+    const uint32_t* lcorr = P.crt_lc + 144 + 2;
+    auto qlimb = [&](int l) -> uint32_t { return lq[l]; };
+    auto pnlimb = [&](int l) -> uint32_t { return lpn[l]; };
     uint32_t x[kSegMax];
     long long carry = 0;
     {
@@ -415,9 +421,8 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
         for (int u = 0; u < kSegMax; ++u) {
             const int l = l0 + u;
             const long long pv = part[min(l, NL - 1) * 64 + lane];
-            const uint32_t corr = P.crt_corr[min(l, L + 1)];
             const uint64_t vp = static_cast<uint64_t>(v) * pnlimb(l);
-            long long s = carry + (l < NL ? pv : 0) + (l < L + 2 ? static_cast<long long>(corr) : 0) +
+            long long s = carry + (l < NL ? pv : 0) + static_cast<long long>(lcorr[l]) +
                           static_cast<long long>(static_cast<uint32_t>(vp)) + static_cast<long long>(c1 >> 32) -
                           q_times_limb(Ql, Qh, qlimb(l), a1, b1, b2);
             c1 = vp;
@@ -515,6 +520,7 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
     __syncthreads();
     uint32_t* dst = c + static_cast<size_t>(nb) * L;
     const uint32_t total = nk * static_cast<uint32_t>(L);
+#pragma unroll 4
     for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
         const uint32_t r = i / L;
         dst[i] = stage[r * (L + 1) + (i - r * L)];
@@ -600,6 +606,20 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
     }
     if (carry) throw std::logic_error("CRT correction overflows L + 2 limbs");
     d.crt_tab = static_cast<const uint32_t*>(upload(tab.data(), tab.size() * sizeof(uint32_t)));
+    constexpr int kFc = 4 * 4 * kCrtMaxKSteps;
+    std::vector<double> fc(2 * kFc, 0.0);
+    for (int j = 0; j < K; ++j) {
+        fc[j] = h.p_recip[j];
+        fc[kFc + j] = h.mq[j];
+    }
+    d.crt_fc = static_cast<const double*>(upload(fc.data(), fc.size() * sizeof(double)));
+    std::vector<uint32_t> lc(3 * 72, 0);
+    for (int l = 0; l < L; ++l) {
+        lc[l + 2] = h.q[l];
+        lc[72 + l + 2] = h.negvP[static_cast<size_t>(L) + l];   // row v = 1: -P mod q
+    }
+    for (int l = 0; l < L + 2; ++l) lc[144 + l + 2] = ccorr[l];
+    d.crt_lc = static_cast<const uint32_t*>(upload(lc.data(), lc.size() * sizeof(uint32_t)));
     d.crt_corr = static_cast<const uint32_t*>(upload(ccorr.data(), ccorr.size() * sizeof(uint32_t)));
     d.crt_ksteps = KS;
     d.crt_stiles = ST;
