@@ -47,7 +47,9 @@ struct DevPlan {
     const double* nq;         // K + 1: negvP_v / q
     // int8 matrix-core forms (pm_mfma.hip); null until prepared.
     const int8_t* dec_digits; // [ceil(K/4)][dec_ksteps][64 lanes][16]: MFMA A fragments
-    const uint32_t* dec_corr; // K: 128 sum_t (256^t mod p_j) mod p_j
+    // K: 128 sum_t (256^t mod p_j) mod p_j plus the least multiple of p_j that
+    // is >= 2^47, so that every decompose sum is positive before its reduction
+    const double* dec_corr;
     // CRT digit table: row s (0..4L) holds byte s of the balanced digits of
     // M_j for primes j = 4J .. 4J + 3 in word J (zero for j >= K); rows of
     // crt_kjs words, crt_kjs = 2 (mod 4) and >= ceil(K / 4).

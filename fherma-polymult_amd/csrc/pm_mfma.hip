@@ -58,15 +58,14 @@ __device__ __forceinline__ v4i mfma16(const v4i& a, const v4i& b, const v4i& c) 
     return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
 }
 
-// r mod p for an integer-valued double |r| < 2^52: one FP64 quotient estimate
-// (off by at most one), an exact FMA remainder, then corrections.
+// r mod p for an integer-valued double 0 <= r < 2^50: the estimate
+// floor(r / p - 1/2) (the FMA's error is below 2^-30) is floor(r / p) or one
+// less, so the exact FMA remainder lies in [0, 2p), below 2^32; it is converted
+// and brought to [0, p) with one unsigned min.
 __device__ __forceinline__ uint32_t mod_p_f64(double r, uint32_t p, double pinv) {
-    const double pd = static_cast<double>(p);
-    const double qd = floor(r * pinv);
-    long long ri = static_cast<long long>(fma(-qd, pd, r));
-    ri += ri < 0 ? static_cast<long long>(p) : 0;
-    ri -= ri >= static_cast<long long>(p) ? static_cast<long long>(p) : 0;
-    return static_cast<uint32_t>(ri);
+    const double qd = floor(fma(r, pinv, -0.5));
+    const uint32_t v = static_cast<uint32_t>(fma(-qd, static_cast<double>(p), r));
+    return min(v, v - p);
 }
 
 // Decompose, one workgroup per 64 coefficients (4 tiles of 16). Their limbs are
@@ -81,10 +80,10 @@ k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
     extern __shared__ uint32_t xs[];
     constexpr int Ls = 16 * KT + 4;
     const int L = P.L;
-    // per-prime constants for the epilogue: p, corr, 1/p
+    // per-prime constants for the epilogue: 1/p, corr, p
     double* kpinv = reinterpret_cast<double*>(xs + 16 * kTiles * Ls);
-    uint32_t* kp = reinterpret_cast<uint32_t*>(kpinv + P.K);
-    uint32_t* kcorr = kp + P.K;
+    double* kcorr = kpinv + P.K;
+    uint32_t* kp = reinterpret_cast<uint32_t*>(kcorr + P.K);
     for (int j = threadIdx.x; j < P.K; j += blockDim.x) {
         kpinv[j] = P.p_recip[j];
         kp[j] = P.pc[j].p;
@@ -165,14 +164,14 @@ k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
         if (j < P.K) {
             const uint32_t p = kp[j];
             const double pinv = kpinv[j];
-            const double corr = static_cast<double>(kcorr[j]);
+            const double corr = kcorr[j];
             uint32_t* out = res + static_cast<size_t>(j) * P.N + nb;
 #pragma unroll
             for (int t = 0; t < kTiles; ++t) {
-                const double r = fma(16777216.0, static_cast<double>(acc[t][3]),
-                                     fma(65536.0, static_cast<double>(acc[t][2]),
-                                         fma(256.0, static_cast<double>(acc[t][1]),
-                                             static_cast<double>(acc[t][0]) + corr)));
+                // |G_b| <= 2^22, so G_0 + 256 G_1 and G_2 + 256 G_3 fit in int32
+                const int lo = acc[t][0] + acc[t][1] * 256;
+                const int hi = acc[t][2] + acc[t][3] * 256;
+                const double r = fma(65536.0, static_cast<double>(hi), static_cast<double>(lo) + corr);
                 const uint32_t v = mod_p_f64(r, p, pinv);
                 if (static_cast<uint32_t>(16 * t + cl) < nk) out[16 * t + cl] = v;
             }
@@ -207,11 +206,12 @@ k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
 // 4 (4 ks + g) + r, byte a of y; the A word for row s is then word J = 4 ks + g
 // of digit-table row s - a.
 constexpr int kYRow = 68;         // 64 coefficients + 4: lane groups g land on alternate bank halves
-constexpr int kTabLoads = 11;     // 16-byte loads per thread for the digit table: (4 64 + 1) 42 words
+constexpr int kTabLoads = 11;     // 16-byte loads per thread for the digit table: (4 64 + 2) 42 words
 constexpr int kSegMax = 17;       // limbs per segment: 4 x 17 >= L + 2 for L <= 64
 
+// 4 L + 1 digit rows and one zero row (the target of out-of-range rows), whole 16-byte units
 __host__ __device__ constexpr uint32_t crt_tab_words(int L, int kjs) {
-    return (static_cast<uint32_t>(4 * L + 1) * kjs + 3u) & ~3u;
+    return (static_cast<uint32_t>(4 * L + 2) * kjs + 3u) & ~3u;
 }
 
 // LDS words: region 0 holds the y digits (phases 1-2), then the limb partials
@@ -310,8 +310,10 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
             }
             uint32_t d[4];
             transpose_bytes(w[0], w[1], w[2], w[3], d);
+            // groups past the last prime hold zeros, so any A word times them is 0
+            const uint32_t off = J < KJ ? 0x80808080u : 0u;
 #pragma unroll
-            for (int a = 0; a < 4; ++a) ydig[(4 * J + a) * kYRow + lane] = d[a] ^ 0x80808080u;
+            for (int a = 0; a < 4; ++a) ydig[(4 * J + a) * kYRow + lane] = J < KJ ? d[a] ^ off : 0u;
         }
         fr[wave * 64 + lane] = f;
         fq[wave * 64 + lane] = f2;
@@ -341,9 +343,21 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
 #pragma unroll
         for (int t = 0; t < kTiles; ++t) acc1[t] = v4i{0, 0, 0, 0};
         const bool has1 = wave == 0 && ST > 16;
+        // A word (tile slot i, byte a) of K-step ks is tab[base[i][a] + 4 ks]: rows
+        // outside the digits point at the zero row. Columns past KJS fall into
+        // the next row (or past the table, still inside LDS) and meet zero B rows.
+        uint32_t base[5][4];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const int s = 16 * (i < 4 ? wave + 4 * i : 16) + cl;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+                const int row = s - a;
+                base[i][a] = static_cast<uint32_t>((row >= 0 && row < nd ? row : nd) * KJS + g);
+            }
+        }
 #pragma unroll 1
         for (int ks = 0; ks < KS; ++ks) {
-            const int J = 4 * ks + g;
             v4i bf[kTiles];
             const uint32_t* yr = ydig + (16 * ks + 4 * g) * kYRow + cl;
 #pragma unroll
@@ -353,20 +367,14 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
                 bf[t][2] = static_cast<int>(yr[2 * kYRow + 16 * t]);
                 bf[t][3] = static_cast<int>(yr[3 * kYRow + 16 * t]);
             }
-            // A fragments of the wave's (up to) five limb tiles, clamped and masked,
-            // all read before the MFMAs; tiles past ST multiply zeros
+            // A fragments of the wave's (up to) five limb tiles, all read before the
+            // MFMAs; tiles past ST read the zero row
             v4i af[5];
+            const uint32_t* tk = tab + 4 * ks;
 #pragma unroll
-            for (int i = 0; i < 5; ++i) {
-                const int s = 16 * (i < 4 ? wave + 4 * i : 16) + cl;
+            for (int i = 0; i < 5; ++i)
 #pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    const int row = s - a;
-                    const bool ok = J < KJ && row >= 0 && row < nd;
-                    const uint32_t v = tab[min(max(row, 0), nd - 1) * KJS + min(J, KJS - 1)];
-                    af[i][a] = static_cast<int>(ok ? v : 0u);
-                }
-            }
+                for (int a = 0; a < 4; ++a) af[i][a] = static_cast<int>(tk[base[i][a]]);
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -548,7 +556,7 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
     const int KT = (4 * L + 63) / 64;
     const int KG = (K + 3) / 4;
     std::vector<int8_t> dec(static_cast<size_t>(KG) * KT * 64 * 16, 0);
-    std::vector<uint32_t> dcorr(K);
+    std::vector<double> dcorr(K);
     for (int j = 0; j < K; ++j) {
         const uint64_t p = h.p[j];
         uint64_t T = 1, sum = 0;
@@ -565,10 +573,13 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
             }
             T = T * 256 % p;
         }
-        dcorr[j] = static_cast<uint32_t>(sum * 128 % p);
+        // |sum_b 256^b G_b| < 2^47: adding ceil(2^47 / p) p keeps it positive and
+        // below 2^49 (exact in FP64), without changing the residue
+        const uint64_t lift = ((uint64_t(1) << 47) + p - 1) / p * p;
+        dcorr[j] = static_cast<double>(sum * 128 % p + lift);
     }
     d.dec_digits = static_cast<const int8_t*>(upload(dec.data(), dec.size()));
-    d.dec_corr = static_cast<const uint32_t*>(upload(dcorr.data(), dcorr.size() * sizeof(uint32_t)));
+    d.dec_corr = static_cast<const double*>(upload(dcorr.data(), dcorr.size() * sizeof(double)));
     d.dec_ksteps = KT;
 
     // CRT tables.
@@ -635,7 +646,7 @@ void launch_decompose_mfma_range(const DevPlan& P, const uint32_t* limbs, uint32
     if (!P.dec_digits) throw std::runtime_error("matrix-core tables not prepared");
     if (count == 0) return;
     const dim3 grid((count + 16 * kTiles - 1) / (16 * kTiles));
-    const size_t lds = sizeof(uint32_t) * 16 * kTiles * (16 * P.dec_ksteps + 4) + 16 * static_cast<size_t>(P.K);
+    const size_t lds = sizeof(uint32_t) * 16 * kTiles * (16 * P.dec_ksteps + 4) + 20 * static_cast<size_t>(P.K);
     switch (P.dec_ksteps) {
         case 1: hipLaunchKernelGGL(k_decompose_mfma<1>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
         case 2: hipLaunchKernelGGL(k_decompose_mfma<2>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
