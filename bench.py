@@ -98,9 +98,9 @@ def phase_bytes(K, L):
 
 
 # Kernel names (as rocprofv3 reports them) of each phase, for the traffic table.
-PHASE_KERNELS = {"decompose_a": "k_decompose_mfma", "decompose_b": "k_decompose_mfma",
-                 "ntt_cols_fwd_a": "k_ntt_col_reg", "ntt_cols_fwd_b": "k_ntt_col_reg",
-                 "ntt_rows_fused": "k_ntt_rows_fused_reg", "ntt_cols_inv": "k_ntt_col_reg",
+PHASE_KERNELS = {"decompose_a": "k_decompose_mfma<4>", "decompose_b": "k_decompose_mfma<4>",
+                 "ntt_cols_fwd_a": "k_ntt_col_reg<8,true>", "ntt_cols_fwd_b": "k_ntt_col_reg<8,true>",
+                 "ntt_rows_fused": "k_ntt_rows_fused_reg<8>", "ntt_cols_inv": "k_ntt_col_reg<8,false>",
                  "crt": "k_crt_mfma"}
 
 
@@ -145,6 +145,56 @@ def cpu_baseline(points, budget_s=12.0):
             "sample": "%d products at N=65536, W=2048 (K=134), %.1f s, OMP threads=%d" % (done, el, threads)}
 
 
+def batch_rates(args, plans, da, db, dc, gold, log):
+    """Products/s of the matrix-core batch (one hipGraph) and of the plain form."""
+    import polymult  # noqa: E402
+    L = plans[0].L
+    graph = polymult.Batch(plans, da, db, dc)
+
+    def step():
+        graph.launch()
+
+    def timed(steps, warmup):
+        for _ in range(warmup):
+            step()
+        polymult.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        polymult.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    # correctness of the batch against the GMP digests
+    step()
+    polymult.synchronize()
+    mismatches = sum(digest(dc[i].download((N, L))) != gold.get(SEED0 + i) for i in range(BATCH))
+    log("mfma batch: %d mismatches vs GMP" % mismatches)
+
+    sec = timed(args.steps, args.warmup)
+    value = BATCH / sec
+    log("mfma: %.3f ms per step, %.1f products/s" % (sec * 1e3, value))
+
+    # The plain HIP form: VALU decompose and CRT, products launched one by one.
+    plain_value = None
+    if not args.no_plain:
+        for p in plans:
+            p.set_form(polymult.FORM_PLAIN)
+
+        def step():  # noqa: F811
+            polymult.batch_multiply_device(plans, da, db, dc)
+
+        step()
+        polymult.synchronize()
+        mismatches += sum(digest(dc[i].download((N, L))) != gold.get(SEED0 + i) for i in range(BATCH))
+        plain_sec = timed(max(3, args.steps // 4), 1)
+        plain_value = BATCH / plain_sec
+        log("plain: %.3f ms per step, %.1f products/s" % (plain_sec * 1e3, plain_value))
+        for p in plans:
+            p.set_form(polymult.FORM_MFMA)
+
+    return value, sec, plain_value, mismatches
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +202,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--phases-only", action="store_true",
+                    help="only the per-kernel HIP-event profile (one product at a time on one stream), so that "
+                         "a kernel trace of this run holds no concurrently running kernels")
     ap.add_argument("--copy-yardstick", action="store_true",
                     help="first run three 256 MiB device copies (the known-size kernel that scales the "
                          "HBM traffic counters)")
@@ -182,54 +235,21 @@ def main():
     db = [polymult.DeviceBuffer.from_array(b) for _, _, b in points]
     dc = [polymult.DeviceBuffer(4 * N * L) for _ in points]
 
-    graph = polymult.Batch(plans, da, db, dc)
-
-    def step():
-        graph.launch()
-
-    def timed(steps, warmup):
-        for _ in range(warmup):
-            step()
-        polymult.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            step()
-        polymult.synchronize()
-        return (time.perf_counter() - t0) / steps
-
-    # correctness of the batch against the GMP digests
-    step()
-    polymult.synchronize()
     gold = golden_digests()
-    mismatches = sum(digest(dc[i].download((N, L))) != gold.get(SEED0 + i) for i in range(BATCH))
-    log("mfma batch: %d mismatches vs GMP" % mismatches)
-
-    sec = timed(args.steps, args.warmup)
-    value = BATCH / sec
-    log("mfma: %.3f ms per step, %.1f products/s" % (sec * 1e3, value))
-
-    # The plain HIP form: VALU decompose and CRT, products launched one by one.
-    plain_value = None
-    if not args.no_plain:
-        for p in plans:
-            p.set_form(polymult.FORM_PLAIN)
-
-        def step():  # noqa: F811
-            polymult.batch_multiply_device(plans, da, db, dc)
-
-        step()
+    value = sec = plain_value = None
+    if args.phases_only:
+        for p, a, b, c in zip(plans, da, db, dc):
+            p.multiply_device(a, b, c)
         polymult.synchronize()
-        mismatches += sum(digest(dc[i].download((N, L))) != gold.get(SEED0 + i) for i in range(BATCH))
-        plain_sec = timed(max(3, args.steps // 4), 1)
-        plain_value = BATCH / plain_sec
-        log("plain: %.3f ms per step, %.1f products/s" % (plain_sec * 1e3, plain_value))
-        for p in plans:
-            p.set_form(polymult.FORM_MFMA)
+        mismatches = sum(digest(dc[i].download((N, L))) != gold.get(SEED0 + i) for i in range(BATCH))
+        log("one product at a time: %d mismatches vs GMP" % mismatches)
+    else:
+        value, sec, plain_value, mismatches = batch_rates(args, plans, da, db, dc, gold, log)
 
     # per-phase HIP-event timings on the plan's stream (mean over the batch)
     phases = {}
     for p, a, b, c in zip(plans, da, db, dc):
-        for k, v in p.profile_phases(a, b, c, 1).items():
+        for k, v in p.profile_phases(a, b, c, 20).items():
             phases[k] = phases.get(k, 0.0) + v / BATCH
     log("phases (ms): " + ", ".join("%s %.4f" % kv for kv in phases.items()))
     nbytes = phase_bytes(K, L)
@@ -244,8 +264,8 @@ def main():
     name, cus = polymult.device_info()
     out = {
         "metric": "polymuls/s at N=65536, 2048-bit q: batched+MFMA vs plain HIP; % HBM roofline",
-        "value": round(value, 2), "unit": "polymuls/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(sec * 1e3, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "value": None if value is None else round(value, 2), "unit": "polymuls/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": None if sec is None else round(sec * 1e3, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32", "data": "synthetic",
         "config": {"workload": "batch of 16 products, N=65536, random odd 2048-bit q (L=64, K=134), "
                                "device-resident", "form": "int8-MFMA decompose/CRT, batch replayed as one hipGraph", "batch": BATCH, "N": N, "W": W, "L": L, "K": K,
@@ -253,7 +273,7 @@ def main():
         "plain_value": None if plain_value is None else round(plain_value, 2),
         "mfma_over_plain": None if plain_value is None else round(value / plain_value, 3),
         "mismatches": int(mismatches),
-        "effective_GBps_per_product_yardstick": round(prod_bytes * value / 1e9, 1),
+        "effective_GBps_per_product_yardstick": None if value is None else round(prod_bytes * value / 1e9, 1),
         "phases_ms": {k: round(v, 5) for k, v in phases.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -181,6 +181,7 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q)
     pl->owned.push_back(pl->B);
     check(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : pl->ev) check(hipEventCreate(&e), "hipEventCreate");
+    for (auto& e : pl->ev_stop) check(hipEventCreate(&e), "hipEventCreate");
     pl->ensure_mfma();
     pl->form = PM_FORM_MFMA;
     return pl.release();
@@ -197,29 +198,25 @@ const char* const kPhaseNames[PM_NPHASES] = {"decompose_a", "decompose_b", "ntt_
 
 namespace pm {
 
-// The phases of one product on device pointers, enqueued on st. When ev is
-// given, ev[i] is recorded before phase i and ev[PM_NPHASES] after the last.
-void run_device(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st, hipEvent_t* ev) {
+// Phase i (0 .. PM_NPHASES - 1) of one product on device pointers, enqueued on st.
+void run_phase(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st, int i) {
     const pm::DevPlan& P = pl->dev;
     const bool mfma = pl->form == PM_FORM_MFMA;
-    auto mark = [&](int i) {
-        if (ev) check(hipEventRecord(ev[i], st), "hipEventRecord");
-    };
-    mark(0);
-    if (mfma) pm::launch_decompose_mfma(P, a, pl->A, st); else pm::launch_decompose_valu(P, a, pl->A, st);
-    mark(1);
-    if (mfma) pm::launch_decompose_mfma(P, b, pl->B, st); else pm::launch_decompose_valu(P, b, pl->B, st);
-    mark(2);
-    pm::launch_ntt_cols(P, pl->A, true, st);
-    mark(3);
-    pm::launch_ntt_cols(P, pl->B, true, st);
-    mark(4);
-    pm::launch_ntt_rows_fused(P, pl->A, pl->B, st);
-    mark(5);
-    pm::launch_ntt_cols(P, pl->A, false, st);
-    mark(6);
-    if (mfma) pm::launch_crt_mfma(P, pl->A, c, st); else pm::launch_crt_valu(P, pl->A, c, st);
-    mark(7);
+    switch (i) {
+        case 0: if (mfma) launch_decompose_mfma(P, a, pl->A, st); else launch_decompose_valu(P, a, pl->A, st); break;
+        case 1: if (mfma) launch_decompose_mfma(P, b, pl->B, st); else launch_decompose_valu(P, b, pl->B, st); break;
+        case 2: launch_ntt_cols(P, pl->A, true, st); break;
+        case 3: launch_ntt_cols(P, pl->B, true, st); break;
+        case 4: launch_ntt_rows_fused(P, pl->A, pl->B, st); break;
+        case 5: launch_ntt_cols(P, pl->A, false, st); break;
+        case 6: if (mfma) launch_crt_mfma(P, pl->A, c, st); else launch_crt_valu(P, pl->A, c, st); break;
+        default: throw std::logic_error("no such phase");
+    }
+}
+
+// The phases of one product on device pointers, enqueued on st.
+void run_device(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st) {
+    for (int i = 0; i < PM_NPHASES; ++i) run_phase(pl, a, b, c, st, i);
     check(hipGetLastError(), "kernel launch");
 }
 
@@ -459,17 +456,23 @@ int pm_profile_phases(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint3
     return guarded([&] {
         need(plan && a && b && c && ms_per_phase && iters > 0, "bad argument");
         hipStream_t st = stream_of(plan, stream);
-        for (int i = 0; i < PM_NPHASES; ++i) ms_per_phase[i] = 0;
-        for (int it = 0; it < iters; ++it) {
-            run_device(plan, a, b, c, st, plan->ev);
-            check(hipEventSynchronize(plan->ev[PM_NPHASES]), "hipEventSynchronize");
-            for (int i = 0; i < PM_NPHASES; ++i) {
-                float ms = 0;
-                check(hipEventElapsedTime(&ms, plan->ev[i], plan->ev[i + 1]), "hipEventElapsedTime");
-                ms_per_phase[i] += ms;
-            }
+        // Each phase's kernel runs `iters` times back to back between two events
+        // (re-running a phase on its own output is well defined: every kernel
+        // maps residues to residues), so the mean includes one dispatch gap and
+        // no host latency. A last full product leaves the true result in c.
+        for (int i = 0; i < PM_NPHASES; ++i) {
+            check(hipEventRecord(plan->ev[i], st), "hipEventRecord");
+            for (int it = 0; it < iters; ++it) pm::run_phase(plan, a, b, c, st, i);
+            check(hipEventRecord(plan->ev_stop[i], st), "hipEventRecord");
         }
-        for (int i = 0; i < PM_NPHASES; ++i) ms_per_phase[i] /= iters;
+        check(hipGetLastError(), "kernel launch");
+        run_device(plan, a, b, c, st);
+        check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        for (int i = 0; i < PM_NPHASES; ++i) {
+            float ms = 0;
+            check(hipEventElapsedTime(&ms, plan->ev[i], plan->ev_stop[i]), "hipEventElapsedTime");
+            ms_per_phase[i] = ms / iters;
+        }
     });
 }
 

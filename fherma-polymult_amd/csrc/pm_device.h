@@ -66,6 +66,12 @@ struct DevPlan {
     int crt_kjs;
 };
 
+// Every kernel of the library is launched through this one function.
+template <typename F, typename... Args>
+inline void launch(F kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st, Args... args) {
+    hipLaunchKernelGGL(kernel, grid, block, lds, st, args...);
+}
+
 // Plain form (VALU): every launcher enqueues on `st` and returns.
 void launch_decompose_valu(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st);
 void launch_ntt_cols(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st);
@@ -87,7 +93,6 @@ void launch_ntt_rows_fused_generic(const DevPlan& P, uint32_t* A, const uint32_t
 
 // Matrix-core form (int8 MFMA, exact).
 void launch_decompose_mfma(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st);
-// mid, when given, is recorded between the partials and the reduction kernels.
 // Plain grid-stride copy of `bytes` (a multiple of 16) with 16-byte accesses.
 void launch_copy16(void* dst, const void* src, size_t bytes, hipStream_t st);
 

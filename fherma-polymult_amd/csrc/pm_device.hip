@@ -280,7 +280,7 @@ void launch_copy16(void* dst, const void* src, size_t bytes, hipStream_t st) {
     const size_t n = bytes / 16;
     if (n == 0) return;
     const unsigned blocks = static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 8192));
-    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, st, static_cast<uint4*>(dst),
+    launch(k_copy16, dim3(blocks), dim3(256), 0, st, static_cast<uint4*>(dst),
                        static_cast<const uint4*>(src), n);
 }
 
@@ -293,7 +293,7 @@ void launch_decompose_valu_range(const DevPlan& P, const uint32_t* limbs, uint32
     if (count == 0) return;
     const dim3 grid((count + kDecCoef - 1) / kDecCoef);
     const size_t lds = sizeof(uint32_t) * kDecCoef * (P.L + 1);
-    hipLaunchKernelGGL(k_decompose_valu, grid, dim3(kDecThreads), lds, st, limbs, res, P, k0, count);
+    launch(k_decompose_valu, grid, dim3(kDecThreads), lds, st, limbs, res, P, k0, count);
 }
 
 void launch_ntt_cols_generic(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
@@ -302,9 +302,9 @@ void launch_ntt_cols_generic(const DevPlan& P, uint32_t* X, bool forward, hipStr
     const dim3 grid(1u << (P.logS - g.logC), P.K);
     const size_t lds = sizeof(uint32_t) << (g.logC + g.logU);
     if (forward)
-        hipLaunchKernelGGL(k_ntt_pass<true>, grid, dim3(kNttThreads), lds, st, X, P, g);
+        launch(k_ntt_pass<true>, grid, dim3(kNttThreads), lds, st, X, P, g);
     else
-        hipLaunchKernelGGL(k_ntt_pass<false>, grid, dim3(kNttThreads), lds, st, X, P, g);
+        launch(k_ntt_pass<false>, grid, dim3(kNttThreads), lds, st, X, P, g);
 }
 
 void launch_ntt_rows_generic(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
@@ -312,21 +312,21 @@ void launch_ntt_rows_generic(const DevPlan& P, uint32_t* X, bool forward, hipStr
     const dim3 grid(1u << (P.logR - g.logU), P.K);
     const size_t lds = sizeof(uint32_t) << (g.logC + g.logU);
     if (forward)
-        hipLaunchKernelGGL(k_ntt_pass<true>, grid, dim3(kNttThreads), lds, st, X, P, g);
+        launch(k_ntt_pass<true>, grid, dim3(kNttThreads), lds, st, X, P, g);
     else
-        hipLaunchKernelGGL(k_ntt_pass<false>, grid, dim3(kNttThreads), lds, st, X, P, g);
+        launch(k_ntt_pass<false>, grid, dim3(kNttThreads), lds, st, X, P, g);
 }
 
 void launch_ntt_rows_fused_generic(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st) {
     const TileGeom g = row_geom(P, 12);
     const dim3 grid(1u << (P.logR - g.logU), P.K);
     const size_t lds = 2 * (sizeof(uint32_t) << (g.logC + g.logU));
-    hipLaunchKernelGGL(k_ntt_rows_fused, grid, dim3(kNttThreads), lds, st, A, B, P, g);
+    launch(k_ntt_rows_fused, grid, dim3(kNttThreads), lds, st, A, B, P, g);
 }
 
 void launch_pointwise(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st) {
     const dim3 grid((P.N + 255) / 256 < 64 ? (P.N + 255) / 256 : 64, P.K);
-    hipLaunchKernelGGL(k_pointwise, grid, dim3(256), 0, st, A, B, P);
+    launch(k_pointwise, grid, dim3(256), 0, st, A, B, P);
 }
 
 void launch_crt_valu(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream_t st) {
@@ -340,7 +340,7 @@ void launch_crt_valu_range(const DevPlan& P, const uint32_t* Y, uint32_t* c, uin
     const int lmax = (P.L + 7) / 8 * 8;
     switch (lmax) {
 #define PM_CRT_CASE(LM) \
-    case LM: hipLaunchKernelGGL(k_crt_valu<LM>, grid, dim3(128), 0, st, Y, c, P, k0, count); break;
+    case LM: launch(k_crt_valu<LM>, grid, dim3(128), 0, st, Y, c, P, k0, count); break;
         PM_CRT_CASE(8)
         PM_CRT_CASE(16)
         PM_CRT_CASE(24)

@@ -648,10 +648,10 @@ void launch_decompose_mfma_range(const DevPlan& P, const uint32_t* limbs, uint32
     const dim3 grid((count + 16 * kTiles - 1) / (16 * kTiles));
     const size_t lds = sizeof(uint32_t) * 16 * kTiles * (16 * P.dec_ksteps + 4) + 20 * static_cast<size_t>(P.K);
     switch (P.dec_ksteps) {
-        case 1: hipLaunchKernelGGL(k_decompose_mfma<1>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
-        case 2: hipLaunchKernelGGL(k_decompose_mfma<2>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
-        case 3: hipLaunchKernelGGL(k_decompose_mfma<3>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
-        case 4: hipLaunchKernelGGL(k_decompose_mfma<4>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
+        case 1: launch(k_decompose_mfma<1>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
+        case 2: launch(k_decompose_mfma<2>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
+        case 3: launch(k_decompose_mfma<3>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
+        case 4: launch(k_decompose_mfma<4>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
         default: throw std::runtime_error("L out of range");
     }
 }
@@ -666,7 +666,7 @@ void launch_crt_mfma_range(const DevPlan& P, const uint32_t* Y, uint32_t* c, uin
     if (count == 0) return;
     const dim3 grid((count + 63) / 64);
     const size_t lds = sizeof(uint32_t) * crt_lds_words(P.crt_ksteps, P.crt_stiles, P.L, P.crt_kjs);
-    hipLaunchKernelGGL(k_crt_mfma, grid, dim3(64 * kWaves), lds, st, Y, c, P, k0, count);
+    launch(k_crt_mfma, grid, dim3(64 * kWaves), lds, st, Y, c, P, k0, count);
 }
 
 }  // namespace pm

@@ -334,9 +334,9 @@ void col_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     const dim3 block(1u << (logC + LOGM - 4));
     const size_t lds = sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logC;
     if (forward)
-        hipLaunchKernelGGL((k_ntt_col_reg<LOGM, true>), grid, block, lds, st, X, P, logC);
+        launch((k_ntt_col_reg<LOGM, true>), grid, block, lds, st, X, P, logC);
     else
-        hipLaunchKernelGGL((k_ntt_col_reg<LOGM, false>), grid, block, lds, st, X, P, logC);
+        launch((k_ntt_col_reg<LOGM, false>), grid, block, lds, st, X, P, logC);
 }
 
 template <int LOGM>
@@ -347,9 +347,9 @@ void row_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (sizeof(uint2) << (LOGM - 4 + logG));
     if (forward)
-        hipLaunchKernelGGL((k_ntt_row_reg<LOGM, true>), grid, block, lds, st, X, P, logG);
+        launch((k_ntt_row_reg<LOGM, true>), grid, block, lds, st, X, P, logG);
     else
-        hipLaunchKernelGGL((k_ntt_row_reg<LOGM, false>), grid, block, lds, st, X, P, logG);
+        launch((k_ntt_row_reg<LOGM, false>), grid, block, lds, st, X, P, logG);
 }
 
 template <int LOGM>
@@ -359,7 +359,7 @@ void fused_launch(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t 
     const dim3 block(1u << (logG + LOGM - 4));
     const size_t lds = 2 * (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (2 * sizeof(uint2) << (LOGM - 4 + logG));
-    hipLaunchKernelGGL((k_ntt_rows_fused_reg<LOGM>), grid, block, lds, st, A, B, P, logG);
+    launch((k_ntt_rows_fused_reg<LOGM>), grid, block, lds, st, A, B, P, logG);
 }
 
 }  // namespace

@@ -48,13 +48,16 @@ struct pm_plan {
     uint32_t* B = nullptr;               // K x N residues of b
     pm::HostPipeline* pipe = nullptr;    // host-pointer calls (created on first use)
     hipStream_t stream = nullptr;
-    hipEvent_t ev[PM_NPHASES + 1] = {};
+    hipEvent_t ev[PM_NPHASES + 1] = {};    // kernel start events of the phase profile, and spares
+    hipEvent_t ev_stop[PM_NPHASES] = {};   // kernel end events of the phase profile
     bool mfma_ready = false;
 
     ~pm_plan() {
         if (stream) (void)hipStreamSynchronize(stream);
         pm::destroy_pipeline(pipe);
         for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto e : ev_stop)
             if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
         for (void* p : owned) (void)hipFree(p);
@@ -86,9 +89,11 @@ struct pm_plan {
 
 namespace pm {
 
-// The five phases of one product on device pointers, enqueued on st (pm_api.cpp).
-void run_device(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st,
-                hipEvent_t* ev = nullptr);
+// The phases of one product on device pointers, enqueued on st (pm_api.cpp):
+// decompose a, decompose b, forward columns of A and of B, fused rows, inverse
+// columns, CRT. run_phase enqueues phase i alone.
+void run_device(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st);
+void run_phase(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st, int i);
 
 // fherma_run on host pointers through the plan's pipeline (pm_host.cpp).
 void host_multiply(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c);

@@ -119,8 +119,9 @@ int pm_stage_crt(pm_plan* plan, const uint32_t* res, uint32_t* c, void* stream);
 /* --- timing (HIP events on the stream the kernels run on) -------------------
  * pm_time_multiply_device: mean milliseconds per product over `iters` calls
  * after `warmup` untimed ones.
- * pm_profile_phases: mean milliseconds per phase over `iters` products, for
- * PM_NPHASES phases named by pm_phase_name(). */
+ * pm_profile_phases: mean milliseconds per launch of each of the PM_NPHASES
+ * phases (named by pm_phase_name()), each phase's kernels run `iters` times back
+ * to back between two events on the plan's stream; c holds the product after. */
 #define PM_NPHASES 7
 int pm_time_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, int warmup,
                             int iters, double* ms_per_product, void* stream);
