@@ -202,6 +202,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (PCIe-inclusive) rate")
     ap.add_argument("--phases-only", action="store_true",
                     help="only the per-kernel HIP-event profile (one product at a time on one stream), so that "
                          "a kernel trace of this run holds no concurrently running kernels")
@@ -246,6 +247,26 @@ def main():
     else:
         value, sec, plain_value, mismatches = batch_rates(args, plans, da, db, dc, gold, log)
 
+    # Host buffers in and out (fherma_run's path: PCIe transfers included), after
+    # the timed choice of the host mode on the first plan. Never the headline value.
+    host = None
+    if not (args.no_host or args.phases_only):
+        mode, _ = plans[0].tune_host()
+        for p in plans[1:]:
+            p.set_host_mode(mode)
+        for p, (_, a, b) in zip(plans, points):
+            p.multiply(a, b)
+        t0 = time.perf_counter()
+        for _ in range(2):
+            for p, (_, a, b) in zip(plans, points):
+                p.multiply(a, b)
+        host_sec = (time.perf_counter() - t0) / (2 * BATCH)
+        host = {"value": round(1.0 / host_sec, 2), "unit": "polymuls/s",
+                "mode": {polymult.HOST_STAGED: "staged", polymult.HOST_MAPPED: "mapped",
+                         polymult.HOST_DIRECT: "direct"}.get(mode, str(mode)),
+                "note": "numpy arrays in and out through the plan's host pipeline, products one after another"}
+        log("host buffers: %.3f ms per product (%s)" % (host_sec * 1e3, host["mode"]))
+
     # per-phase HIP-event timings on the plan's stream (mean over the batch)
     phases = {}
     for p, a, b, c in zip(plans, da, db, dc):
@@ -275,6 +296,7 @@ def main():
         "mismatches": int(mismatches),
         "effective_GBps_per_product_yardstick": None if value is None else round(prod_bytes * value / 1e9, 1),
         "phases_ms": {k: round(v, 5) for k, v in phases.items()},
+        "host_inclusive": host,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
