@@ -203,6 +203,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (PCIe-inclusive) rate")
+    ap.add_argument("--phase-iters", type=int, default=20,
+                    help="back-to-back launches of each phase's kernel in the per-phase timing")
     ap.add_argument("--phases-only", action="store_true",
                     help="only the per-kernel HIP-event profile (one product at a time on one stream), so that "
                          "a kernel trace of this run holds no concurrently running kernels")
@@ -270,7 +272,7 @@ def main():
     # per-phase HIP-event timings on the plan's stream (mean over the batch)
     phases = {}
     for p, a, b, c in zip(plans, da, db, dc):
-        for k, v in p.profile_phases(a, b, c, 20).items():
+        for k, v in p.profile_phases(a, b, c, args.phase_iters).items():
             phases[k] = phases.get(k, 0.0) + v / BATCH
     log("phases (ms): " + ", ".join("%s %.4f" % kv for kv in phases.items()))
     nbytes = phase_bytes(K, L)

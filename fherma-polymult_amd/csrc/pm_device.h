@@ -50,9 +50,9 @@ struct DevPlan {
     // K: 128 sum_t (256^t mod p_j) mod p_j plus the least multiple of p_j that
     // is >= 2^47, so that every decompose sum is positive before its reduction
     const double* dec_corr;
-    // CRT digit table: row s (0..4L) holds byte s of the balanced digits of
-    // M_j for primes j = 4J .. 4J + 3 in word J (zero for j >= K); rows of
-    // crt_kjs words, crt_kjs = 2 (mod 4) and >= ceil(K / 4).
+    // CRT digit table, column-major: word J kTabRows + R (pm_mfma.hip) holds
+    // byte R - 3 of the balanced digits of M_j for primes j = 4J .. 4J + 3
+    // (zero for j >= K and outside digit rows 0 .. 4L).
     const uint32_t* crt_tab;
     // CRT constants, zero-padded so that every index the kernel forms is in
     // range: crt_fc = 1 / p_j (160 entries), then M_j / q (160); crt_lc = q,
@@ -63,7 +63,7 @@ struct DevPlan {
     int dec_ksteps;           // ceil(4L / 64)
     int crt_ksteps;           // ceil(4K / 64)
     int crt_stiles;           // ceil((4L + 4) / 16)
-    int crt_kjs;
+    int pad2;
 };
 
 // Every kernel of the library is launched through this one function.

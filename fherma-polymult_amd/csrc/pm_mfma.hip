@@ -27,9 +27,9 @@
 //     sum_j y_j M_j = sum_s 256^s G[s][n] + 0x80808080 sum_j M_j   (exactly)
 //   MFMA rows are byte positions s, columns coefficients, K-dim (prime, byte).
 //   A lane's four rows are the four bytes of one 32-bit limb, so it folds them
-//   into one signed 64-bit limb partial, written out limb-major; a second kernel
-//   runs the carry chain, + negvP_v and the reduction mod q, one coefficient per
-//   thread. v = round(sum_j y_j / p_j) and the FP64 quotient estimate's sum are
+//   into one signed 64-bit limb partial; the same kernel then runs the carry
+//   chain, adds v (-P mod q) and reduces mod q (k_crt_mfma below).
+//   v = round(sum_j y_j / p_j) and the FP64 quotient estimate's sum are
 //   accumulated while the y bytes are loaded.
 //
 // MFMA operand layout (16x16x64 i8): lane l supplies row (A) / column (B)
@@ -205,36 +205,48 @@ k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
 // K-dim order: K-step ks, lane group g, fragment word a, byte r is prime
 // 4 (4 ks + g) + r, byte a of y; the A word for row s is then word J = 4 ks + g
 // of digit-table row s - a.
-constexpr int kYRow = 68;         // 64 coefficients + 4: lane groups g land on alternate bank halves
-constexpr int kTabLoads = 11;     // 16-byte loads per thread for the digit table: (4 64 + 2) 42 words
-constexpr int kSegMax = 17;       // limbs per segment: 4 x 17 >= L + 2 for L <= 64
+constexpr int kTabRows = 16 * 17 + 3;   // digit rows -3 .. 271 of the transposed table
+constexpr int kTabLoads = 11;           // 16-byte loads per thread for the table: 40 x 275 words
+constexpr int kSegMax = 17;             // limbs per segment: 4 x 17 >= L + 2 for L <= 64
 
-// 4 L + 1 digit rows and one zero row (the target of out-of-range rows), whole 16-byte units
-__host__ __device__ constexpr uint32_t crt_tab_words(int L, int kjs) {
-    return (static_cast<uint32_t>(4 * L + 2) * kjs + 3u) & ~3u;
+// Digit table words, column-major: word (J, R) at J kTabRows + R holds byte R - 3
+// of the balanced digits of M_4J .. M_4J+3 (zero outside 0 .. 4L); whole 16-byte units.
+__host__ __device__ constexpr uint32_t crt_tab_words(int K) {
+    return (static_cast<uint32_t>((K + 3) / 4) * kTabRows + 3u) & ~3u;
 }
 
-// LDS words: region 0 holds the y digits (phases 1-2), then the limb partials
-// (int64, 4 ST x 64) and the segment exports (1280 words), then the output
-// stage; the digit table; 1024 words of FP64 partial sums, reused for the
-// segment comparisons; 192 words of (v, Ql, Qh).
-__host__ __device__ constexpr uint32_t crt_region0_words(int KS, int ST) {
-    return static_cast<uint32_t>(16 * KS * kYRow) > static_cast<uint32_t>(512 * ST + 1280)
-               ? static_cast<uint32_t>(16 * KS * kYRow)
-               : static_cast<uint32_t>(512 * ST + 1280);
+// LDS words, three regions:
+//   R0  y digits [4 KS][64] as 16-byte words (phases 1-2), then the limb
+//       partials (int64, [4 ST][64]), then the output stage (64 x (L + 1));
+//   R1  the digit table (phases 1-2), then the segment exports (1280 words);
+//   aux FP64 partial sums (1024 words), reused for the segment comparisons,
+//       and (v, Ql, Qh) per coefficient (192 words).
+// A-fragment reads for prime groups past the table (J < 4 KS) stay inside
+// R1 + aux: 3 kTabRows < 1216.
+__host__ __device__ constexpr uint32_t crt_r0_words(int KS, int ST) {
+    return static_cast<uint32_t>(1024 * KS) > static_cast<uint32_t>(512 * ST) ? static_cast<uint32_t>(1024 * KS)
+                                                                                : static_cast<uint32_t>(512 * ST);
 }
 
-__host__ __device__ constexpr uint32_t crt_lds_words(int KS, int ST, int L, int kjs) {
-    return crt_region0_words(KS, ST) + crt_tab_words(L, kjs) + 1024 + 192;
+__host__ __device__ constexpr uint32_t crt_r1_words(int K) {
+    return crt_tab_words(K) > 1280u ? crt_tab_words(K) : 1280u;
 }
 
+__host__ __device__ constexpr uint32_t crt_lds_words(int KS, int ST, int K) {
+    return crt_r0_words(KS, ST) + crt_r1_words(K) + 1024 + 192;
+}
+
+// d[a] = byte a of w0, w1, w2, w3 (in that byte order), by byte permutes:
+// v_perm_b32 picks bytes 0-3 from its second operand and 4-7 from its first.
 __device__ __forceinline__ void transpose_bytes(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t (&d)[4]) {
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const int sh = 8 * a;
-        d[a] = ((w0 >> sh) & 0xffu) | (((w1 >> sh) & 0xffu) << 8) | (((w2 >> sh) & 0xffu) << 16) |
-               (((w3 >> sh) & 0xffu) << 24);
-    }
+    const uint32_t lo01 = __builtin_amdgcn_perm(w1, w0, 0x05010400u);   // w0.b0 w1.b0 w0.b1 w1.b1
+    const uint32_t hi01 = __builtin_amdgcn_perm(w1, w0, 0x07030602u);   // w0.b2 w1.b2 w0.b3 w1.b3
+    const uint32_t lo23 = __builtin_amdgcn_perm(w3, w2, 0x05010400u);
+    const uint32_t hi23 = __builtin_amdgcn_perm(w3, w2, 0x07030602u);
+    d[0] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
+    d[1] = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
+    d[2] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
+    d[3] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
 }
 
 __device__ __forceinline__ long long fold_limb(const v4i& a) {
@@ -256,16 +268,16 @@ __device__ __forceinline__ long long seg_carry(long long C, long long cout, uint
 __global__ void __launch_bounds__(64 * kWaves)
 k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, uint32_t k0, uint32_t count) {
     extern __shared__ uint32_t smem[];
-    const int K = P.K, L = P.L, KS = P.crt_ksteps, ST = P.crt_stiles, KJS = P.crt_kjs;
-    const int KJ = (K + 3) >> 2, nd = 4 * L + 1;
-    const uint32_t tab_words = crt_tab_words(L, KJS);
-    uint32_t* ydig = smem;                                                     // 16 KS rows of kYRow words
+    const int K = P.K, L = P.L, KS = P.crt_ksteps, ST = P.crt_stiles;
+    const int KJ = (K + 3) >> 2;
+    const uint32_t r0 = crt_r0_words(KS, ST), r1 = crt_r1_words(K);
+    uint4* yq = reinterpret_cast<uint4*>(smem);                                // [4 KS][64]
     long long* part = reinterpret_cast<long long*>(smem);                      // after phase 2: [limb][64]
-    long long* ex64 = part + 256 * ST;                                         // segment exports
-    uint32_t* ex32 = reinterpret_cast<uint32_t*>(ex64 + 512);
     uint32_t* stage = smem;                                                    // after phase 3: 64 x (L + 1)
-    uint32_t* tab = smem + crt_region0_words(KS, ST);                          // nd rows of KJS words
-    double* fr = reinterpret_cast<double*>(tab + tab_words);                   // 4 x 64
+    uint32_t* tab = smem + r0;                                                 // [KJ][kTabRows]
+    long long* ex64 = reinterpret_cast<long long*>(tab);                       // phase 3: segment exports
+    uint32_t* ex32 = tab + 1024;
+    double* fr = reinterpret_cast<double*>(tab + r1);                          // 4 x 64
     double* fq = fr + 4 * 64;                                                  // 4 x 64
     int* cmpw = reinterpret_cast<int*>(fr);                                    // phase 3: 4 x 64
     uint32_t* vq = reinterpret_cast<uint32_t*>(fq + 4 * 64);                   // 3 x 64: v, Ql, Qh
@@ -277,7 +289,7 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
     // 1. digit table and y words: every load of the thread in flight at once
     {
         const uint4* tab4 = reinterpret_cast<const uint4*>(P.crt_tab);
-        const uint32_t tab4n = tab_words / 4;
+        const uint32_t tab4n = crt_tab_words(K) / 4;
         uint4 tv[kTabLoads];
 #pragma unroll
         for (int r = 0; r < kTabLoads; ++r) tv[r] = tab4[min(threadIdx.x + r * blockDim.x, tab4n - 1)];
@@ -310,10 +322,11 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
             }
             uint32_t d[4];
             transpose_bytes(w[0], w[1], w[2], w[3], d);
-            // groups past the last prime hold zeros, so any A word times them is 0
-            const uint32_t off = J < KJ ? 0x80808080u : 0u;
-#pragma unroll
-            for (int a = 0; a < 4; ++a) ydig[(4 * J + a) * kYRow + lane] = J < KJ ? d[a] ^ off : 0u;
+            // B word k is byte 3 - k (the A word k is digit row s - 3 + k); groups
+            // past the last prime hold zeros, so any A word times them is 0
+            const uint32_t keep = J < KJ ? 0xffffffffu : 0u, bias = keep & 0x80808080u;
+            yq[J * 64 + lane] = make_uint4((d[3] & keep) ^ bias, (d[2] & keep) ^ bias, (d[1] & keep) ^ bias,
+                                           (d[0] & keep) ^ bias);
         }
         fr[wave * 64 + lane] = f;
         fq[wave * 64 + lane] = f2;
@@ -330,66 +343,55 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
         vq[128 + lane] = static_cast<uint32_t>(Q >> 32);
     }
 
-    // 2. limb partials
+    // 2. limb partials: tiles st = wave + 4 i (i < 4) for all four column tiles,
+    // and the 17th limb tile (st = 16) for column tile t = wave. Tiles past ST
+    // read zero table rows.
     const int g = lane >> 4, cl = lane & 15;
-    long long p0[4][kTiles], p1[kTiles];
+    long long p0[4][kTiles], p1;
     {
-        v4i acc[4][kTiles];
+        v4i acc[4][kTiles], acc1 = v4i{0, 0, 0, 0};
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int t = 0; t < kTiles; ++t) acc[i][t] = v4i{0, 0, 0, 0};
-        v4i acc1[kTiles];
+        const uint32_t* ta = tab + g * kTabRows + 16 * wave + cl;   // slot i at + 64 i, word k at + k
+        const uint32_t* ta4 = tab + g * kTabRows + 256 + cl;
+        const uint4* yb = yq + g * 64 + cl;                          // column tile t at + 16 t
+        const uint4* yx = yq + g * 64 + 16 * wave + cl;
+        // unrolled: the accumulators keep fixed registers, the reads immediate offsets
 #pragma unroll
-        for (int t = 0; t < kTiles; ++t) acc1[t] = v4i{0, 0, 0, 0};
-        const bool has1 = wave == 0 && ST > 16;
-        // A word (tile slot i, byte a) of K-step ks is tab[base[i][a] + 4 ks]: rows
-        // outside the digits point at the zero row. Columns past KJS fall into
-        // the next row (or past the table, still inside LDS) and meet zero B rows.
-        uint32_t base[5][4];
+        for (int ks = 0; ks < kCrtMaxKSteps; ++ks) {
+            if (ks >= KS) continue;
+            const uint32_t* tk = ta + 4 * ks * kTabRows;
+            const uint32_t* tk4 = ta4 + 4 * ks * kTabRows;
+            const uint4* yk = yb + 4 * ks * 64;
+            v4i af[4], a4, bf[kTiles], bx;
 #pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            const int s = 16 * (i < 4 ? wave + 4 * i : 16) + cl;
+            for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const int row = s - a;
-                base[i][a] = static_cast<uint32_t>((row >= 0 && row < nd ? row : nd) * KJS + g);
-            }
-        }
-#pragma unroll 1
-        for (int ks = 0; ks < KS; ++ks) {
-            v4i bf[kTiles];
-            const uint32_t* yr = ydig + (16 * ks + 4 * g) * kYRow + cl;
+                for (int k = 0; k < 4; ++k) af[i][k] = static_cast<int>(tk[64 * i + k]);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) a4[k] = static_cast<int>(tk4[k]);
 #pragma unroll
             for (int t = 0; t < kTiles; ++t) {
-                bf[t][0] = static_cast<int>(yr[16 * t]);
-                bf[t][1] = static_cast<int>(yr[kYRow + 16 * t]);
-                bf[t][2] = static_cast<int>(yr[2 * kYRow + 16 * t]);
-                bf[t][3] = static_cast<int>(yr[3 * kYRow + 16 * t]);
+                const uint4 u = yk[16 * t];
+                bf[t] = v4i{static_cast<int>(u.x), static_cast<int>(u.y), static_cast<int>(u.z), static_cast<int>(u.w)};
             }
-            // A fragments of the wave's (up to) five limb tiles, all read before the
-            // MFMAs; tiles past ST read the zero row
-            v4i af[5];
-            const uint32_t* tk = tab + 4 * ks;
-#pragma unroll
-            for (int i = 0; i < 5; ++i)
-#pragma unroll
-                for (int a = 0; a < 4; ++a) af[i][a] = static_cast<int>(tk[base[i][a]]);
+            {
+                const uint4 u = yx[4 * ks * 64];
+                bx = v4i{static_cast<int>(u.x), static_cast<int>(u.y), static_cast<int>(u.z), static_cast<int>(u.w)};
+            }
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
                 for (int t = 0; t < kTiles; ++t) acc[i][t] = mfma16(af[i], bf[t], acc[i][t]);
-            if (has1) {
-#pragma unroll
-                for (int t = 0; t < kTiles; ++t) acc1[t] = mfma16(af[4], bf[t], acc1[t]);
-            }
+            acc1 = mfma16(a4, bx, acc1);
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int t = 0; t < kTiles; ++t) p0[i][t] = fold_limb(acc[i][t]);
-#pragma unroll
-        for (int t = 0; t < kTiles; ++t) p1[t] = fold_limb(acc1[t]);
+        p1 = fold_limb(acc1);
     }
     __syncthreads();   // y digits and the digit table are dead from here
     // lane (g, cl), tile st, column tile t: limb 4 st + g of coefficient 16 t + cl
@@ -401,10 +403,7 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
             for (int t = 0; t < kTiles; ++t) part[(4 * st + g) * 64 + 16 * t + cl] = p0[i][t];
         }
     }
-    if (wave == 0 && ST > 16) {
-#pragma unroll
-        for (int t = 0; t < kTiles; ++t) part[(64 + g) * 64 + 16 * t + cl] = p1[t];
-    }
+    if (ST > 16) part[(64 + g) * 64 + 16 * wave + cl] = p1;
     __syncthreads();
 
     // 3. one segment of 17 limbs per wave, one coefficient per lane; limbs past
@@ -526,12 +525,11 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
     for (int u = 0; u < kSegMax; ++u)
         if (l0 + u < L) stage[lane * (L + 1) + l0 + u] = x[u];
     __syncthreads();
+    // one coefficient's L words per wave and step (lane = limb)
     uint32_t* dst = c + static_cast<size_t>(nb) * L;
-    const uint32_t total = nk * static_cast<uint32_t>(L);
+    if (lane < L) {
 #pragma unroll 4
-    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
-        const uint32_t r = i / L;
-        dst[i] = stage[r * (L + 1) + (i - r * L)];
+        for (uint32_t r = wave; r < nk; r += kWaves) dst[r * L + lane] = stage[r * (L + 1) + lane];
     }
 }
 
@@ -590,13 +588,14 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
     for (int j = 0; j < K; ++j) balanced_digits(h.M.data() + static_cast<size_t>(j) * L, L, m.data() + static_cast<size_t>(j) * nd, nd);
     if (ST > 17) throw std::logic_error("CRT limb tiles exceed the kernel's 17");
     const int KJ = (K + 3) / 4;
-    const int kjs = KJ + ((2 - KJ % 4) + 4) % 4;   // = 2 (mod 4): conflict-free A-fragment reads
-    if (crt_tab_words(L, kjs) > 4u * kTabLoads * 64 * kWaves) throw std::logic_error("CRT digit table too large");
-    std::vector<uint32_t> tab(crt_tab_words(L, kjs), 0);
-    for (int sr = 0; sr < nd; ++sr)
-        for (int j = 0; j < K; ++j)
-            tab[static_cast<size_t>(sr) * kjs + j / 4] |=
-                static_cast<uint32_t>(static_cast<uint8_t>(m[static_cast<size_t>(j) * nd + sr])) << (8 * (j % 4));
+    if (crt_tab_words(K) > 4u * kTabLoads * 64 * kWaves) throw std::logic_error("CRT digit table too large");
+    std::vector<uint32_t> tab(crt_tab_words(K), 0);
+    for (int J = 0; J < KJ; ++J)
+        for (int R = 3; R < kTabRows && R - 3 < nd; ++R)
+            for (int r = 0; r < 4 && 4 * J + r < K; ++r)
+                tab[static_cast<size_t>(J) * kTabRows + R] |=
+                    static_cast<uint32_t>(static_cast<uint8_t>(m[static_cast<size_t>(4 * J + r) * nd + (R - 3)]))
+                    << (8 * r);
     // 0x80808080 * sum_j M_j, exactly (L + 2 limbs).
     std::vector<uint64_t> acc(L + 2, 0);
     for (int j = 0; j < K; ++j)
@@ -634,7 +633,6 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
     d.crt_corr = static_cast<const uint32_t*>(upload(ccorr.data(), ccorr.size() * sizeof(uint32_t)));
     d.crt_ksteps = KS;
     d.crt_stiles = ST;
-    d.crt_kjs = kjs;
 }
 
 void launch_decompose_mfma(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st) {
@@ -665,7 +663,7 @@ void launch_crt_mfma_range(const DevPlan& P, const uint32_t* Y, uint32_t* c, uin
     if (!P.crt_tab) throw std::runtime_error("matrix-core tables not prepared");
     if (count == 0) return;
     const dim3 grid((count + 63) / 64);
-    const size_t lds = sizeof(uint32_t) * crt_lds_words(P.crt_ksteps, P.crt_stiles, P.L, P.crt_kjs);
+    const size_t lds = sizeof(uint32_t) * crt_lds_words(P.crt_ksteps, P.crt_stiles, P.K);
     launch(k_crt_mfma, grid, dim3(64 * kWaves), lds, st, Y, c, P, k0, count);
 }
 
