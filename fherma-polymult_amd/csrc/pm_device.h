@@ -63,7 +63,7 @@ struct DevPlan {
     int dec_ksteps;           // ceil(4L / 64)
     int crt_ksteps;           // ceil(4K / 64)
     int crt_stiles;           // ceil((4L + 4) / 16)
-    int pad2;
+    uint32_t dec_magic;       // floor(2^32 / (L / 4)) + 1: i / (L / 4) = mulhi(i, dec_magic) for i < 2^12
 };
 
 // Every kernel of the library is launched through this one function.

@@ -119,7 +119,7 @@ __device__ __forceinline__ void tile_stages(uint32_t* tile, const uint2* __restr
                 tile[hi] = sub_mod(x, v, p);
             } else {
                 tile[lo] = add_mod(x, y, p);
-                tile[hi] = mul_shoup(sub_mod(x, y, p), w.x, w.y, p);
+                tile[hi] = mul_shoup(x - y + p, w.x, w.y, p);   // (0, 2p): no reduction needed first
             }
         }
         __syncthreads();

@@ -37,10 +37,12 @@
 // the result D[i][j] sits in lane 16 (i / 4) + j, element i % 4. The K order
 // inside a lane only has to agree between A and B, which it does by
 // construction.
+#include "pm_arith.h"
 #include "pm_bigint.h"
 #include "pm_device.h"
 #include "pm_mfma_host.h"
 
+#include <algorithm>
 #include <stdexcept>
 #include <vector>
 
@@ -93,9 +95,32 @@ k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
     const uint32_t nk = min(static_cast<uint32_t>(16 * kTiles), k0 + count - nb);
     const uint32_t* src = limbs + static_cast<size_t>(nb) * L;
     constexpr uint32_t kSlots = 16u * kTiles * Ls;
-    if (nk == 16u * kTiles) {
-        // A full tile: its 64 L words are contiguous and 16-byte aligned (nb is a
-        // multiple of 64). At most KT 16-byte loads per thread, all in flight.
+    if (nk == 16u * kTiles && (L & 3) == 0 && L >= 8) {
+        // A full tile of whole 16-byte units per row: its 64 L words are
+        // contiguous and 16-byte aligned (nb is a multiple of 64), and each unit
+        // lands in one LDS row (Ls is a multiple of 4). All loads in flight.
+        const uint32_t q4 = static_cast<uint32_t>(L) / 4, units = 16u * kTiles * q4;
+        const uint4* src4 = reinterpret_cast<const uint4*>(src);
+        uint4 v[KT];
+#pragma unroll
+        for (int r = 0; r < KT; ++r) v[r] = src4[min(threadIdx.x + r * blockDim.x, units - 1)];
+#pragma unroll
+        for (int r = 0; r < KT; ++r) {
+            const uint32_t i4 = threadIdx.x + r * blockDim.x;
+            if (i4 < units) {
+                const uint32_t row = mulhi32(i4, P.dec_magic), c4 = i4 - row * q4;
+                *reinterpret_cast<uint4*>(xs + row * Ls + 4 * c4) =
+                    make_uint4(v[r].x ^ 0x80808080u, v[r].y ^ 0x80808080u, v[r].z ^ 0x80808080u, v[r].w ^ 0x80808080u);
+            }
+        }
+        // digits of the padding words past L: byte 0
+        const uint32_t pad = Ls - L;
+        for (uint32_t i = threadIdx.x; i < 16u * kTiles * pad; i += blockDim.x) {
+            const uint32_t row = i / pad;
+            xs[row * Ls + L + (i - row * pad)] = 0x80808080u;
+        }
+    } else if (nk == 16u * kTiles) {
+        // A full tile, L not a multiple of 4: at most KT 16-byte loads per thread.
         const uint32_t words = 16u * kTiles * static_cast<uint32_t>(L);
         const uint4* src4 = reinterpret_cast<const uint4*>(src);
         uint4 v[KT];
@@ -579,6 +604,8 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
     d.dec_digits = static_cast<const int8_t*>(upload(dec.data(), dec.size()));
     d.dec_corr = static_cast<const double*>(upload(dcorr.data(), dcorr.size() * sizeof(double)));
     d.dec_ksteps = KT;
+    // used only for L >= 8, so L / 4 >= 2 and the magic fits 32 bits
+    d.dec_magic = static_cast<uint32_t>((uint64_t(1) << 32) / static_cast<uint64_t>(std::max(L / 4, 2)) + 1);
 
     // CRT tables.
     const int KS = (4 * K + 63) / 64;

@@ -34,10 +34,11 @@ __device__ __forceinline__ void ct(uint32_t& x, uint32_t& y, const uint2 w, uint
     x = add_mod(x, v, p);
 }
 
+// x - y + p lies in (0, 2p), below 2^32, which the Shoup product accepts as is.
 __device__ __forceinline__ void gs(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
     const uint32_t u = x;
     x = add_mod(u, y, p);
-    y = mul_shoup(sub_mod(u, y, p), w.x, w.y, p);
+    y = mul_shoup(u - y + p, w.x, w.y, p);
 }
 
 template <bool kFwd>
