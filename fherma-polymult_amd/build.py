@@ -17,9 +17,12 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 # Packed FP32 instructions are switched off for all device code: the kernels
-# use none on purpose, and the compiler must not form them either.
+# use none on purpose, and the compiler must not form them either. The matrix
+# cores accumulate in ordinary VGPRs (gfx950 allows it): the MFMA kernels'
+# epilogues then read their sums without accumulator-register copies.
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-mllvm", "-amdgpu-mfma-vgpr-form",
          "-Wall", "-Wno-unused-function", "-I" + os.path.join(HERE, "..", "include")]
 
 
