@@ -51,10 +51,14 @@ __device__ __forceinline__ void bfly(uint32_t& x, uint32_t& y, const uint2 w, ui
 
 // Twiddle sources: tw(e, b) is the twiddle of block b (counted from the start
 // of the sub-transform) of the stage with distance 2^e.
+// tw.b(e, t, blk): phase B's twiddle for thread t, block blk of its 16 elements.
 struct TableTw {   // the full K x N table: index (B >> (e + 1)) + b
     const uint2* __restrict__ tab;
     uint32_t B;
     __device__ __forceinline__ uint2 operator()(int e, uint32_t b) const { return tab[(B >> (e + 1)) + b]; }
+    __device__ __forceinline__ uint2 b(int e, uint32_t t, int blk) const {
+        return (*this)(e, (t << (3 - e)) + blk);
+    }
 };
 
 // w = Z * Om mod p and its Shoup companion floor(w 2^32 / p), from
@@ -72,6 +76,9 @@ struct GenTw {     // row rho: Z[rho][e] * Om[b]
     const uint32_t* __restrict__ om;
     PrimeConst pc;
     __device__ __forceinline__ uint2 operator()(int e, uint32_t b) const { return make_twiddle(om[b], z[e], pc); }
+    __device__ __forceinline__ uint2 b(int e, uint32_t t, int blk) const {
+        return (*this)(e, (t << (3 - e)) + blk);
+    }
 };
 
 // Phase A twiddles of one row, precomputed in LDS: stage e, block b at
@@ -125,7 +132,7 @@ __device__ __forceinline__ void phase_b(uint32_t (&x)[16], uint32_t (&y)[16], co
         const int d = 1 << e;
 #pragma unroll
         for (int blk = 0; blk < 8 / d; ++blk) {
-            const uint2 w = tw(e, (t << (3 - e)) + blk);
+            const uint2 w = tw.b(e, t, blk);
 #pragma unroll
             for (int k = blk * 2 * d; k < blk * 2 * d + d; ++k) {
                 bfly<kFwd>(x[k], x[k + d], w, p);
@@ -137,52 +144,98 @@ __device__ __forceinline__ void phase_b(uint32_t (&x)[16], uint32_t (&y)[16], co
 
 __device__ __forceinline__ uint32_t padded(uint32_t u) { return u + (u >> 4); }
 
-// Column pass: C = 2^logC adjacent columns of one prime, R = M rows each.
-template <int LOGM, bool kFwd>
+// A raw buffer over `bytes` bytes from `base` (gfx9 resource word 3): a lane's
+// address is its byte offset plus a wave-uniform offset in a scalar register,
+// so strided accesses cost no per-access address arithmetic, and accesses past
+// `bytes` read 0 / are dropped by the hardware.
+struct Buf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ Buf(const void* base, uint32_t bytes)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000)) {}
+    __device__ __forceinline__ uint32_t ld(uint32_t voff, uint32_t soff) const {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+    }
+    __device__ __forceinline__ void st(uint32_t v, uint32_t voff, uint32_t soff) const {
+        __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, 0);
+    }
+    __device__ __forceinline__ uint2 ld2(uint32_t voff, uint32_t soff) const {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        return make_uint2(v[0], v[1]);
+    }
+};
+
+// The column pass's twiddles from the table through a buffer: phase A's are
+// wave-uniform (scalar loads), phase B's index (B >> (e + 1)) + (t << (3 - e)) + blk
+// splits into a lane part (t << (3 - e), four values fixed per thread) and a
+// uniform part.
+struct BufTw {
+    const uint2* __restrict__ tab;
+    Buf buf;
+    uint32_t B, vt[4];
+    __device__ __forceinline__ BufTw(const uint2* tab_, uint32_t N, uint32_t B_, uint32_t t)
+        : tab(tab_), buf(tab_, 8u * N), B(B_) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) vt[e] = 8u * (t << (3 - e));
+    }
+    __device__ __forceinline__ uint2 operator()(int e, uint32_t b) const { return tab[(B >> (e + 1)) + b]; }
+    __device__ __forceinline__ uint2 b(int e, uint32_t, int blk) const {
+        return buf.ld2(vt[e], 8u * ((B >> (e + 1)) + blk));
+    }
+};
+
+// Column pass: C = 2^LOGC adjacent columns of one prime, R = M rows each. LDS
+// slot padded(u) of column c is at padded(u) C + c; with t < TPS and
+// TPS NG = 16, padded(t + TPS i + 16 k) = t + TPS i + 17 k and
+// padded(16 t + k) = 17 t + k, so every slot is a lane base plus a constant.
+template <int LOGM, bool kFwd, int LOGC>
 __global__ void __launch_bounds__(512)
-k_ntt_col_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logC) {
+k_ntt_col_reg(uint32_t* __restrict__ X, DevPlan P) {
     extern __shared__ uint32_t lds[];
-    constexpr uint32_t M = 1u << LOGM, TPS = M / 16, GA = TPS, NG = 16 / GA;
-    const uint32_t C = 1u << logC;
-    const uint32_t c = threadIdx.x & (C - 1), t = threadIdx.x >> logC;
+    constexpr uint32_t M = 1u << LOGM, TPS = M / 16, GA = TPS, NG = 16 / GA, C = 1u << LOGC;
+    const uint32_t c = threadIdx.x & (C - 1), t = threadIdx.x >> LOGC;
     const int j = blockIdx.y;
-    const size_t S = size_t(1) << P.logS;
-    uint32_t* x = X + static_cast<size_t>(j) * P.N + (blockIdx.x << logC) + c;
-    const TableTw tw{(kFwd ? P.tw : P.itw) + static_cast<size_t>(j) * P.N, M};
+    const uint32_t S = 1u << P.logS;
+    // element (row u, this column) at byte 4 (u S + column): lane part + uniform part
+    const Buf x(X + static_cast<size_t>(j) * P.N, 4u * P.N);
+    const uint32_t col = (blockIdx.x << LOGC) + c;
+    const uint32_t va = 4u * (t * S + col), vb = 4u * (16u * t * S + col);   // phase A / phase B rows
+    const BufTw tw((kFwd ? P.tw : P.itw) + static_cast<size_t>(j) * P.N, P.N, M, t);
+    uint32_t* la = lds + t * C + c;          // phase A slots: + (TPS i + 17 k) C
+    uint32_t* lb = lds + 17u * t * C + c;    // phase B slots: + k C
     const uint32_t p = P.pc[j].p;
     uint32_t r[16];
     if (kFwd) {
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
-            for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = x[(t + TPS * i + 16 * k) * S];
+            for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = x.ld(va, 4u * (TPS * i + 16 * k) * S);
         phase_a<LOGM, true, false>(r, r, tw, p);
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
-            for (uint32_t k = 0; k < GA; ++k) lds[padded(t + TPS * i + 16 * k) * C + c] = r[i * GA + k];
+            for (uint32_t k = 0; k < GA; ++k) la[(TPS * i + 17 * k) * C] = r[i * GA + k];
         __syncthreads();
 #pragma unroll
-        for (uint32_t k = 0; k < 16; ++k) r[k] = lds[padded(16 * t + k) * C + c];
+        for (uint32_t k = 0; k < 16; ++k) r[k] = lb[k * C];
         phase_b<true, false>(r, r, tw, t, p);
 #pragma unroll
-        for (uint32_t k = 0; k < 16; ++k) x[(16 * t + k) * S] = r[k];
+        for (uint32_t k = 0; k < 16; ++k) x.st(r[k], vb, 4u * k * S);
     } else {
 #pragma unroll
-        for (uint32_t k = 0; k < 16; ++k) r[k] = x[(16 * t + k) * S];
+        for (uint32_t k = 0; k < 16; ++k) r[k] = x.ld(vb, 4u * k * S);
         phase_b<false, false>(r, r, tw, t, p);
 #pragma unroll
-        for (uint32_t k = 0; k < 16; ++k) lds[padded(16 * t + k) * C + c] = r[k];
+        for (uint32_t k = 0; k < 16; ++k) lb[k * C] = r[k];
         __syncthreads();
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
-            for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = lds[padded(t + TPS * i + 16 * k) * C + c];
+            for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = la[(TPS * i + 17 * k) * C];
         phase_a<LOGM, false, false>(r, r, tw, p);
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
-            for (uint32_t k = 0; k < GA; ++k) x[(t + TPS * i + 16 * k) * S] = r[i * GA + k];
+            for (uint32_t k = 0; k < GA; ++k) x.st(r[i * GA + k], va, 4u * (TPS * i + 16 * k) * S);
     }
 }
 
@@ -328,16 +381,32 @@ uint32_t row_logG(const DevPlan& P) {
     return want < P.logR ? want : P.logR;
 }
 
+template <int LOGM, int LOGC>
+void col_launch2(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
+    const dim3 grid(1u << (P.logS - LOGC), P.K);
+    const dim3 block(1u << (LOGC + LOGM - 4));
+    const size_t lds = sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << LOGC;
+    if (forward)
+        launch((k_ntt_col_reg<LOGM, true, LOGC>), grid, block, lds, st, X, P);
+    else
+        launch((k_ntt_col_reg<LOGM, false, LOGC>), grid, block, lds, st, X, P);
+}
+
+// The (R, C) shapes that N = 2^8 .. 2^17 produce; any other goes generic.
 template <int LOGM>
 void col_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     const uint32_t logC = col_logC(P);
-    const dim3 grid(1u << (P.logS - logC), P.K);
-    const dim3 block(1u << (logC + LOGM - 4));
-    const size_t lds = sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logC;
-    if (forward)
-        launch((k_ntt_col_reg<LOGM, true>), grid, block, lds, st, X, P, logC);
-    else
-        launch((k_ntt_col_reg<LOGM, false>), grid, block, lds, st, X, P, logC);
+    switch (LOGM * 16 + logC) {
+        case 4 * 16 + 4: col_launch2<4, 4>(P, X, forward, st); return;
+        case 4 * 16 + 5: col_launch2<4, 5>(P, X, forward, st); return;
+        case 5 * 16 + 5: col_launch2<5, 5>(P, X, forward, st); return;
+        case 5 * 16 + 6: col_launch2<5, 6>(P, X, forward, st); return;
+        case 6 * 16 + 6: col_launch2<6, 6>(P, X, forward, st); return;
+        case 6 * 16 + 7: col_launch2<6, 7>(P, X, forward, st); return;
+        case 7 * 16 + 6: col_launch2<7, 6>(P, X, forward, st); return;
+        case 8 * 16 + 5: col_launch2<8, 5>(P, X, forward, st); return;
+        default: launch_ntt_cols_generic(P, X, forward, st);
+    }
 }
 
 template <int LOGM>
