@@ -162,6 +162,7 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q)
     d.q_top = h.q_top;
     d.pc = pl->upload(pc);
     d.pow2 = pl->upload(h.pow2);
+    d.pow2b = pl->upload(h.pow2b);
     d.tw = pl->tw->tw;
     d.itw = pl->tw->itw;
     d.rz = pl->tw->rz;
@@ -191,6 +192,52 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q)
 // streams (independent products run concurrently), replayed as one hipGraph.
 constexpr int kBranches = 4;
 
+// Products that share N, L and the primes, all in the matrix-core form, each
+// with its own plan, on a geometry with register NTT kernels: they can run as
+// phase-batched launches (five per kBatchMax products).
+bool batchable(pm_plan* const* plans, int count) {
+    if (count < 1) return false;
+    const pm_plan* p0 = plans[0];
+    if (p0->form != PM_FORM_MFMA || !pm::batch_kernels_ok(p0->dev)) return false;
+    for (int i = 1; i < count; ++i) {
+        const pm_plan* p = plans[i];
+        if (p->form != PM_FORM_MFMA || p->host.N != p0->host.N || p->host.L != p0->host.L || p->host.p != p0->host.p)
+            return false;
+        for (int k = 0; k < i; ++k)
+            if (plans[k] == p) return false;
+    }
+    return true;
+}
+
+// Decompose every a and b, forward columns of every A and B, fused rows, inverse
+// columns, CRT: one launch each per kBatchMax products, all on st.
+void run_batched(pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b, uint32_t* const* c,
+                 int count, hipStream_t st) {
+    const pm::DevPlan& P = plans[0]->dev;
+    for (int i0 = 0; i0 < count; i0 += pm::kBatchMax) {
+        const int n = std::min(count - i0, pm::kBatchMax);
+        pm::BatchArgs ba{};
+        for (int k = 0; k < n; ++k) {
+            const pm_plan* p = plans[i0 + k];
+            ba.a[k] = a[i0 + k];
+            ba.b[k] = b[i0 + k];
+            ba.c[k] = c[i0 + k];
+            ba.A[k] = p->A;
+            ba.B[k] = p->B;
+            ba.crt_tab[k] = p->dev.crt_tab;
+            ba.crt_fc[k] = p->dev.crt_fc;
+            ba.nq[k] = p->dev.nq;
+            ba.crt_lc[k] = p->dev.crt_lc;
+        }
+        pm::launch_decompose_mfma_batch(P, ba, n, st);
+        pm::launch_ntt_cols_batch(P, ba, n, true, st);
+        pm::launch_ntt_rows_fused_batch(P, ba, n, st);
+        pm::launch_ntt_cols_batch(P, ba, n, false, st);
+        pm::launch_crt_mfma_batch(P, ba, n, st);
+    }
+    check(hipGetLastError(), "kernel launch");
+}
+
 const char* const kPhaseNames[PM_NPHASES] = {"decompose_a", "decompose_b", "ntt_cols_fwd_a", "ntt_cols_fwd_b",
                                              "ntt_rows_fused", "ntt_cols_inv", "crt"};
 
@@ -204,7 +251,9 @@ void run_phase(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, h
     const bool mfma = pl->form == PM_FORM_MFMA;
     switch (i) {
         case 0: if (mfma) launch_decompose_mfma(P, a, pl->A, st); else launch_decompose_valu(P, a, pl->A, st); break;
-        case 1: if (mfma) launch_decompose_mfma(P, b, pl->B, st); else launch_decompose_valu(P, b, pl->B, st); break;
+        case 1:   // b scaled by the pointwise constant (b_side)
+            if (mfma) launch_decompose_mfma(b_side(P), b, pl->B, st); else launch_decompose_valu(b_side(P), b, pl->B, st);
+            break;
         case 2: launch_ntt_cols(P, pl->A, true, st); break;
         case 3: launch_ntt_cols(P, pl->B, true, st); break;
         case 4: launch_ntt_rows_fused(P, pl->A, pl->B, st); break;
@@ -228,6 +277,7 @@ struct pm_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<pm_plan*> plans;
+    bool phased = false;
     ~pm_batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -320,10 +370,12 @@ int pm_batch_multiply_device(int count, pm_plan* const* plans, const uint32_t* c
                              uint32_t* const* c, void* stream) {
     return guarded([&] {
         need(count >= 0 && (count == 0 || (plans && a && b && c)), "null argument");
-        for (int i = 0; i < count; ++i) {
-            need(plans[i] && a[i] && b[i] && c[i], "null entry in batch");
-            run_device(plans[i], a[i], b[i], c[i], stream_of(plans[i], stream));
+        for (int i = 0; i < count; ++i) need(plans[i] && a[i] && b[i] && c[i], "null entry in batch");
+        if (stream && batchable(plans, count)) {
+            run_batched(plans, a, b, c, count, static_cast<hipStream_t>(stream));
+            return;
         }
+        for (int i = 0; i < count; ++i) run_device(plans[i], a[i], b[i], c[i], stream_of(plans[i], stream));
     });
 }
 
@@ -340,7 +392,15 @@ int pm_batch_create(int count, pm_plan* const* plans, const uint32_t* const* a, 
         }
         hipStream_t origin = nullptr;
         check(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking), "hipStreamCreate");
-        std::vector<hipStream_t> branch(std::min(count, kBranches), nullptr);
+        // Phase-batched: the products split into `groups` runs of consecutive
+        // products, each run's five batched launches on a branch stream, so
+        // that different phases of different runs overlap.
+        int groups = 16, branches = kBranches;
+        if (const char* e = std::getenv("PM_BATCH_GROUPS")) groups = std::max(0, std::atoi(e));
+        if (const char* e = std::getenv("PM_BATCH_BRANCHES")) branches = std::max(1, std::atoi(e));
+        const bool phased = groups > 0 && batchable(plans, count);
+        groups = std::min(groups, count);
+        std::vector<hipStream_t> branch(phased ? groups : std::min(count, branches), nullptr);
         std::vector<hipEvent_t> done(branch.size(), nullptr);
         hipEvent_t fork = nullptr;
         auto cleanup = [&] {
@@ -359,7 +419,15 @@ int pm_batch_create(int count, pm_plan* const* plans, const uint32_t* const* a, 
             try {
                 check(hipEventRecord(fork, origin), "hipEventRecord");
                 for (size_t s = 0; s < branch.size(); ++s) check(hipStreamWaitEvent(branch[s], fork, 0), "wait");
-                for (int i = 0; i < count; ++i) run_device(plans[i], a[i], b[i], c[i], branch[i % branch.size()]);
+                if (phased) {
+                    const int ng = static_cast<int>(branch.size());
+                    for (int g = 0; g < ng; ++g) {
+                        const int i0 = count * g / ng, i1 = count * (g + 1) / ng;
+                        run_batched(plans + i0, a + i0, b + i0, c + i0, i1 - i0, branch[g]);
+                    }
+                } else {
+                    for (int i = 0; i < count; ++i) run_device(plans[i], a[i], b[i], c[i], branch[i % branch.size()]);
+                }
                 for (size_t s = 0; s < branch.size(); ++s) {
                     check(hipEventRecord(done[s], branch[s]), "hipEventRecord");
                     check(hipStreamWaitEvent(origin, done[s], 0), "wait");
@@ -377,6 +445,7 @@ int pm_batch_create(int count, pm_plan* const* plans, const uint32_t* const* a, 
             throw;
         }
         cleanup();
+        bt->phased = phased;
         *out = bt.release();
     });
 }
@@ -387,6 +456,8 @@ int pm_batch_launch(pm_batch* batch, void* stream) {
         check(hipGraphLaunch(batch->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
     });
 }
+
+int pm_batch_phased(const pm_batch* batch) { return batch && batch->phased ? 1 : 0; }
 
 void pm_batch_destroy(pm_batch* batch) { delete batch; }
 

@@ -28,6 +28,13 @@ struct DevPlan {
     double q_top;
     const PrimeConst* pc;     // K
     const uint32_t* pow2;     // K x L: 2^(32 l) mod p_j
+    // The b side of a product: tables that decompose b into b scale_j mod p_j
+    // (scale_j the pointwise constant of PrimeConst), so that the fused row
+    // pass needs only a Montgomery product. pow2b: K x L; the MFMA tables
+    // dec_digits_b / dec_corr_b are laid out as dec_digits / dec_corr.
+    const uint32_t* pow2b;
+    const int8_t* dec_digits_b;
+    const double* dec_corr_b;
     const uint2* tw;          // K x N: (psi^bitrev(i), Shoup companion)
     const uint2* itw;         // K x N: (psi^-bitrev(i), Shoup companion)
     // Row-pass twiddles, generated: the twiddle of row rho, stage distance 2^e,
@@ -66,6 +73,15 @@ struct DevPlan {
     uint32_t dec_magic;       // floor(2^32 / (L / 4)) + 1: i / (L / 4) = mulhi(i, dec_magic) for i < 2^12
 };
 
+// P with the b-side decompose tables in place of the plain ones.
+inline DevPlan b_side(const DevPlan& P) {
+    DevPlan Q = P;
+    Q.pow2 = P.pow2b;
+    Q.dec_digits = P.dec_digits_b;
+    Q.dec_corr = P.dec_corr_b;
+    return Q;
+}
+
 // Every kernel of the library is launched through this one function.
 template <typename F, typename... Args>
 inline void launch(F kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st, Args... args) {
@@ -76,6 +92,8 @@ inline void launch(F kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st, 
 void launch_decompose_valu(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st);
 void launch_ntt_cols(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st);
 void launch_ntt_rows(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st);
+// Forward rows of A and B, pointwise product into A, inverse rows; B must come
+// from the b-side decompose (b_side).
 void launch_ntt_rows_fused(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st);
 void launch_pointwise(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st);
 void launch_crt_valu(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream_t st);
@@ -93,6 +111,30 @@ void launch_ntt_rows_fused_generic(const DevPlan& P, uint32_t* A, const uint32_t
 
 // Matrix-core form (int8 MFMA, exact).
 void launch_decompose_mfma(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st);
+// Batches of up to kBatchMax products that share N, the primes and L (so
+// every grid and every prime table): one plan P for all, and per product its
+// operands a, b, answer c, residue buffers A, B and the tables that depend on
+// q, all passed as one kernel argument. Each batched launch covers every
+// product at once (blockIdx.z picks the product) and is large enough to run in
+// steady state.
+constexpr int kBatchMax = 32;
+struct BatchArgs {
+    const uint32_t* a[kBatchMax];
+    const uint32_t* b[kBatchMax];
+    uint32_t* c[kBatchMax];
+    uint32_t* A[kBatchMax];
+    uint32_t* B[kBatchMax];
+    const uint32_t* crt_tab[kBatchMax];
+    const double* crt_fc[kBatchMax];
+    const double* nq[kBatchMax];
+    const uint32_t* crt_lc[kBatchMax];
+};
+bool batch_kernels_ok(const DevPlan& P);   // register NTT kernels exist for this geometry
+void launch_decompose_mfma_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st);
+void launch_ntt_cols_batch(const DevPlan& P, const BatchArgs& ba, int count, bool forward, hipStream_t st);
+void launch_ntt_rows_fused_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st);
+void launch_crt_mfma_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st);
+
 // Plain grid-stride copy of `bytes` (a multiple of 16) with 16-byte accesses.
 void launch_copy16(void* dst, const void* src, size_t bytes, hipStream_t st);
 

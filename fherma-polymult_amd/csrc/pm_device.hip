@@ -149,6 +149,7 @@ __device__ __forceinline__ uint32_t pointwise(uint32_t a, uint32_t b, const Prim
 }
 
 // Row pass fused: forward rows of A and B, pointwise into A, inverse rows of A.
+// B must come from the b-side decompose (b_side in pm_device.h).
 __global__ void __launch_bounds__(kNttThreads)
 k_ntt_rows_fused(uint32_t* __restrict__ A, const uint32_t* __restrict__ B, DevPlan P, TileGeom g) {
     extern __shared__ uint32_t ntt_tile[];
@@ -166,7 +167,8 @@ k_ntt_rows_fused(uint32_t* __restrict__ A, const uint32_t* __restrict__ B, DevPl
     __syncthreads();
     tile_stages<true>(ta, P.tw + row, pc.p, base, g, P);
     tile_stages<true>(tb, P.tw + row, pc.p, base, g, P);
-    for (uint32_t i = threadIdx.x; i < T; i += blockDim.x) ta[i] = pointwise(ta[i], tb[i], pc);
+    // B holds b scale_j (b-side decompose): the Montgomery product is the pointwise term
+    for (uint32_t i = threadIdx.x; i < T; i += blockDim.x) ta[i] = mont_mul(ta[i], tb[i], pc.p, pc.pneg);
     __syncthreads();
     tile_stages<false>(ta, P.itw + row, pc.p, base, g, P);
     for (uint32_t i = threadIdx.x; i < T; i += blockDim.x) A[row + base + i] = ta[i];

@@ -201,11 +201,13 @@ HostPipeline& pipeline(pm_plan* pl) {
     return *pl->pipe;
 }
 
+// b goes into B scaled by the pointwise constant (b_side), as in run_device.
 void decompose_range(pm_plan* pl, const uint32_t* limbs, uint32_t* res, const Span& s, hipStream_t st) {
+    const DevPlan P = res == pl->B ? b_side(pl->dev) : pl->dev;
     if (pl->form == PM_FORM_MFMA)
-        launch_decompose_mfma_range(pl->dev, limbs, res, s.k0, s.count, st);
+        launch_decompose_mfma_range(P, limbs, res, s.k0, s.count, st);
     else
-        launch_decompose_valu_range(pl->dev, limbs, res, s.k0, s.count, st);
+        launch_decompose_valu_range(P, limbs, res, s.k0, s.count, st);
 }
 
 void crt_range(pm_plan* pl, const uint32_t* Y, uint32_t* c, const Span& s, hipStream_t st) {

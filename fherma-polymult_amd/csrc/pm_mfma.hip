@@ -76,9 +76,8 @@ __device__ __forceinline__ uint32_t mod_p_f64(double r, uint32_t p, double pinv)
 // and the four waves split the prime groups. A fragments of the next group are
 // loaded before the current group's MFMAs.
 template <int KT>
-__global__ void __launch_bounds__(64 * kWaves)
-k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res, DevPlan P, uint32_t k0,
-                 uint32_t count) {
+__device__ __forceinline__ void decompose_tile(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
+                                               const DevPlan& P, uint32_t k0, uint32_t count) {
     extern __shared__ uint32_t xs[];
     constexpr int Ls = 16 * KT + 4;
     const int L = P.L;
@@ -204,6 +203,24 @@ k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
     }
 }
 
+template <int KT>
+__global__ void __launch_bounds__(64 * kWaves)
+k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res, DevPlan P, uint32_t k0,
+                 uint32_t count) {
+    decompose_tile<KT>(limbs, res, P, k0, count);
+}
+
+// A batch: blockIdx.z = 2 (product) + (0: a into A, 1: b into B).
+template <int KT>
+__global__ void __launch_bounds__(64 * kWaves) k_decompose_mfma_batch(DevPlan P, BatchArgs ba) {
+    const int i = blockIdx.z >> 1;
+    const bool second = (blockIdx.z & 1) != 0;   // b: scaled by the pointwise constant
+    DevPlan Q = P;
+    Q.dec_digits = second ? P.dec_digits_b : P.dec_digits;
+    Q.dec_corr = second ? P.dec_corr_b : P.dec_corr;
+    decompose_tile<KT>(second ? ba.b[i] : ba.a[i], second ? ba.B[i] : ba.A[i], Q, 0, P.N);
+}
+
 // CRT, one workgroup (four waves) per 64 coefficients, in one kernel:
 //   1. all threads: y[j][nb .. nb + 64) for every prime, byte-transposed per
 //      group of four primes (word a of group J holds byte a of y_4J .. y_4J+3),
@@ -290,8 +307,8 @@ __device__ __forceinline__ long long seg_carry(long long C, long long cout, uint
     return cout + adj;
 }
 
-__global__ void __launch_bounds__(64 * kWaves)
-k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, uint32_t k0, uint32_t count) {
+__device__ __forceinline__ void crt_tile(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, const DevPlan& P,
+                                         uint32_t k0, uint32_t count) {
     extern __shared__ uint32_t smem[];
     const int K = P.K, L = P.L, KS = P.crt_ksteps, ST = P.crt_stiles;
     const int KJ = (K + 3) >> 2;
@@ -558,6 +575,22 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
     }
 }
 
+__global__ void __launch_bounds__(64 * kWaves)
+k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, uint32_t k0, uint32_t count) {
+    crt_tile(Y, c, P, k0, count);
+}
+
+// A batch: blockIdx.z = product, with that product's q-dependent tables.
+__global__ void __launch_bounds__(64 * kWaves) k_crt_mfma_batch(DevPlan P, BatchArgs ba) {
+    const int i = blockIdx.z;
+    DevPlan Q = P;
+    Q.crt_tab = ba.crt_tab[i];
+    Q.crt_fc = ba.crt_fc[i];
+    Q.nq = ba.nq[i];
+    Q.crt_lc = ba.crt_lc[i];
+    crt_tile(ba.A[i], ba.c[i], Q, 0, P.N);
+}
+
 // Balanced base-256 digits of a signed value |x| < 2^31.
 void balanced4(long long x, int8_t out[4]) {
     for (int i = 0; i < 4; ++i) {
@@ -575,34 +608,40 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
     if (L > 4 * kDecMaxKSteps * 4 || (4 * K + 63) / 64 > kCrtMaxKSteps) {
         throw std::runtime_error("point too large for the matrix-core tables");
     }
-    // Decompose tables.
+    // Decompose tables: T_{j,t} = f_j 256^t mod p_j with f_j = 1 (the plain
+    // tables) or f_j = scale_j (the b side: b scale_j mod p_j).
     const int KT = (4 * L + 63) / 64;
     const int KG = (K + 3) / 4;
-    std::vector<int8_t> dec(static_cast<size_t>(KG) * KT * 64 * 16, 0);
-    std::vector<double> dcorr(K);
-    for (int j = 0; j < K; ++j) {
-        const uint64_t p = h.p[j];
-        uint64_t T = 1, sum = 0;
-        for (int t = 0; t < 4 * L; ++t) {
-            sum = (sum + T) % p;
-            const long long centred = T <= p / 2 ? static_cast<long long>(T) : static_cast<long long>(T) - static_cast<long long>(p);
-            int8_t e[4];
-            balanced4(centred, e);
-            const int ks = t / 64, g = (t % 64) / 16, byte = t % 16;
-            for (int b = 0; b < 4; ++b) {
-                const int row = 4 * (j % 4) + b;
-                const int lane = 16 * g + row;
-                dec[((static_cast<size_t>(j / 4) * KT + ks) * 64 + lane) * 16 + byte] = e[b];
+    auto dec_tables = [&](bool scaled, const int8_t** digits, const double** corr) {
+        std::vector<int8_t> dec(static_cast<size_t>(KG) * KT * 64 * 16, 0);
+        std::vector<double> dcorr(K);
+        for (int j = 0; j < K; ++j) {
+            const uint64_t p = h.p[j];
+            uint64_t T = scaled ? h.scale[j] % p : 1, sum = 0;
+            for (int t = 0; t < 4 * L; ++t) {
+                sum = (sum + T) % p;
+                const long long centred =
+                    T <= p / 2 ? static_cast<long long>(T) : static_cast<long long>(T) - static_cast<long long>(p);
+                int8_t e[4];
+                balanced4(centred, e);
+                const int ks = t / 64, g = (t % 64) / 16, byte = t % 16;
+                for (int b = 0; b < 4; ++b) {
+                    const int row = 4 * (j % 4) + b;
+                    const int lane = 16 * g + row;
+                    dec[((static_cast<size_t>(j / 4) * KT + ks) * 64 + lane) * 16 + byte] = e[b];
+                }
+                T = T * 256 % p;
             }
-            T = T * 256 % p;
+            // |sum_b 256^b G_b| < 2^47: adding ceil(2^47 / p) p keeps it positive and
+            // below 2^49 (exact in FP64), without changing the residue
+            const uint64_t lift = ((uint64_t(1) << 47) + p - 1) / p * p;
+            dcorr[j] = static_cast<double>(sum * 128 % p + lift);
         }
-        // |sum_b 256^b G_b| < 2^47: adding ceil(2^47 / p) p keeps it positive and
-        // below 2^49 (exact in FP64), without changing the residue
-        const uint64_t lift = ((uint64_t(1) << 47) + p - 1) / p * p;
-        dcorr[j] = static_cast<double>(sum * 128 % p + lift);
-    }
-    d.dec_digits = static_cast<const int8_t*>(upload(dec.data(), dec.size()));
-    d.dec_corr = static_cast<const double*>(upload(dcorr.data(), dcorr.size() * sizeof(double)));
+        *digits = static_cast<const int8_t*>(upload(dec.data(), dec.size()));
+        *corr = static_cast<const double*>(upload(dcorr.data(), dcorr.size() * sizeof(double)));
+    };
+    dec_tables(false, &d.dec_digits, &d.dec_corr);
+    dec_tables(true, &d.dec_digits_b, &d.dec_corr_b);
     d.dec_ksteps = KT;
     // used only for L >= 8, so L / 4 >= 2 and the magic fits 32 bits
     d.dec_magic = static_cast<uint32_t>((uint64_t(1) << 32) / static_cast<uint64_t>(std::max(L / 4, 2)) + 1);
@@ -692,6 +731,26 @@ void launch_crt_mfma_range(const DevPlan& P, const uint32_t* Y, uint32_t* c, uin
     const dim3 grid((count + 63) / 64);
     const size_t lds = sizeof(uint32_t) * crt_lds_words(P.crt_ksteps, P.crt_stiles, P.K);
     launch(k_crt_mfma, grid, dim3(64 * kWaves), lds, st, Y, c, P, k0, count);
+}
+
+void launch_decompose_mfma_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st) {
+    if (!P.dec_digits) throw std::runtime_error("matrix-core tables not prepared");
+    const dim3 grid((P.N + 16 * kTiles - 1) / (16 * kTiles), 1, 2 * count);
+    const size_t lds = sizeof(uint32_t) * 16 * kTiles * (16 * P.dec_ksteps + 4) + 20 * static_cast<size_t>(P.K);
+    switch (P.dec_ksteps) {
+        case 1: launch(k_decompose_mfma_batch<1>, grid, dim3(64 * kWaves), lds, st, P, ba); break;
+        case 2: launch(k_decompose_mfma_batch<2>, grid, dim3(64 * kWaves), lds, st, P, ba); break;
+        case 3: launch(k_decompose_mfma_batch<3>, grid, dim3(64 * kWaves), lds, st, P, ba); break;
+        case 4: launch(k_decompose_mfma_batch<4>, grid, dim3(64 * kWaves), lds, st, P, ba); break;
+        default: throw std::runtime_error("L out of range");
+    }
+}
+
+void launch_crt_mfma_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st) {
+    if (!P.crt_tab) throw std::runtime_error("matrix-core tables not prepared");
+    const dim3 grid((P.N + 63) / 64, 1, count);
+    const size_t lds = sizeof(uint32_t) * crt_lds_words(P.crt_ksteps, P.crt_stiles, P.K);
+    launch(k_crt_mfma_batch, grid, dim3(64 * kWaves), lds, st, P, ba);
 }
 
 }  // namespace pm

@@ -188,8 +188,7 @@ struct BufTw {
 // TPS NG = 16, padded(t + TPS i + 16 k) = t + TPS i + 17 k and
 // padded(16 t + k) = 17 t + k, so every slot is a lane base plus a constant.
 template <int LOGM, bool kFwd, int LOGC>
-__global__ void __launch_bounds__(512)
-k_ntt_col_reg(uint32_t* __restrict__ X, DevPlan P) {
+__device__ __forceinline__ void col_pass(uint32_t* __restrict__ X, const DevPlan& P) {
     extern __shared__ uint32_t lds[];
     constexpr uint32_t M = 1u << LOGM, TPS = M / 16, GA = TPS, NG = 16 / GA, C = 1u << LOGC;
     const uint32_t c = threadIdx.x & (C - 1), t = threadIdx.x >> LOGC;
@@ -237,6 +236,18 @@ k_ntt_col_reg(uint32_t* __restrict__ X, DevPlan P) {
 #pragma unroll
             for (uint32_t k = 0; k < GA; ++k) x.st(r[i * GA + k], va, 4u * (TPS * i + 16 * k) * S);
     }
+}
+
+template <int LOGM, bool kFwd, int LOGC>
+__global__ void __launch_bounds__(512) k_ntt_col_reg(uint32_t* __restrict__ X, DevPlan P) {
+    col_pass<LOGM, kFwd, LOGC>(X, P);
+}
+
+// A batch: forward, blockIdx.z = 2 (product) + (0: A, 1: B); inverse, blockIdx.z = product (A).
+template <int LOGM, bool kFwd, int LOGC>
+__global__ void __launch_bounds__(512) k_ntt_col_batch(DevPlan P, BatchArgs ba) {
+    const int i = kFwd ? blockIdx.z >> 1 : blockIdx.z;
+    col_pass<LOGM, kFwd, LOGC>(kFwd && (blockIdx.z & 1) ? ba.B[i] : ba.A[i], P);
 }
 
 // Row pass alone (forward or inverse): G = 2^logG rows of M = S elements.
@@ -298,15 +309,18 @@ k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
     }
 }
 
+// In a product, B was decomposed with the b-side tables (b scale_j, scale_j =
+// N^-1 (P/p_j)^-1 2^32), so the Montgomery product a b scale_j 2^-32 is already
+// the pointwise term a b N^-1 (P/p_j)^-1.
 __device__ __forceinline__ uint32_t pointwise_mul(uint32_t a, uint32_t b, const PrimeConst& pc) {
-    return mul_shoup(mont_mul(a, b, pc.p, pc.pneg), pc.scale, pc.scale_s, pc.p);
+    return mont_mul(a, b, pc.p, pc.pneg);
 }
 
 // Row pass fused: forward rows of A and B (sharing twiddle loads), the pointwise
 // product with N^-1 (P/p_j)^-1 into A, inverse rows of A.
 template <int LOGM>
-__global__ void __launch_bounds__(512)
-k_ntt_rows_fused_reg(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, DevPlan P, uint32_t logG) {
+__device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, const DevPlan& P,
+                                           uint32_t logG) {
     extern __shared__ uint32_t lds[];
     constexpr uint32_t M = 1u << LOGM, TPS = M / 16, GA = TPS, NG = 16 / GA, ROW = M + M / 16;
     const uint32_t t = threadIdx.x & (TPS - 1), rl = threadIdx.x / TPS;
@@ -367,6 +381,18 @@ k_ntt_rows_fused_reg(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, 
         for (uint32_t k = 0; k < GA; ++k) A[row + t + TPS * i + 16 * k] = a[i * GA + k];
 }
 
+template <int LOGM>
+__global__ void __launch_bounds__(512)
+k_ntt_rows_fused_reg(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, DevPlan P, uint32_t logG) {
+    rows_fused<LOGM>(A, Bm, P, logG);
+}
+
+// A batch: blockIdx.z = product.
+template <int LOGM>
+__global__ void __launch_bounds__(512) k_ntt_rows_fused_batch(DevPlan P, BatchArgs ba, uint32_t logG) {
+    rows_fused<LOGM>(ba.A[blockIdx.z], ba.B[blockIdx.z], P, logG);
+}
+
 bool reg_ok(uint32_t logm) { return logm >= 4 && logm <= 8; }
 
 // Columns per column-pass workgroup: up to 512 threads, at most S columns.
@@ -382,14 +408,32 @@ uint32_t row_logG(const DevPlan& P) {
 }
 
 template <int LOGM, int LOGC>
-void col_launch2(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
-    const dim3 grid(1u << (P.logS - LOGC), P.K);
+void col_launch2(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st, const BatchArgs* ba = nullptr,
+                 int count = 0) {
+    const dim3 grid(1u << (P.logS - LOGC), P.K, ba ? (forward ? 2 * count : count) : 1);
     const dim3 block(1u << (LOGC + LOGM - 4));
     const size_t lds = sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << LOGC;
-    if (forward)
+    if (ba) {
+        if (forward)
+            launch((k_ntt_col_batch<LOGM, true, LOGC>), grid, block, lds, st, P, *ba);
+        else
+            launch((k_ntt_col_batch<LOGM, false, LOGC>), grid, block, lds, st, P, *ba);
+    } else if (forward) {
         launch((k_ntt_col_reg<LOGM, true, LOGC>), grid, block, lds, st, X, P);
-    else
+    } else {
         launch((k_ntt_col_reg<LOGM, false, LOGC>), grid, block, lds, st, X, P);
+    }
+}
+
+// (R, C) shape of the column pass as one switch key, or -1 if only the generic kernels apply.
+int col_shape(const DevPlan& P) {
+    if (P.logR < 4 || P.logR > 8) return -1;
+    const int key = static_cast<int>(P.logR * 16 + col_logC(P));
+    switch (key) {
+        case 4 * 16 + 4: case 4 * 16 + 5: case 5 * 16 + 5: case 5 * 16 + 6:
+        case 6 * 16 + 6: case 6 * 16 + 7: case 7 * 16 + 6: case 8 * 16 + 5: return key;
+        default: return -1;
+    }
 }
 
 // The (R, C) shapes that N = 2^8 .. 2^17 produce; any other goes generic.
@@ -420,6 +464,16 @@ void row_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
         launch((k_ntt_row_reg<LOGM, true>), grid, block, lds, st, X, P, logG);
     else
         launch((k_ntt_row_reg<LOGM, false>), grid, block, lds, st, X, P, logG);
+}
+
+template <int LOGM>
+void fused_launch_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st) {
+    const uint32_t logG = row_logG(P);
+    const dim3 grid(1u << (P.logR - logG), P.K, count);
+    const dim3 block(1u << (logG + LOGM - 4));
+    const size_t lds = 2 * (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
+                       (2 * sizeof(uint2) << (LOGM - 4 + logG));
+    launch((k_ntt_rows_fused_batch<LOGM>), grid, block, lds, st, P, ba, logG);
 }
 
 template <int LOGM>
@@ -465,6 +519,35 @@ void launch_ntt_rows_fused(const DevPlan& P, uint32_t* A, const uint32_t* B, hip
         case 7: fused_launch<7>(P, A, B, st); return;
         case 8: fused_launch<8>(P, A, B, st); return;
         default: launch_ntt_rows_fused_generic(P, A, B, st);
+    }
+}
+
+bool batch_kernels_ok(const DevPlan& P) {
+    return P.logR > 0 && col_shape(P) >= 0 && reg_ok(P.logS);
+}
+
+void launch_ntt_cols_batch(const DevPlan& P, const BatchArgs& ba, int count, bool forward, hipStream_t st) {
+    switch (col_shape(P)) {
+        case 4 * 16 + 4: col_launch2<4, 4>(P, nullptr, forward, st, &ba, count); return;
+        case 4 * 16 + 5: col_launch2<4, 5>(P, nullptr, forward, st, &ba, count); return;
+        case 5 * 16 + 5: col_launch2<5, 5>(P, nullptr, forward, st, &ba, count); return;
+        case 5 * 16 + 6: col_launch2<5, 6>(P, nullptr, forward, st, &ba, count); return;
+        case 6 * 16 + 6: col_launch2<6, 6>(P, nullptr, forward, st, &ba, count); return;
+        case 6 * 16 + 7: col_launch2<6, 7>(P, nullptr, forward, st, &ba, count); return;
+        case 7 * 16 + 6: col_launch2<7, 6>(P, nullptr, forward, st, &ba, count); return;
+        case 8 * 16 + 5: col_launch2<8, 5>(P, nullptr, forward, st, &ba, count); return;
+        default: throw std::runtime_error("no batched column pass for this geometry");
+    }
+}
+
+void launch_ntt_rows_fused_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st) {
+    switch (reg_ok(P.logS) ? P.logS : 0) {
+        case 4: fused_launch_batch<4>(P, ba, count, st); return;
+        case 5: fused_launch_batch<5>(P, ba, count, st); return;
+        case 6: fused_launch_batch<6>(P, ba, count, st); return;
+        case 7: fused_launch_batch<7>(P, ba, count, st); return;
+        case 8: fused_launch_batch<8>(P, ba, count, st); return;
+        default: throw std::runtime_error("no batched row pass for this geometry");
     }
 }
 

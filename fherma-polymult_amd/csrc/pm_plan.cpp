@@ -200,6 +200,11 @@ HostPlan make_plan(uint32_t N, uint32_t W, const uint32_t* q, int L) {
         pl.scale_s[j] = shoup_companion(pl.scale[j], p);
         pl.p_recip[j] = 1.0 / p;
     }
+    pl.pow2b.resize(static_cast<size_t>(K) * L);
+    for (int j = 0; j < K; ++j)
+        for (int l = 0; l < L; ++l)
+            pl.pow2b[static_cast<size_t>(j) * L + l] = static_cast<uint32_t>(
+                static_cast<uint64_t>(pl.scale[j]) * pl.pow2[static_cast<size_t>(j) * L + l] % pl.p[j]);
 
     // M_j = prod_{i != j} p_i mod q and P mod q (rns_plan.h:224-236).
     pl.M.assign(static_cast<size_t>(K) * L, 0);

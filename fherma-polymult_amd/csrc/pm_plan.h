@@ -27,6 +27,10 @@ struct HostPlan {
     std::vector<uint32_t> p, pneg;    // K; pneg = -p^-1 mod 2^32
     std::vector<uint32_t> psi, ipsi;  // K: primitive 2N-th roots of unity and inverses
     std::vector<uint32_t> pow2, pow2_s;    // K x L: 2^(32 l) mod p_j, Shoup companions
+    // K x L: scale_j 2^(32 l) mod p_j, scale_j the pointwise constant below.
+    // Decomposing b with these gives b scale_j mod p_j, so in a product the
+    // fused row pass's Montgomery product alone yields a b N^-1 (P/p_j)^-1.
+    std::vector<uint32_t> pow2b;
     // Pointwise constant N^-1 (P/p_j)^-1 2^32 mod p_j (the 2^32 undoes the
     // Montgomery product), with its Shoup companion.
     std::vector<uint32_t> scale, scale_s;

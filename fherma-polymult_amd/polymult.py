@@ -58,6 +58,7 @@ def _load() -> ctypes.CDLL:
         "pm_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                            ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
         "pm_batch_launch": (ctypes.c_int, [_vp, _vp]),
+        "pm_batch_phased": (ctypes.c_int, [_vp]),
         "pm_batch_destroy": (None, [_vp]),
         "pm_stage_decompose": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
         "pm_stage_ntt": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
@@ -150,6 +151,31 @@ class DeviceBuffer:
             pass
 
 
+class Stream:
+    """A HIP stream from the library (pm_stream_create); pass it as `stream=`."""
+
+    def __init__(self):
+        h = _vp()
+        _check(lib.pm_stream_create(ctypes.byref(h)))
+        self.handle = h.value
+
+    def synchronize(self) -> None:
+        _check(lib.pm_stream_synchronize(_vp(self.handle)))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib.pm_stream_destroy(_vp(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+def _sv(stream):
+    """A stream argument: None (the default stream or the plan's), a Stream, or a raw handle."""
+    return _vp(stream.handle if isinstance(stream, Stream) else stream)
+
+
 def synchronize() -> None:
     _check(lib.pm_device_synchronize())
 
@@ -159,7 +185,7 @@ def device_copy(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int = None, stream
     n = min(dst.nbytes, src.nbytes) if nbytes is None else int(nbytes)
     if n > dst.nbytes or n > src.nbytes:
         raise ValueError("copy larger than its buffers")
-    _check(lib.pm_device_copy(_vp(dst.ptr), _vp(src.ptr), n, _vp(stream)))
+    _check(lib.pm_device_copy(_vp(dst.ptr), _vp(src.ptr), n, _sv(stream)))
 
 
 def device_info():
@@ -223,32 +249,32 @@ class Plan:
         return m.value, ms.value
 
     def multiply_device(self, a: DeviceBuffer, b: DeviceBuffer, c: DeviceBuffer, stream=None) -> None:
-        _check(lib.pm_multiply_device(self.handle, _vp(a.ptr), _vp(b.ptr), _vp(c.ptr), _vp(stream)))
+        _check(lib.pm_multiply_device(self.handle, _vp(a.ptr), _vp(b.ptr), _vp(c.ptr), _sv(stream)))
 
     # the five steps, device to device
     def stage_decompose(self, limbs: DeviceBuffer, res: DeviceBuffer, stream=None) -> None:
-        _check(lib.pm_stage_decompose(self.handle, _vp(limbs.ptr), _vp(res.ptr), _vp(stream)))
+        _check(lib.pm_stage_decompose(self.handle, _vp(limbs.ptr), _vp(res.ptr), _sv(stream)))
 
     def stage_ntt(self, res: DeviceBuffer, forward: bool, stream=None) -> None:
-        _check(lib.pm_stage_ntt(self.handle, _vp(res.ptr), 1 if forward else 0, _vp(stream)))
+        _check(lib.pm_stage_ntt(self.handle, _vp(res.ptr), 1 if forward else 0, _sv(stream)))
 
     def stage_pointwise(self, A: DeviceBuffer, B: DeviceBuffer, stream=None) -> None:
-        _check(lib.pm_stage_pointwise(self.handle, _vp(A.ptr), _vp(B.ptr), _vp(stream)))
+        _check(lib.pm_stage_pointwise(self.handle, _vp(A.ptr), _vp(B.ptr), _sv(stream)))
 
     def stage_crt(self, res: DeviceBuffer, c: DeviceBuffer, stream=None) -> None:
-        _check(lib.pm_stage_crt(self.handle, _vp(res.ptr), _vp(c.ptr), _vp(stream)))
+        _check(lib.pm_stage_crt(self.handle, _vp(res.ptr), _vp(c.ptr), _sv(stream)))
 
     def time_device(self, a: DeviceBuffer, b: DeviceBuffer, c: DeviceBuffer, warmup: int, iters: int,
                     stream=None) -> float:
         ms = ctypes.c_double(0)
         _check(lib.pm_time_multiply_device(self.handle, _vp(a.ptr), _vp(b.ptr), _vp(c.ptr), int(warmup),
-                                           int(iters), ctypes.byref(ms), _vp(stream)))
+                                           int(iters), ctypes.byref(ms), _sv(stream)))
         return ms.value
 
     def profile_phases(self, a: DeviceBuffer, b: DeviceBuffer, c: DeviceBuffer, iters: int, stream=None) -> dict:
         out = (ctypes.c_double * NPHASES)()
         _check(lib.pm_profile_phases(self.handle, _vp(a.ptr), _vp(b.ptr), _vp(c.ptr), int(iters), out,
-                                     _vp(stream)))
+                                     _sv(stream)))
         return {lib.pm_phase_name(i).decode(): out[i] for i in range(NPHASES)}
 
     def close(self) -> None:
@@ -270,7 +296,7 @@ def batch_multiply_device(plans, a_bufs, b_bufs, c_bufs, stream=None) -> None:
     n = len(plans)
     arr = lambda xs: (_vp * n)(*[_vp(x) for x in xs])  # noqa: E731
     _check(lib.pm_batch_multiply_device(n, arr([p._h for p in plans]), arr([x.ptr for x in a_bufs]),
-                                        arr([x.ptr for x in b_bufs]), arr([x.ptr for x in c_bufs]), _vp(stream)))
+                                        arr([x.ptr for x in b_bufs]), arr([x.ptr for x in c_bufs]), _sv(stream)))
 
 
 class Batch:
@@ -286,7 +312,12 @@ class Batch:
         self._keep = (list(plans), list(a_bufs), list(b_bufs), list(c_bufs))
 
     def launch(self, stream=None) -> None:
-        _check(lib.pm_batch_launch(_vp(self._h), _vp(stream)))
+        _check(lib.pm_batch_launch(_vp(self._h), _sv(stream)))
+
+    @property
+    def phased(self) -> bool:
+        """True if captured as phase-batched launches (one kernel per phase for all products)."""
+        return bool(lib.pm_batch_phased(_vp(self._h)))
 
     def close(self) -> None:
         if getattr(self, "_h", None):

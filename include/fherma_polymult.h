@@ -92,18 +92,27 @@ int pm_plan_tune_host(pm_plan* plan, int* mode, double* ms_per_product);
 
 /* A batch of independent products, each with its own plan (N and q may differ),
  * all device pointers, enqueued on `stream`. No counterpart in the reference:
- * it is for callers with many products per call (uneven batches). */
+ * it is for callers with many products per call (uneven batches). With an
+ * explicit stream, distinct plans that share N, L and the primes in the
+ * matrix-core form run as phase-batched launches (each kernel covers every
+ * product); otherwise product by product (on each plan's stream if `stream`
+ * is null). */
 int pm_batch_multiply_device(int count, pm_plan* const* plans, const uint32_t* const* a,
                              const uint32_t* const* b, uint32_t* const* c, void* stream);
 
-/* The same batch captured once as a hipGraph (independent products forked over
- * several streams) and replayed by pm_batch_launch: one launch per batch, no
- * per-kernel host overhead. The device pointers are fixed at creation; each
- * product needs its own plan, and the plans must outlive the batch. */
+/* The same batch captured once as a hipGraph and replayed by pm_batch_launch:
+ * one launch per batch, no per-kernel host overhead. Products that share N, L
+ * and the primes in the matrix-core form are captured as five phase-batched
+ * launches per 32 products; other batches as every product's kernel chain
+ * forked over several streams. The device pointers and the plans' forms are
+ * fixed at creation; each product needs its own plan, and the plans must
+ * outlive the batch. */
 typedef struct pm_batch pm_batch;
 int pm_batch_create(int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
                     uint32_t* const* c, pm_batch** out);
 int pm_batch_launch(pm_batch* batch, void* stream);
+/* 1 if the batch was captured as phase-batched launches, 0 if as per-product chains. */
+int pm_batch_phased(const pm_batch* batch);
 void pm_batch_destroy(pm_batch* batch);
 
 /* --- the five steps, one at a time (device pointers, enqueued on `stream`) --

@@ -162,3 +162,75 @@ def test_graph_batch_replays_match_gmp(golden, form):
     batch.close()
     for p in plans:
         p.close()
+
+
+def _qbits(q):
+    return int(oracle.int_of(q)).bit_length()
+
+
+@pytest.mark.parametrize("N,W", [(4096, 256), (32768, 868), (65536, 2048)])
+def test_phase_batched_graph_matches_gmp(golden, N, W):
+    """Products that share N, L and the primes are captured as five phase-batched
+    launches; two replays, eager products in between, every answer vs GMP."""
+    cases = sorted((c for c in golden.values() if c["N"] == N and c["W"] == W), key=lambda c: c["name"])
+    plans, bufs, keep = [], [], []
+    for case in cases:
+        q, a, b = pointgen.operands(case)
+        if _qbits(q) != W:   # other q sizes have other primes: not one batch
+            continue
+        plans.append(polymult.Plan(N, W, q, form=polymult.FORM_MFMA))
+        bufs.append((polymult.DeviceBuffer.from_array(a), polymult.DeviceBuffer.from_array(b),
+                     polymult.DeviceBuffer(a.nbytes)))
+        keep.append(case)
+    assert len(plans) >= 3
+    say("phase-batched graph", len(plans), "products, N", N, "W", W)
+    batch = polymult.Batch(plans, [x[0] for x in bufs], [x[1] for x in bufs], [x[2] for x in bufs])
+    assert batch.phased
+    for rep in range(2):
+        for _, _, dc in bufs:
+            dc.upload(np.zeros(dc.nbytes // 4, dtype=np.uint32))
+        batch.launch()
+        polymult.synchronize()
+        for case, (_, _, dc) in zip(keep, bufs):
+            assert oracle.digest(dc.download((N, case["L"]))) == case["c_sha"], (rep, case["name"])
+        q0, a0, b0 = pointgen.operands(keep[0])
+        assert oracle.digest(plans[0].multiply(a0, b0)) == keep[0]["c_sha"]
+    batch.close()
+    for p in plans:
+        p.close()
+
+
+def test_phase_batched_chunks_match_single_products():
+    """40 products (two launches of at most 32) at N = 256, W = 128, random q:
+    the phase-batched results equal the product-by-product ones, through the
+    graph and through pm_batch_multiply_device on an explicit stream."""
+    N, W, n = 256, 128, 40
+    pts = [oracle.gen_point(7000 + i, N, W) for i in range(n)]
+    pts = [pt for pt in pts if _qbits(pt[0]) == W]
+    plans = [polymult.Plan(N, W, q, form=polymult.FORM_MFMA) for q, _, _ in pts]
+    L = plans[0].L
+    da = [polymult.DeviceBuffer.from_array(a) for _, a, _ in pts]
+    db = [polymult.DeviceBuffer.from_array(b) for _, _, b in pts]
+    ref = [polymult.DeviceBuffer(4 * N * L) for _ in pts]
+    out = [polymult.DeviceBuffer(4 * N * L) for _ in pts]
+    for p, a, b, c in zip(plans, da, db, ref):
+        p.multiply_device(a, b, c)
+    polymult.synchronize()
+    want = [c.download((N, L)) for c in ref]
+    say("phase-batched chunks", len(plans))
+    batch = polymult.Batch(plans, da, db, out)
+    assert batch.phased
+    batch.launch()
+    polymult.synchronize()
+    for i, c in enumerate(out):
+        assert np.array_equal(c.download((N, L)), want[i]), i
+    stream = polymult.Stream()
+    for c in out:
+        c.upload(np.zeros(N * L, dtype=np.uint32))
+    polymult.batch_multiply_device(plans, da, db, out, stream=stream)
+    stream.synchronize()
+    for i, c in enumerate(out):
+        assert np.array_equal(c.download((N, L)), want[i]), ("stream", i)
+    batch.close()
+    for p in plans:
+        p.close()
