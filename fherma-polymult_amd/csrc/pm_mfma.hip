@@ -260,9 +260,9 @@ __host__ __device__ constexpr uint32_t crt_tab_words(int K) {
 // LDS words, three regions:
 //   R0  y digits [4 KS][64] as 16-byte words (phases 1-2), then the limb
 //       partials (int64, [4 ST][64]), then the output stage (64 x (L + 1));
-//   R1  the digit table (phases 1-2), then the segment exports (1280 words);
-//   aux FP64 partial sums (1024 words), reused for the segment comparisons,
-//       and (v, Ql, Qh) per coefficient (192 words).
+//   R1  the digit table, loaded once and kept for every tile of the workgroup;
+//   aux FP64 partial sums (1024 words), reused in phase 3 for the segment
+//       exports and comparisons, and (v, Ql, Qh) per coefficient (192 words).
 // A-fragment reads for prime groups past the table (J < 4 KS) stay inside
 // R1 + aux: 3 kTabRows < 1216.
 __host__ __device__ constexpr uint32_t crt_r0_words(int KS, int ST) {
@@ -271,7 +271,7 @@ __host__ __device__ constexpr uint32_t crt_r0_words(int KS, int ST) {
 }
 
 __host__ __device__ constexpr uint32_t crt_r1_words(int K) {
-    return crt_tab_words(K) > 1280u ? crt_tab_words(K) : 1280u;
+    return crt_tab_words(K);
 }
 
 __host__ __device__ constexpr uint32_t crt_lds_words(int KS, int ST, int K) {
@@ -296,19 +296,27 @@ __device__ __forceinline__ long long fold_limb(const v4i& a) {
            static_cast<long long>(a[3]) * 16777216;
 }
 
+// A segment's low 64 bits, as far as a carry-in |C| < 2^30 can tell: lo itself
+// below 2^30, lo - 2^64 above 2^64 - 2^30, else kFar.
+constexpr int kFar = -0x7fffffff - 1;
+__device__ __forceinline__ int lo_summary(uint64_t lo) {
+    const long long s = static_cast<long long>(lo);
+    return (lo < (1ull << 30)) ? static_cast<int>(lo) : (s >= -(1ll << 30) && s < 0 ? static_cast<int>(s) : kFar);
+}
+
 // Carry out of a segment of kSegMax limbs whose own chain (carry-in 0) ended
-// with carry-out cout, low two limbs lo and upper limbs all zero (uz) / all
-// ones (uo), when carry-in C (|C| < 2^63) is added instead:
+// with carry-out cout, low-64-bit summary lo_s and upper limbs all zero (uz) /
+// all ones (uo), when carry-in C (|C| < 2^30) is added instead:
 // cout + floor((X + C) / 2^(32 kSegMax)).
-__device__ __forceinline__ long long seg_carry(long long C, long long cout, uint64_t lo, bool uz, bool uo) {
-    long long adj = 0;
-    if (C > 0 && uo && lo + static_cast<uint64_t>(C) < lo) adj = 1;
-    if (C < 0 && uz && lo < static_cast<uint64_t>(-C)) adj = -1;
+__device__ __forceinline__ int seg_carry(int C, int cout, int lo_s, bool uz, bool uo) {
+    int adj = 0;
+    if (C > 0 && uo && lo_s != kFar && lo_s < 0 && lo_s + C >= 0) adj = 1;
+    if (C < 0 && uz && lo_s != kFar && lo_s >= 0 && lo_s < -C) adj = -1;
     return cout + adj;
 }
 
-__device__ __forceinline__ void crt_tile(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, const DevPlan& P,
-                                         uint32_t k0, uint32_t count) {
+__device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, const DevPlan& P,
+                                          uint32_t k0, uint32_t count, uint32_t tile0, uint32_t tstep) {
     extern __shared__ uint32_t smem[];
     const int K = P.K, L = P.L, KS = P.crt_ksteps, ST = P.crt_stiles;
     const int KJ = (K + 3) >> 2;
@@ -317,38 +325,50 @@ __device__ __forceinline__ void crt_tile(const uint32_t* __restrict__ Y, uint32_
     long long* part = reinterpret_cast<long long*>(smem);                      // after phase 2: [limb][64]
     uint32_t* stage = smem;                                                    // after phase 3: 64 x (L + 1)
     uint32_t* tab = smem + r0;                                                 // [KJ][kTabRows]
-    long long* ex64 = reinterpret_cast<long long*>(tab);                       // phase 3: segment exports
-    uint32_t* ex32 = tab + 1024;
     double* fr = reinterpret_cast<double*>(tab + r1);                          // 4 x 64
     double* fq = fr + 4 * 64;                                                  // 4 x 64
-    int* cmpw = reinterpret_cast<int*>(fr);                                    // phase 3: 4 x 64
+    // phase 3, over fr / fq (dead by then): segment exports, 4 x 64 each
+    int* ex_c = reinterpret_cast<int*>(fr);          // carry-out
+    int* ex_s = ex_c + 256;                          // low-64-bit summary
+    uint32_t* ex_f = reinterpret_cast<uint32_t*>(ex_c + 512);   // all-zero / all-ones flags
+    int* cmpw = ex_c + 768;                          // comparison with q
     uint32_t* vq = reinterpret_cast<uint32_t*>(fq + 4 * 64);                   // 3 x 64: v, Ql, Qh
-    const uint32_t nb = k0 + blockIdx.x * 64;
-    const uint32_t nk = min(64u, k0 + count - nb);
+    const uint32_t ntiles = (count + 63) / 64;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: scalar limb indices
 
-    // 1. digit table and y words: every load of the thread in flight at once
+    // 1. the digit table, once per workgroup, and the first tile's y words
     {
         const uint4* tab4 = reinterpret_cast<const uint4*>(P.crt_tab);
         const uint32_t tab4n = crt_tab_words(K) / 4;
         uint4 tv[kTabLoads];
 #pragma unroll
         for (int r = 0; r < kTabLoads; ++r) tv[r] = tab4[min(threadIdx.x + r * blockDim.x, tab4n - 1)];
+        // past the table's end every thread rewrites its last entry with the same value
+        uint4* tab4s = reinterpret_cast<uint4*>(tab);
+#pragma unroll
+        for (int r = 0; r < kTabLoads; ++r) tab4s[min(threadIdx.x + r * blockDim.x, tab4n - 1)] = tv[r];
+    }
+    // prime groups J = wave + 4 i, i < KS: primes 4 J + r; all loads of a tile in flight at once
+    constexpr int kRows = 4 * kCrtMaxKSteps;
+    uint32_t y[kRows];
+    auto load_y = [&](uint32_t tile) {
+        const uint32_t nb = k0 + tile * 64, nk = min(64u, k0 + count - nb);
         const uint32_t col = nb + min(static_cast<uint32_t>(lane), nk - 1);
-        const bool lane_in = static_cast<uint32_t>(lane) < nk;
-        // prime groups J = wave + 4 i, i < KS: primes 4 J + r
-        constexpr int kRows = 4 * kCrtMaxKSteps;
-        uint32_t y[kRows];
 #pragma unroll
         for (int i = 0; i < kRows; ++i) {
             const int j = 4 * (wave + 4 * (i / 4)) + (i % 4);
             y[i] = Y[static_cast<size_t>(min(j, K - 1)) * P.N + col];
         }
-        // past the table's end every thread rewrites its last entry with the same value
-        uint4* tab4s = reinterpret_cast<uint4*>(tab);
-#pragma unroll
-        for (int r = 0; r < kTabLoads; ++r) tab4s[min(threadIdx.x + r * blockDim.x, tab4n - 1)] = tv[r];
+    };
+    if (tile0 < ntiles) load_y(tile0);
+    // Tiles tile0, tile0 + tstep, ...: the next tile's y words are loaded into
+    // registers while this one runs its MFMA and carry phases.
+    for (uint32_t tile = tile0; tile < ntiles; tile += tstep) {
+    const uint32_t nb = k0 + tile * 64;
+    const uint32_t nk = min(64u, k0 + count - nb);
+    {
+        const bool lane_in = static_cast<uint32_t>(lane) < nk;
         double f = 0, f2 = 0;
 #pragma unroll
         for (int i = 0; i < kCrtMaxKSteps; ++i) {
@@ -384,6 +404,7 @@ __device__ __forceinline__ void crt_tile(const uint32_t* __restrict__ Y, uint32_
         vq[64 + lane] = static_cast<uint32_t>(Q);
         vq[128 + lane] = static_cast<uint32_t>(Q >> 32);
     }
+    if (tile + tstep < ntiles) load_y(tile + tstep);
 
     // 2. limb partials: tiles st = wave + 4 i (i < 4) for all four column tiles,
     // and the 17th limb tile (st = 16) for column tile t = wave. Tiles past ST
@@ -486,20 +507,19 @@ __device__ __forceinline__ void crt_tile(const uint32_t* __restrict__ Y, uint32_
             uz = uz && x[u] == 0u;
             uo = uo && x[u] == 0xffffffffu;
         }
-        ex64[wave * 64 + lane] = carry;
-        ex64[256 + wave * 64 + lane] = static_cast<long long>(static_cast<uint64_t>(x[0]) | (static_cast<uint64_t>(x[1]) << 32));
-        ex32[wave * 64 + lane] = (uz ? 1u : 0u) | (uo ? 2u : 0u);
+        ex_c[wave * 64 + lane] = static_cast<int>(carry);   // |carry| < 2^17
+        ex_s[wave * 64 + lane] = lo_summary(static_cast<uint64_t>(x[0]) | (static_cast<uint64_t>(x[1]) << 32));
+        ex_f[wave * 64 + lane] = (uz ? 1u : 0u) | (uo ? 2u : 0u);
     }
     __syncthreads();
-    long long Cin = 0, Ctop = 0;
+    int Cin = 0, Ctop = 0;
     {
-        long long C = 0;
+        int C = 0;
 #pragma unroll
         for (int w = 0; w < 4; ++w) {
             Cin = w == wave ? C : Cin;
-            const uint32_t fl = ex32[w * 64 + lane];
-            C = seg_carry(C, ex64[w * 64 + lane], static_cast<uint64_t>(ex64[256 + w * 64 + lane]), fl & 1u,
-                          (fl & 2u) != 0u);
+            const uint32_t fl = ex_f[w * 64 + lane];
+            C = seg_carry(C, ex_c[w * 64 + lane], ex_s[w * 64 + lane], fl & 1u, (fl & 2u) != 0u);
         }
         Ctop = C;   // R's sign: |R| < 2^(32 L + 1)
     }
@@ -543,17 +563,17 @@ __device__ __forceinline__ void crt_tile(const uint32_t* __restrict__ Y, uint32_
             ao = ao && x[u] == 0xffffffffu;
         }
         // exports of the first resolution are dead (every wave read them before the last barrier)
-        ex64[wave * 64 + lane] = cc;
-        ex32[wave * 64 + lane] = (az ? 1u : 0u) | (ao ? 2u : 0u);
+        ex_c[wave * 64 + lane] = static_cast<int>(cc);
+        ex_f[wave * 64 + lane] = (az ? 1u : 0u) | (ao ? 2u : 0u);
     }
     __syncthreads();
     {
         long long C = 0;
 #pragma unroll
         for (int w = 0; w < 3; ++w) {
-            const uint32_t fl = ex32[w * 64 + lane];
+            const uint32_t fl = ex_f[w * 64 + lane];
             const long long over = (C == 1 && (fl & 2u)) ? 1 : ((C == -1 && (fl & 1u)) ? -1 : 0);
-            C = w < wave ? ex64[w * 64 + lane] + over : C;
+            C = w < wave ? ex_c[w * 64 + lane] + over : C;
         }
 #pragma unroll
         for (int u = 0; u < kSegMax; ++u) {
@@ -573,11 +593,13 @@ __device__ __forceinline__ void crt_tile(const uint32_t* __restrict__ Y, uint32_
 #pragma unroll 4
         for (uint32_t r = wave; r < nk; r += kWaves) dst[r * L + lane] = stage[r * (L + 1) + lane];
     }
+    __syncthreads();   // the stores read the stage (region 0) before the next tile's digits overwrite it
+    }
 }
 
 __global__ void __launch_bounds__(64 * kWaves)
 k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, uint32_t k0, uint32_t count) {
-    crt_tile(Y, c, P, k0, count);
+    crt_tiles(Y, c, P, k0, count, blockIdx.x, gridDim.x);
 }
 
 // A batch: blockIdx.z = product, with that product's q-dependent tables.
@@ -588,7 +610,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_crt_mfma_batch(DevPlan P, Batch
     Q.crt_fc = ba.crt_fc[i];
     Q.nq = ba.nq[i];
     Q.crt_lc = ba.crt_lc[i];
-    crt_tile(ba.A[i], ba.c[i], Q, 0, P.N);
+    crt_tiles(ba.A[i], ba.c[i], Q, 0, P.N, blockIdx.x, gridDim.x);
 }
 
 // Balanced base-256 digits of a signed value |x| < 2^31.
@@ -724,12 +746,28 @@ void launch_crt_mfma(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream
     launch_crt_mfma_range(P, Y, c, 0, P.N, st);
 }
 
+// Workgroups of the CRT kernel that fit on the current device at once: its LDS
+// allows floor(160 KiB / lds) per CU (the workgroups are persistent and loop
+// over 64-coefficient tiles).
+uint32_t crt_slots(size_t lds) {
+    static thread_local int dev = -1, cus = 0;
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess) throw std::runtime_error("hipGetDevice failed");
+    if (d != dev) {
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d) != hipSuccess || cus <= 0)
+            throw std::runtime_error("hipDeviceGetAttribute failed");
+        dev = d;
+    }
+    const size_t per_cu = std::max<size_t>(1, (160u * 1024u) / lds);
+    return static_cast<uint32_t>(per_cu * static_cast<size_t>(cus));
+}
+
 void launch_crt_mfma_range(const DevPlan& P, const uint32_t* Y, uint32_t* c, uint32_t k0, uint32_t count,
                            hipStream_t st) {
     if (!P.crt_tab) throw std::runtime_error("matrix-core tables not prepared");
     if (count == 0) return;
-    const dim3 grid((count + 63) / 64);
     const size_t lds = sizeof(uint32_t) * crt_lds_words(P.crt_ksteps, P.crt_stiles, P.K);
+    const dim3 grid(std::min((count + 63) / 64, crt_slots(lds)));
     launch(k_crt_mfma, grid, dim3(64 * kWaves), lds, st, Y, c, P, k0, count);
 }
 
@@ -748,8 +786,9 @@ void launch_decompose_mfma_batch(const DevPlan& P, const BatchArgs& ba, int coun
 
 void launch_crt_mfma_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st) {
     if (!P.crt_tab) throw std::runtime_error("matrix-core tables not prepared");
-    const dim3 grid((P.N + 63) / 64, 1, count);
     const size_t lds = sizeof(uint32_t) * crt_lds_words(P.crt_ksteps, P.crt_stiles, P.K);
+    const uint32_t per_product = std::max<uint32_t>(1, crt_slots(lds) / static_cast<uint32_t>(count));
+    const dim3 grid(std::min((P.N + 63) / 64, per_product), 1, count);
     launch(k_crt_mfma_batch, grid, dim3(64 * kWaves), lds, st, P, ba);
 }
 
