@@ -292,7 +292,8 @@ def main():
         "dtype": "u32", "data": "synthetic",
         "config": {"workload": "batch of 16 products, N=65536, random odd 2048-bit q (L=64, K=134), "
                                "device-resident", "form": "int8-MFMA decompose/CRT, batch replayed as one hipGraph", "batch": BATCH, "N": N, "W": W, "L": L, "K": K,
-                   "parallelism": "single GPU", "device": name, "cus": cus},
+                   "parallelism": "single GPU", "device": name, "cus": cus,
+                   "row_twiddles": "generated" if plans[0].twiddles == polymult.TWIDDLES_GEN else "table"},
         "plain_value": None if plain_value is None else round(plain_value, 2),
         "mfma_over_plain": None if plain_value is None else round(value / plain_value, 3),
         "mismatches": int(mismatches),

@@ -127,12 +127,25 @@ void need(bool ok, const char* msg) {
     if (!ok) throw std::invalid_argument(msg);
 }
 
+// Row-pass twiddle source, the reference's rule (rns_plan.h:269-280):
+// generated when the full tables (16 K N bytes) exceed 64 MB, else read from
+// them; FHERMA_TWIDDLES=table|gen|auto overrides, anything else is refused.
+int row_twiddle_policy(const pm::HostPlan& h) {
+    if (const char* v = std::getenv("FHERMA_TWIDDLES")) {
+        if (!std::strcmp(v, "gen")) return 1;
+        if (!std::strcmp(v, "table")) return 0;
+        need(!std::strcmp(v, "auto"), "FHERMA_TWIDDLES must be table, gen or auto");
+    }
+    return 16.0 * h.K * h.N > 64e6 ? 1 : 0;
+}
+
 pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q) {
     need(q != nullptr, "q is null");
     std::unique_ptr<pm_plan> pl(new pm_plan());
     pl->host = pm::make_plan(N, W, q, static_cast<int>(L));
     const pm::HostPlan& h = pl->host;
     const int K = h.K;
+    const int row_gen = row_twiddle_policy(h);   // before any device work
 
     std::vector<pm::PrimeConst> pc(K);
     for (int j = 0; j < K; ++j) {
@@ -158,6 +171,7 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q)
     d.logR = h.logR;
     d.L = h.L;
     d.K = K;
+    d.row_gen = row_gen;
     d.q_drop = h.q_drop;
     d.q_top = h.q_top;
     d.pc = pl->upload(pc);
@@ -324,6 +338,19 @@ int pm_choose_primes(uint32_t N, uint32_t qbits, uint32_t* out, uint32_t cap, ui
         std::memcpy(out, p.data(), p.size() * sizeof(uint32_t));
         *K = static_cast<uint32_t>(p.size());
     });
+}
+
+int pm_plan_set_twiddles(pm_plan* plan, int source) {
+    return guarded([&] {
+        need(plan != nullptr, "plan is null");
+        need(source == PM_TWIDDLES_TABLE || source == PM_TWIDDLES_GEN,
+             "source must be PM_TWIDDLES_TABLE or PM_TWIDDLES_GEN");
+        plan->dev.row_gen = source == PM_TWIDDLES_GEN ? 1 : 0;
+    });
+}
+
+int pm_plan_get_twiddles(const pm_plan* plan) {
+    return plan ? (plan->dev.row_gen ? PM_TWIDDLES_GEN : PM_TWIDDLES_TABLE) : -1;
 }
 
 int pm_plan_set_form(pm_plan* plan, int form) {

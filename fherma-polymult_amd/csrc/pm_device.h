@@ -24,7 +24,8 @@ struct PrimeConst {
 
 struct DevPlan {
     uint32_t N, logN, logS, logR;
-    int L, K, q_drop, pad;
+    int L, K, q_drop;
+    int row_gen;              // row-pass twiddles: 1 generated (rz, rom), 0 from the full tables
     double q_top;
     const PrimeConst* pc;     // K
     const uint32_t* pow2;     // K x L: 2^(32 l) mod p_j

@@ -91,12 +91,26 @@ struct LdsTwA {
     }
 };
 
-template <int LOGM>
-__device__ __forceinline__ void fill_row_phase_a(uint2* tab, const GenTw& g, uint32_t t) {
+template <int LOGM, class TW>
+__device__ __forceinline__ void fill_row_phase_a(uint2* tab, const TW& g, uint32_t t) {
     constexpr uint32_t n = (1u << (LOGM - 4)) - 1;
     if (t < n) {
         const uint32_t m = 31 - __clz(t + 1);   // floor(log2(t + 1))
         tab[t] = g(LOGM - 1 - m, t + 1 - (1u << m));
+    }
+}
+
+// The row pass's twiddles for row rho of prime j: generated from the per-row
+// and per-prime factors (kGen), or read from the full table, whose index for
+// row rho is ((N + rho S) >> (e + 1)) + b.
+template <int LOGM, bool kGen>
+__device__ __forceinline__ auto row_tw(const DevPlan& P, int j, uint32_t rho, bool fwd, const PrimeConst& pc) {
+    if constexpr (kGen) {
+        const uint32_t R = 1u << P.logR;
+        return GenTw{(fwd ? P.rz : P.rzi) + (static_cast<size_t>(j) * R + rho) * LOGM,
+                     (fwd ? P.rom : P.romi) + static_cast<size_t>(j) * ((1u << LOGM) / 2), pc};
+    } else {
+        return TableTw{(fwd ? P.tw : P.itw) + static_cast<size_t>(j) * P.N, P.N + (rho << LOGM)};
     }
 }
 
@@ -251,9 +265,9 @@ __global__ void __launch_bounds__(512) k_ntt_col_batch(DevPlan P, BatchArgs ba) 
 }
 
 // Row pass alone (forward or inverse): G = 2^logG rows of M = S elements.
-// Twiddles are generated per row (GenTw); phase A's are computed once per row
-// into LDS by the row's first threads.
-template <int LOGM, bool kFwd>
+// Twiddles are generated per row (kGen) or read from the full table; phase
+// A's are staged once per row into LDS by the row's first threads.
+template <int LOGM, bool kFwd, bool kGen>
 __global__ void __launch_bounds__(512)
 k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
     extern __shared__ uint32_t lds[];
@@ -264,9 +278,7 @@ k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
     const int j = blockIdx.y;
     uint32_t* x = X + static_cast<size_t>(j) * P.N + static_cast<size_t>(rho) * M;
     const PrimeConst pc = P.pc[j];
-    const uint32_t R = 1u << P.logR;
-    const GenTw gen{(kFwd ? P.rz : P.rzi) + (static_cast<size_t>(j) * R + rho) * LOGM,
-                    (kFwd ? P.rom : P.romi) + static_cast<size_t>(j) * (M / 2), pc};
+    const auto gen = row_tw<LOGM, kGen>(P, j, rho, kFwd, pc);
     uint32_t* l = lds + rl * ROW;
     uint2* twa = reinterpret_cast<uint2*>(lds + G * ROW) + rl * GA;
     fill_row_phase_a<LOGM>(twa, gen, t);
@@ -318,7 +330,7 @@ __device__ __forceinline__ uint32_t pointwise_mul(uint32_t a, uint32_t b, const 
 
 // Row pass fused: forward rows of A and B (sharing twiddle loads), the pointwise
 // product with N^-1 (P/p_j)^-1 into A, inverse rows of A.
-template <int LOGM>
+template <int LOGM, bool kGen>
 __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, const DevPlan& P,
                                            uint32_t logG) {
     extern __shared__ uint32_t lds[];
@@ -329,10 +341,8 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
     const int j = blockIdx.y;
     const size_t row = static_cast<size_t>(j) * P.N + static_cast<size_t>(rho) * M;
     const PrimeConst pc = P.pc[j];
-    const uint32_t R = 1u << P.logR;
-    const size_t zoff = (static_cast<size_t>(j) * R + rho) * LOGM, ooff = static_cast<size_t>(j) * (M / 2);
-    const GenTw fw{P.rz + zoff, P.rom + ooff, pc};
-    const GenTw iw{P.rzi + zoff, P.romi + ooff, pc};
+    const auto fw = row_tw<LOGM, kGen>(P, j, rho, true, pc);
+    const auto iw = row_tw<LOGM, kGen>(P, j, rho, false, pc);
     uint32_t* la = lds + rl * ROW;
     uint32_t* lb = lds + (G + rl) * ROW;
     uint2* twa = reinterpret_cast<uint2*>(lds + 2 * G * ROW) + rl * 2 * GA;
@@ -381,16 +391,16 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
         for (uint32_t k = 0; k < GA; ++k) A[row + t + TPS * i + 16 * k] = a[i * GA + k];
 }
 
-template <int LOGM>
+template <int LOGM, bool kGen>
 __global__ void __launch_bounds__(512)
 k_ntt_rows_fused_reg(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, DevPlan P, uint32_t logG) {
-    rows_fused<LOGM>(A, Bm, P, logG);
+    rows_fused<LOGM, kGen>(A, Bm, P, logG);
 }
 
 // A batch: blockIdx.z = product.
-template <int LOGM>
+template <int LOGM, bool kGen>
 __global__ void __launch_bounds__(512) k_ntt_rows_fused_batch(DevPlan P, BatchArgs ba, uint32_t logG) {
-    rows_fused<LOGM>(ba.A[blockIdx.z], ba.B[blockIdx.z], P, logG);
+    rows_fused<LOGM, kGen>(ba.A[blockIdx.z], ba.B[blockIdx.z], P, logG);
 }
 
 bool reg_ok(uint32_t logm) { return logm >= 4 && logm <= 8; }
@@ -460,10 +470,14 @@ void row_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     const dim3 block(1u << (logG + LOGM - 4));
     const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (sizeof(uint2) << (LOGM - 4 + logG));
-    if (forward)
-        launch((k_ntt_row_reg<LOGM, true>), grid, block, lds, st, X, P, logG);
+    if (forward && P.row_gen)
+        launch((k_ntt_row_reg<LOGM, true, true>), grid, block, lds, st, X, P, logG);
+    else if (forward)
+        launch((k_ntt_row_reg<LOGM, true, false>), grid, block, lds, st, X, P, logG);
+    else if (P.row_gen)
+        launch((k_ntt_row_reg<LOGM, false, true>), grid, block, lds, st, X, P, logG);
     else
-        launch((k_ntt_row_reg<LOGM, false>), grid, block, lds, st, X, P, logG);
+        launch((k_ntt_row_reg<LOGM, false, false>), grid, block, lds, st, X, P, logG);
 }
 
 template <int LOGM>
@@ -473,7 +487,10 @@ void fused_launch_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStr
     const dim3 block(1u << (logG + LOGM - 4));
     const size_t lds = 2 * (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (2 * sizeof(uint2) << (LOGM - 4 + logG));
-    launch((k_ntt_rows_fused_batch<LOGM>), grid, block, lds, st, P, ba, logG);
+    if (P.row_gen)
+        launch((k_ntt_rows_fused_batch<LOGM, true>), grid, block, lds, st, P, ba, logG);
+    else
+        launch((k_ntt_rows_fused_batch<LOGM, false>), grid, block, lds, st, P, ba, logG);
 }
 
 template <int LOGM>
@@ -483,7 +500,10 @@ void fused_launch(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t 
     const dim3 block(1u << (logG + LOGM - 4));
     const size_t lds = 2 * (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (2 * sizeof(uint2) << (LOGM - 4 + logG));
-    launch((k_ntt_rows_fused_reg<LOGM>), grid, block, lds, st, A, B, P, logG);
+    if (P.row_gen)
+        launch((k_ntt_rows_fused_reg<LOGM, true>), grid, block, lds, st, A, B, P, logG);
+    else
+        launch((k_ntt_rows_fused_reg<LOGM, false>), grid, block, lds, st, A, B, P, logG);
 }
 
 }  // namespace

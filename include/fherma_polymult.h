@@ -66,6 +66,18 @@ int pm_choose_primes(uint32_t N, uint32_t qbits, uint32_t* out, uint32_t cap, ui
 /* Selects the plain or the matrix-core form for later calls on this plan. */
 int pm_plan_set_form(pm_plan* plan, int form);
 
+/* Row-pass twiddle source (reference rns_plan.h:269-280): the full K x N
+ * tables, or generation from per-row and per-prime factors. Chosen at plan
+ * creation by the reference's rule (generate when 16 K N bytes exceed 64 MB)
+ * unless FHERMA_TWIDDLES=table|gen|auto says otherwise (any other value makes
+ * plan creation fail with PM_EINVAL); both give identical results. */
+typedef enum {
+    PM_TWIDDLES_TABLE = 0,
+    PM_TWIDDLES_GEN = 1
+} pm_twiddles;
+int pm_plan_set_twiddles(pm_plan* plan, int source);
+int pm_plan_get_twiddles(const pm_plan* plan);   /* pm_twiddles, or -1 for a null plan */
+
 /* --- products -------------------------------------------------------------
  * pm_multiply: host pointers, synchronous; the fherma_run equivalent
  * (reference solve.cu:871-1059). Inputs cross PCIe through pinned staging.

@@ -234,3 +234,19 @@ def test_phase_batched_chunks_match_single_products():
     batch.close()
     for p in plans:
         p.close()
+
+
+@pytest.mark.parametrize("name", ["config-n4096-w256-s1", "config-n32768-w868-s1", "config-n65536-w2048-01",
+                                  "matrix-n131072-w512-prime"])
+def test_row_twiddle_sources_match_gmp(golden, name):
+    """Both row-pass twiddle sources (full tables, generated) and both forms
+    give the GMP product (the reference's fuzz runs tw-gen and tw-table)."""
+    case = golden[name]
+    q, a, b = pointgen.operands(case)
+    for form in (polymult.FORM_PLAIN, polymult.FORM_MFMA):
+        with polymult.Plan(case["N"], case["W"], q, form=form) as plan:
+            for source in (polymult.TWIDDLES_TABLE, polymult.TWIDDLES_GEN):
+                plan.set_twiddles(source)
+                assert plan.twiddles == source
+                say("twiddles", name, "form", form, "source", source)
+                assert oracle.digest(plan.multiply(a, b)) == case["c_sha"], (name, form, source)

@@ -4,6 +4,7 @@ logic agrees with the oracle. No kernel is launched here."""
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -87,3 +88,17 @@ def test_driver_refuses_out_of_range_points(tmp_path):
         r = subprocess.run([solution, str(d)], capture_output=True, text=True, timeout=60)
         assert r.returncode != 0
         assert not (d / "out" / "000000").exists()
+
+
+def test_invalid_twiddle_policy_is_refused_before_device_work():
+    """FHERMA_TWIDDLES other than table, gen or auto fails plan creation with
+    PM_EINVAL, as the reference's fherma_init does (rns_plan.h:273-280)."""
+    code = ("import sys; sys.path.insert(0, %r); import numpy as np, polymult\n"
+            "try:\n"
+            "    polymult.Plan(64, 64, np.array([7, 1], dtype=np.uint32))\n"
+            "except polymult.PolymulError as e:\n"
+            "    print(e.code, str(e))\n" % os.path.join(ROOT, "fherma-polymult_amd"))
+    env = dict(os.environ, FHERMA_TWIDDLES="fast")
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert out.stdout.startswith("%d " % polymult.PM_EINVAL), out.stdout + out.stderr
+    assert "FHERMA_TWIDDLES" in out.stdout
