@@ -158,6 +158,14 @@ __device__ __forceinline__ void phase_b(uint32_t (&x)[16], uint32_t (&y)[16], co
 
 __device__ __forceinline__ uint32_t padded(uint32_t u) { return u + (u >> 4); }
 
+// The M / 16 threads of a row are consecutive lanes of one wave, so a row's
+// LDS exchange needs only wave-level ordering (no workgroup barrier).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // A raw buffer over `bytes` bytes from `base` (gfx9 resource word 3): a lane's
 // address is its byte offset plus a wave-uniform offset in a scalar register,
 // so strided accesses cost no per-access address arithmetic, and accesses past
@@ -343,9 +351,9 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
     const PrimeConst pc = P.pc[j];
     const auto fw = row_tw<LOGM, kGen>(P, j, rho, true, pc);
     const auto iw = row_tw<LOGM, kGen>(P, j, rho, false, pc);
+    // one exchange buffer per row, used by a and then b
     uint32_t* la = lds + rl * ROW;
-    uint32_t* lb = lds + (G + rl) * ROW;
-    uint2* twa = reinterpret_cast<uint2*>(lds + 2 * G * ROW) + rl * 2 * GA;
+    uint2* twa = reinterpret_cast<uint2*>(lds + G * ROW) + rl * 2 * GA;
     fill_row_phase_a<LOGM>(twa, fw, t);
     fill_row_phase_a<LOGM>(twa + GA, iw, t);
     const LdsTwA<LOGM> fa{twa}, ia{twa + GA};
@@ -357,29 +365,31 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
             a[i * GA + k] = A[row + t + TPS * i + 16 * k];
             b[i * GA + k] = Bm[row + t + TPS * i + 16 * k];
         }
-    __syncthreads();
+    wave_sync();
     phase_a<LOGM, true, true>(a, b, fa, pc.p);
 #pragma unroll
     for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
-        for (uint32_t k = 0; k < GA; ++k) {
-            la[padded(t + TPS * i + 16 * k)] = a[i * GA + k];
-            lb[padded(t + TPS * i + 16 * k)] = b[i * GA + k];
-        }
-    __syncthreads();
+        for (uint32_t k = 0; k < GA; ++k) la[padded(t + TPS * i + 16 * k)] = a[i * GA + k];
+    wave_sync();
 #pragma unroll
-    for (uint32_t k = 0; k < 16; ++k) {
-        a[k] = la[padded(16 * t + k)];
-        b[k] = lb[padded(16 * t + k)];
-    }
+    for (uint32_t k = 0; k < 16; ++k) a[k] = la[padded(16 * t + k)];
+    wave_sync();
+#pragma unroll
+    for (uint32_t i = 0; i < NG; ++i)
+#pragma unroll
+        for (uint32_t k = 0; k < GA; ++k) la[padded(t + TPS * i + 16 * k)] = b[i * GA + k];
+    wave_sync();
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) b[k] = la[padded(16 * t + k)];
     phase_b<true, true>(a, b, fw, t, pc.p);
 #pragma unroll
     for (uint32_t k = 0; k < 16; ++k) a[k] = pointwise_mul(a[k], b[k], pc);
     phase_b<false, false>(a, a, iw, t, pc.p);
-    // Each thread rewrites only the LDS words it read above: no barrier needed first.
+    // Each thread rewrites only the LDS words it read above (b's, phase B layout).
 #pragma unroll
     for (uint32_t k = 0; k < 16; ++k) la[padded(16 * t + k)] = a[k];
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -485,7 +495,7 @@ void fused_launch_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStr
     const uint32_t logG = row_logG(P);
     const dim3 grid(1u << (P.logR - logG), P.K, count);
     const dim3 block(1u << (logG + LOGM - 4));
-    const size_t lds = 2 * (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
+    const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (2 * sizeof(uint2) << (LOGM - 4 + logG));
     if (P.row_gen)
         launch((k_ntt_rows_fused_batch<LOGM, true>), grid, block, lds, st, P, ba, logG);
@@ -498,7 +508,7 @@ void fused_launch(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t 
     const uint32_t logG = row_logG(P);
     const dim3 grid(1u << (P.logR - logG), P.K);
     const dim3 block(1u << (logG + LOGM - 4));
-    const size_t lds = 2 * (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
+    const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (2 * sizeof(uint2) << (LOGM - 4 + logG));
     if (P.row_gen)
         launch((k_ntt_rows_fused_reg<LOGM, true>), grid, block, lds, st, A, B, P, logG);
