@@ -290,10 +290,11 @@ def main():
         "value": None if value is None else round(value, 2), "unit": "polymuls/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": None if sec is None else round(sec * 1e3, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32", "data": "synthetic",
-        "config": {"workload": "batch of 16 products, N=65536, random odd 2048-bit q (L=64, K=134), "
-                               "device-resident", "form": "int8-MFMA decompose/CRT, batch replayed as one hipGraph", "batch": BATCH, "N": N, "W": W, "L": L, "K": K,
+        "config": {"workload": "batch of 16 products, N=65536, random odd 2048-bit q (L=64), device-resident", "form": "int8-MFMA decompose/CRT, batch replayed as one hipGraph", "batch": BATCH, "N": N, "W": W, "L": L, "K": K,
                    "parallelism": "single GPU", "device": name, "cus": cus,
-                   "row_twiddles": "generated" if plans[0].twiddles == polymult.TWIDDLES_GEN else "table"},
+                   "row_twiddles": "generated" if plans[0].twiddles == polymult.TWIDDLES_GEN else "table",
+                   "prime_basis": "p < 2^%d" % plans[0].basis,
+                   "K_reference_basis": int(len(polymult.choose_primes(N, W)))},
         "plain_value": None if plain_value is None else round(plain_value, 2),
         "mfma_over_plain": None if plain_value is None else round(value / plain_value, 3),
         "mismatches": int(mismatches),

@@ -139,10 +139,49 @@ int row_twiddle_policy(const pm::HostPlan& h) {
     return 16.0 * h.K * h.N > 64e6 ? 1 : 0;
 }
 
-pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q) {
+// The prime basis of a new plan. AUTO (after PM_BASIS=p30|p31|auto, anything
+// else refused): primes below 2^30 when the geometry has register kernels for
+// every NTT pass, the only kernels with lazily reduced butterflies, and enough
+// such primes exist; otherwise the reference's primes below 2^31.
+pm::HostPlan plan_in_basis(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, int basis, bool& lazy) {
+    if (basis == PM_BASIS_AUTO) {
+        if (const char* v = std::getenv("PM_BASIS")) {
+            if (!std::strcmp(v, "p30"))
+                basis = PM_BASIS_P30;
+            else if (!std::strcmp(v, "p31"))
+                basis = PM_BASIS_P31;
+            else
+                need(!std::strcmp(v, "auto"), "PM_BASIS must be p30, p31 or auto");
+        }
+    }
+    need(basis == PM_BASIS_AUTO || basis == PM_BASIS_P30 || basis == PM_BASIS_P31,
+         "basis must be PM_BASIS_AUTO, PM_BASIS_P30 or PM_BASIS_P31");
+    uint32_t logN = 0;
+    while (logN < 31 && (1u << logN) < N) ++logN;
+    pm::DevPlan g{};
+    pm::ntt_geometry(logN, g.logS, g.logR);
+    const bool kernels = pm::batch_kernels_ok(g);
+    pm::HostPlan h;
+    if (basis == PM_BASIS_AUTO && kernels) {
+        try {
+            h = pm::make_plan(N, W, q, static_cast<int>(L), 30);
+        } catch (const std::runtime_error&) {
+            // too few primes below 2^30 (or an invalid point, which the
+            // reference's basis refuses again with the reference's message)
+            h = pm::make_plan(N, W, q, static_cast<int>(L), 31);
+        }
+    } else {
+        h = pm::make_plan(N, W, q, static_cast<int>(L), basis == PM_BASIS_P30 ? 30 : 31);
+    }
+    lazy = kernels && h.pbits == 30;
+    return h;
+}
+
+pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, int basis) {
     need(q != nullptr, "q is null");
     std::unique_ptr<pm_plan> pl(new pm_plan());
-    pl->host = pm::make_plan(N, W, q, static_cast<int>(L));
+    bool lazy = false;
+    pl->host = plan_in_basis(N, W, L, q, basis, lazy);
     const pm::HostPlan& h = pl->host;
     const int K = h.K;
     const int row_gen = row_twiddle_policy(h);   // before any device work
@@ -172,6 +211,7 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q)
     d.L = h.L;
     d.K = K;
     d.row_gen = row_gen;
+    d.lazy = lazy ? 1 : 0;
     d.q_drop = h.q_drop;
     d.q_top = h.q_top;
     d.pc = pl->upload(pc);
@@ -303,12 +343,18 @@ extern "C" {
 const char* pm_last_error(void) { return g_err.c_str(); }
 
 int pm_plan_create(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, pm_plan** out) {
+    return pm_plan_create_basis(N, W, L, q, PM_BASIS_AUTO, out);
+}
+
+int pm_plan_create_basis(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, int basis, pm_plan** out) {
     return guarded([&] {
         need(out != nullptr, "out is null");
         *out = nullptr;
-        *out = make_plan_object(N, W, L, q);
+        *out = make_plan_object(N, W, L, q, basis);
     });
 }
+
+int pm_plan_basis(const pm_plan* plan) { return plan ? plan->host.pbits : -1; }
 
 void pm_plan_destroy(pm_plan* plan) { delete plan; }
 
@@ -330,10 +376,15 @@ int pm_plan_primes(const pm_plan* plan, uint32_t* out, uint32_t cap) {
 }
 
 int pm_choose_primes(uint32_t N, uint32_t qbits, uint32_t* out, uint32_t cap, uint32_t* K) {
+    return pm_choose_primes_basis(N, qbits, PM_BASIS_P31, out, cap, K);
+}
+
+int pm_choose_primes_basis(uint32_t N, uint32_t qbits, int basis, uint32_t* out, uint32_t cap, uint32_t* K) {
     return guarded([&] {
         need(out != nullptr && K != nullptr, "null argument");
         need(N >= 2 && (N & (N - 1)) == 0 && N <= (1u << pm::kMaxLogN), "N must be a power of two in [2, 2^22]");
-        const std::vector<uint32_t> p = pm::choose_primes(N, static_cast<int>(qbits));
+        need(basis == PM_BASIS_P30 || basis == PM_BASIS_P31, "basis must be PM_BASIS_P30 or PM_BASIS_P31");
+        const std::vector<uint32_t> p = pm::choose_primes(N, static_cast<int>(qbits), basis);
         need(p.size() <= cap, "cap is smaller than K");
         std::memcpy(out, p.data(), p.size() * sizeof(uint32_t));
         *K = static_cast<uint32_t>(p.size());

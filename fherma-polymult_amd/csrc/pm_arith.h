@@ -1,9 +1,11 @@
 // Word arithmetic modulo a prime p < 2^31, shared by the host plan and the
 // gfx950 kernels.
 //
-// Every residue is kept canonical in [0, p): with p this close to 2^31 there is
-// no headroom in a 32-bit word for lazily reduced sums (2p + 2p > 2^32), so each
-// butterfly reduces its sum and difference with one unsigned min each.
+// With the reference's primes (just below 2^31) every residue is kept canonical
+// in [0, p): there is no headroom in a 32-bit word for lazily reduced sums
+// (2p + 2p > 2^32), so each butterfly reduces its sum and difference with one
+// unsigned min each. Primes below 2^30 leave that headroom; the NTT kernels'
+// lazily reduced butterflies for them are in pm_ntt.hip.
 //
 // Reference counterparts: rns.cuh:46-93 (mulhi, add_mod, sub_mod, mul_shoup,
 // mont_mul, bitrev_bits).
@@ -57,14 +59,18 @@ PM_HD uint32_t mul_shoup_lazy(uint32_t a, uint32_t w, uint32_t ws, uint32_t p) {
     return a * w - mulhi32(a, ws) * p;
 }
 
-// a * b * 2^-32 mod p for a, b < p; pneg = -p^-1 mod 2^32.
-PM_HD uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t p, uint32_t pneg) {
+// a * b * 2^-32 mod p, left in [0, 2p), for a b < 2^32 p; pneg = -p^-1 mod 2^32.
+PM_HD uint32_t mont_mul_lazy(uint32_t a, uint32_t b, uint32_t p, uint32_t pneg) {
     const uint32_t lo = a * b;
     const uint32_t hi = mulhi32(a, b);
     const uint32_t m = lo * pneg;
     // (a b + m p) / 2^32: the low words cancel, with a carry out unless lo == 0.
-    const uint32_t u = hi + mulhi32(m, p) + (lo != 0u);
-    return csub(u, p);
+    return hi + mulhi32(m, p) + (lo != 0u);
+}
+
+// The same reduced to [0, p), for a, b < p.
+PM_HD uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t p, uint32_t pneg) {
+    return csub(mont_mul_lazy(a, b, p, pneg), p);
 }
 
 PM_HD uint32_t shoup_companion(uint32_t w, uint32_t p) {

@@ -26,6 +26,9 @@ struct DevPlan {
     uint32_t N, logN, logS, logR;
     int L, K, q_drop;
     int row_gen;              // row-pass twiddles: 1 generated (rz, rom), 0 from the full tables
+    // 1: every prime is below 2^30 and every pass has a register kernel, which
+    // then keep NTT-domain residues lazily reduced (pm_ntt.hip); 0: canonical.
+    int lazy;
     double q_top;
     const PrimeConst* pc;     // K
     const uint32_t* pow2;     // K x L: 2^(32 l) mod p_j

@@ -41,10 +41,37 @@ __device__ __forceinline__ void gs(uint32_t& x, uint32_t& y, const uint2 w, uint
     y = mul_shoup(u - y + p, w.x, w.y, p);
 }
 
-template <bool kFwd>
+// Lazily reduced butterflies (Harvey's) for primes p < 2^30, where 4p < 2^32:
+// residues stay in [0, 4p) through the forward (CT) stages and in [0, 2p)
+// through the inverse (GS) ones, for 8 and 9 VALU instructions instead of the
+// canonical 12 and 11.
+// CT: x in [0, 4p) comes down to [0, 2p); v = y w mod p, lazily in [0, 2p) for
+// any y < 2^32, is formed negated (nv = -v mod 2^32), so that x + v in [0, 4p)
+// and x - v + 2p in (0, 4p) take one instruction each.
+__device__ __forceinline__ void ct_lazy(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
+    const uint32_t p2 = p + p;
+    const uint32_t nv = mulhi32(y, w.y) * p - y * w.x;
+    const uint32_t xr = csub(x, p2);
+    x = xr - nv;
+    y = xr + nv + p2;
+}
+
+// GS: x, y in [0, 2p) give x + y back in [0, 2p) and (x - y + 2p) w lazily in [0, 2p).
+__device__ __forceinline__ void gs_lazy(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
+    const uint32_t p2 = p + p;
+    const uint32_t s = x + y, d = x - y + p2;
+    x = csub(s, p2);
+    y = d * w.x - mulhi32(d, w.y) * p;
+}
+
+template <bool kFwd, bool kLazy>
 __device__ __forceinline__ void bfly(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    if (kFwd)
+    if (kFwd && kLazy)
+        ct_lazy(x, y, w, p);
+    else if (kFwd)
         ct(x, y, w, p);
+    else if (kLazy)
+        gs_lazy(x, y, w, p);
     else
         gs(x, y, w, p);
 }
@@ -115,7 +142,7 @@ __device__ __forceinline__ auto row_tw(const DevPlan& P, int j, uint32_t rho, bo
 }
 
 // Phase A on one or two register sets (kTwo: x and y share the twiddles).
-template <int LOGM, bool kFwd, bool kTwo, class TW>
+template <int LOGM, bool kFwd, bool kTwo, bool kLazy, class TW>
 __device__ __forceinline__ void phase_a(uint32_t (&x)[16], uint32_t (&y)[16], const TW& tw, uint32_t p) {
     constexpr int GA = 1 << (LOGM - 4), NG = 16 / GA;
 #pragma unroll
@@ -129,15 +156,15 @@ __device__ __forceinline__ void phase_a(uint32_t (&x)[16], uint32_t (&y)[16], co
             for (int i = 0; i < NG; ++i) {
 #pragma unroll
                 for (int k = blk * 2 * d; k < blk * 2 * d + d; ++k) {
-                    bfly<kFwd>(x[i * GA + k], x[i * GA + k + d], w, p);
-                    if (kTwo) bfly<kFwd>(y[i * GA + k], y[i * GA + k + d], w, p);
+                    bfly<kFwd, kLazy>(x[i * GA + k], x[i * GA + k + d], w, p);
+                    if (kTwo) bfly<kFwd, kLazy>(y[i * GA + k], y[i * GA + k + d], w, p);
                 }
             }
         }
     }
 }
 
-template <bool kFwd, bool kTwo, class TW>
+template <bool kFwd, bool kTwo, bool kLazy, class TW>
 __device__ __forceinline__ void phase_b(uint32_t (&x)[16], uint32_t (&y)[16], const TW& tw, uint32_t t,
                                         uint32_t p) {
 #pragma unroll
@@ -149,8 +176,8 @@ __device__ __forceinline__ void phase_b(uint32_t (&x)[16], uint32_t (&y)[16], co
             const uint2 w = tw.b(e, t, blk);
 #pragma unroll
             for (int k = blk * 2 * d; k < blk * 2 * d + d; ++k) {
-                bfly<kFwd>(x[k], x[k + d], w, p);
-                if (kTwo) bfly<kFwd>(y[k], y[k + d], w, p);
+                bfly<kFwd, kLazy>(x[k], x[k + d], w, p);
+                if (kTwo) bfly<kFwd, kLazy>(y[k], y[k + d], w, p);
             }
         }
     }
@@ -209,7 +236,9 @@ struct BufTw {
 // slot padded(u) of column c is at padded(u) C + c; with t < TPS and
 // TPS NG = 16, padded(t + TPS i + 16 k) = t + TPS i + 17 k and
 // padded(16 t + k) = 17 t + k, so every slot is a lane base plus a constant.
-template <int LOGM, bool kFwd, int LOGC>
+// kLazy (p < 2^30): the forward pass leaves its output in [0, 4p), the inverse
+// pass takes input in [0, 2p) and leaves it canonical.
+template <int LOGM, bool kFwd, int LOGC, bool kLazy>
 __device__ __forceinline__ void col_pass(uint32_t* __restrict__ X, const DevPlan& P) {
     extern __shared__ uint32_t lds[];
     constexpr uint32_t M = 1u << LOGM, TPS = M / 16, GA = TPS, NG = 16 / GA, C = 1u << LOGC;
@@ -230,7 +259,7 @@ __device__ __forceinline__ void col_pass(uint32_t* __restrict__ X, const DevPlan
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
             for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = x.ld(va, 4u * (TPS * i + 16 * k) * S);
-        phase_a<LOGM, true, false>(r, r, tw, p);
+        phase_a<LOGM, true, false, kLazy>(r, r, tw, p);
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -238,13 +267,13 @@ __device__ __forceinline__ void col_pass(uint32_t* __restrict__ X, const DevPlan
         __syncthreads();
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) r[k] = lb[k * C];
-        phase_b<true, false>(r, r, tw, t, p);
+        phase_b<true, false, kLazy>(r, r, tw, t, p);
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) x.st(r[k], vb, 4u * k * S);
     } else {
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) r[k] = x.ld(vb, 4u * k * S);
-        phase_b<false, false>(r, r, tw, t, p);
+        phase_b<false, false, kLazy>(r, r, tw, t, p);
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) lb[k * C] = r[k];
         __syncthreads();
@@ -252,7 +281,11 @@ __device__ __forceinline__ void col_pass(uint32_t* __restrict__ X, const DevPlan
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
             for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = la[(TPS * i + 17 * k) * C];
-        phase_a<LOGM, false, false>(r, r, tw, p);
+        phase_a<LOGM, false, false, kLazy>(r, r, tw, p);
+        if (kLazy) {
+#pragma unroll
+            for (uint32_t k = 0; k < 16; ++k) r[k] = csub(r[k], p);
+        }
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -260,22 +293,23 @@ __device__ __forceinline__ void col_pass(uint32_t* __restrict__ X, const DevPlan
     }
 }
 
-template <int LOGM, bool kFwd, int LOGC>
+template <int LOGM, bool kFwd, int LOGC, bool kLazy>
 __global__ void __launch_bounds__(512) k_ntt_col_reg(uint32_t* __restrict__ X, DevPlan P) {
-    col_pass<LOGM, kFwd, LOGC>(X, P);
+    col_pass<LOGM, kFwd, LOGC, kLazy>(X, P);
 }
 
 // A batch: forward, blockIdx.z = 2 (product) + (0: A, 1: B); inverse, blockIdx.z = product (A).
-template <int LOGM, bool kFwd, int LOGC>
+template <int LOGM, bool kFwd, int LOGC, bool kLazy>
 __global__ void __launch_bounds__(512) k_ntt_col_batch(DevPlan P, BatchArgs ba) {
     const int i = kFwd ? blockIdx.z >> 1 : blockIdx.z;
-    col_pass<LOGM, kFwd, LOGC>(kFwd && (blockIdx.z & 1) ? ba.B[i] : ba.A[i], P);
+    col_pass<LOGM, kFwd, LOGC, kLazy>(kFwd && (blockIdx.z & 1) ? ba.B[i] : ba.A[i], P);
 }
 
 // Row pass alone (forward or inverse): G = 2^logG rows of M = S elements.
 // Twiddles are generated per row (kGen) or read from the full table; phase
-// A's are staged once per row into LDS by the row's first threads.
-template <int LOGM, bool kFwd, bool kGen>
+// A's are staged once per row into LDS by the row's first threads. The output
+// is canonical; kLazy (p < 2^30) accepts forward input in [0, 4p).
+template <int LOGM, bool kFwd, bool kGen, bool kLazy>
 __global__ void __launch_bounds__(512)
 k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
     extern __shared__ uint32_t lds[];
@@ -298,7 +332,7 @@ k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
 #pragma unroll
             for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = x[t + TPS * i + 16 * k];
         __syncthreads();
-        phase_a<LOGM, true, false>(r, r, ta, pc.p);
+        phase_a<LOGM, true, false, kLazy>(r, r, ta, pc.p);
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -306,14 +340,18 @@ k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
         __syncthreads();
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) r[k] = l[padded(16 * t + k)];
-        phase_b<true, false>(r, r, gen, t, pc.p);
+        phase_b<true, false, kLazy>(r, r, gen, t, pc.p);
+        if (kLazy) {
+#pragma unroll
+            for (uint32_t k = 0; k < 16; ++k) r[k] = csub(csub(r[k], 2 * pc.p), pc.p);
+        }
         // phase B layout: 16 consecutive words per thread
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) x[16 * t + k] = r[k];
     } else {
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) r[k] = x[16 * t + k];
-        phase_b<false, false>(r, r, gen, t, pc.p);
+        phase_b<false, false, kLazy>(r, r, gen, t, pc.p);
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) l[padded(16 * t + k)] = r[k];
         __syncthreads();
@@ -321,7 +359,11 @@ k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
             for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = l[padded(t + TPS * i + 16 * k)];
-        phase_a<LOGM, false, false>(r, r, ta, pc.p);
+        phase_a<LOGM, false, false, kLazy>(r, r, ta, pc.p);
+        if (kLazy) {
+#pragma unroll
+            for (uint32_t k = 0; k < 16; ++k) r[k] = csub(r[k], pc.p);
+        }
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -337,8 +379,11 @@ __device__ __forceinline__ uint32_t pointwise_mul(uint32_t a, uint32_t b, const 
 }
 
 // Row pass fused: forward rows of A and B (sharing twiddle loads), the pointwise
-// product with N^-1 (P/p_j)^-1 into A, inverse rows of A.
-template <int LOGM, bool kGen>
+// product with N^-1 (P/p_j)^-1 into A, inverse rows of A. kLazy (p < 2^30):
+// input in [0, 4p); both factors of the pointwise product are brought to
+// [0, 2p), which keeps a b below 2^32 p and the Montgomery product in [0, 2p),
+// the inverse stages' input range; the output is left in [0, 2p).
+template <int LOGM, bool kGen, bool kLazy>
 __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, const DevPlan& P,
                                            uint32_t logG) {
     extern __shared__ uint32_t lds[];
@@ -366,7 +411,7 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
             b[i * GA + k] = Bm[row + t + TPS * i + 16 * k];
         }
     wave_sync();
-    phase_a<LOGM, true, true>(a, b, fa, pc.p);
+    phase_a<LOGM, true, true, kLazy>(a, b, fa, pc.p);
 #pragma unroll
     for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -382,10 +427,16 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
     wave_sync();
 #pragma unroll
     for (uint32_t k = 0; k < 16; ++k) b[k] = la[padded(16 * t + k)];
-    phase_b<true, true>(a, b, fw, t, pc.p);
+    phase_b<true, true, kLazy>(a, b, fw, t, pc.p);
+    if (kLazy) {
 #pragma unroll
-    for (uint32_t k = 0; k < 16; ++k) a[k] = pointwise_mul(a[k], b[k], pc);
-    phase_b<false, false>(a, a, iw, t, pc.p);
+        for (uint32_t k = 0; k < 16; ++k)
+            a[k] = mont_mul_lazy(csub(a[k], 2 * pc.p), csub(b[k], 2 * pc.p), pc.p, pc.pneg);
+    } else {
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) a[k] = pointwise_mul(a[k], b[k], pc);
+    }
+    phase_b<false, false, kLazy>(a, a, iw, t, pc.p);
     // Each thread rewrites only the LDS words it read above (b's, phase B layout).
 #pragma unroll
     for (uint32_t k = 0; k < 16; ++k) la[padded(16 * t + k)] = a[k];
@@ -394,23 +445,23 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
     for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
         for (uint32_t k = 0; k < GA; ++k) a[i * GA + k] = la[padded(t + TPS * i + 16 * k)];
-    phase_a<LOGM, false, false>(a, a, ia, pc.p);
+    phase_a<LOGM, false, false, kLazy>(a, a, ia, pc.p);
 #pragma unroll
     for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
         for (uint32_t k = 0; k < GA; ++k) A[row + t + TPS * i + 16 * k] = a[i * GA + k];
 }
 
-template <int LOGM, bool kGen>
+template <int LOGM, bool kGen, bool kLazy>
 __global__ void __launch_bounds__(512)
 k_ntt_rows_fused_reg(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, DevPlan P, uint32_t logG) {
-    rows_fused<LOGM, kGen>(A, Bm, P, logG);
+    rows_fused<LOGM, kGen, kLazy>(A, Bm, P, logG);
 }
 
 // A batch: blockIdx.z = product.
-template <int LOGM, bool kGen>
+template <int LOGM, bool kGen, bool kLazy>
 __global__ void __launch_bounds__(512) k_ntt_rows_fused_batch(DevPlan P, BatchArgs ba, uint32_t logG) {
-    rows_fused<LOGM, kGen>(ba.A[blockIdx.z], ba.B[blockIdx.z], P, logG);
+    rows_fused<LOGM, kGen, kLazy>(ba.A[blockIdx.z], ba.B[blockIdx.z], P, logG);
 }
 
 bool reg_ok(uint32_t logm) { return logm >= 4 && logm <= 8; }
@@ -427,22 +478,31 @@ uint32_t row_logG(const DevPlan& P) {
     return want < P.logR ? want : P.logR;
 }
 
+template <int LOGM, int LOGC, bool kLazy>
+void col_launch3(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st, dim3 grid, dim3 block, size_t lds,
+                 const BatchArgs* ba) {
+    if (ba) {
+        if (forward)
+            launch((k_ntt_col_batch<LOGM, true, LOGC, kLazy>), grid, block, lds, st, P, *ba);
+        else
+            launch((k_ntt_col_batch<LOGM, false, LOGC, kLazy>), grid, block, lds, st, P, *ba);
+    } else if (forward) {
+        launch((k_ntt_col_reg<LOGM, true, LOGC, kLazy>), grid, block, lds, st, X, P);
+    } else {
+        launch((k_ntt_col_reg<LOGM, false, LOGC, kLazy>), grid, block, lds, st, X, P);
+    }
+}
+
 template <int LOGM, int LOGC>
 void col_launch2(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st, const BatchArgs* ba = nullptr,
                  int count = 0) {
     const dim3 grid(1u << (P.logS - LOGC), P.K, ba ? (forward ? 2 * count : count) : 1);
     const dim3 block(1u << (LOGC + LOGM - 4));
     const size_t lds = sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << LOGC;
-    if (ba) {
-        if (forward)
-            launch((k_ntt_col_batch<LOGM, true, LOGC>), grid, block, lds, st, P, *ba);
-        else
-            launch((k_ntt_col_batch<LOGM, false, LOGC>), grid, block, lds, st, P, *ba);
-    } else if (forward) {
-        launch((k_ntt_col_reg<LOGM, true, LOGC>), grid, block, lds, st, X, P);
-    } else {
-        launch((k_ntt_col_reg<LOGM, false, LOGC>), grid, block, lds, st, X, P);
-    }
+    if (P.lazy)
+        col_launch3<LOGM, LOGC, true>(P, X, forward, st, grid, block, lds, ba);
+    else
+        col_launch3<LOGM, LOGC, false>(P, X, forward, st, grid, block, lds, ba);
 }
 
 // (R, C) shape of the column pass as one switch key, or -1 if only the generic kernels apply.
@@ -473,6 +533,19 @@ void col_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     }
 }
 
+template <int LOGM, bool kLazy>
+void row_launch3(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st, dim3 grid, dim3 block, size_t lds,
+                 uint32_t logG) {
+    if (forward && P.row_gen)
+        launch((k_ntt_row_reg<LOGM, true, true, kLazy>), grid, block, lds, st, X, P, logG);
+    else if (forward)
+        launch((k_ntt_row_reg<LOGM, true, false, kLazy>), grid, block, lds, st, X, P, logG);
+    else if (P.row_gen)
+        launch((k_ntt_row_reg<LOGM, false, true, kLazy>), grid, block, lds, st, X, P, logG);
+    else
+        launch((k_ntt_row_reg<LOGM, false, false, kLazy>), grid, block, lds, st, X, P, logG);
+}
+
 template <int LOGM>
 void row_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     const uint32_t logG = row_logG(P);
@@ -480,14 +553,10 @@ void row_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     const dim3 block(1u << (logG + LOGM - 4));
     const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (sizeof(uint2) << (LOGM - 4 + logG));
-    if (forward && P.row_gen)
-        launch((k_ntt_row_reg<LOGM, true, true>), grid, block, lds, st, X, P, logG);
-    else if (forward)
-        launch((k_ntt_row_reg<LOGM, true, false>), grid, block, lds, st, X, P, logG);
-    else if (P.row_gen)
-        launch((k_ntt_row_reg<LOGM, false, true>), grid, block, lds, st, X, P, logG);
+    if (P.lazy)
+        row_launch3<LOGM, true>(P, X, forward, st, grid, block, lds, logG);
     else
-        launch((k_ntt_row_reg<LOGM, false, false>), grid, block, lds, st, X, P, logG);
+        row_launch3<LOGM, false>(P, X, forward, st, grid, block, lds, logG);
 }
 
 template <int LOGM>
@@ -497,10 +566,14 @@ void fused_launch_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStr
     const dim3 block(1u << (logG + LOGM - 4));
     const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (2 * sizeof(uint2) << (LOGM - 4 + logG));
-    if (P.row_gen)
-        launch((k_ntt_rows_fused_batch<LOGM, true>), grid, block, lds, st, P, ba, logG);
+    if (P.row_gen && P.lazy)
+        launch((k_ntt_rows_fused_batch<LOGM, true, true>), grid, block, lds, st, P, ba, logG);
+    else if (P.row_gen)
+        launch((k_ntt_rows_fused_batch<LOGM, true, false>), grid, block, lds, st, P, ba, logG);
+    else if (P.lazy)
+        launch((k_ntt_rows_fused_batch<LOGM, false, true>), grid, block, lds, st, P, ba, logG);
     else
-        launch((k_ntt_rows_fused_batch<LOGM, false>), grid, block, lds, st, P, ba, logG);
+        launch((k_ntt_rows_fused_batch<LOGM, false, false>), grid, block, lds, st, P, ba, logG);
 }
 
 template <int LOGM>
@@ -510,10 +583,14 @@ void fused_launch(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t 
     const dim3 block(1u << (logG + LOGM - 4));
     const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (2 * sizeof(uint2) << (LOGM - 4 + logG));
-    if (P.row_gen)
-        launch((k_ntt_rows_fused_reg<LOGM, true>), grid, block, lds, st, A, B, P, logG);
+    if (P.row_gen && P.lazy)
+        launch((k_ntt_rows_fused_reg<LOGM, true, true>), grid, block, lds, st, A, B, P, logG);
+    else if (P.row_gen)
+        launch((k_ntt_rows_fused_reg<LOGM, true, false>), grid, block, lds, st, A, B, P, logG);
+    else if (P.lazy)
+        launch((k_ntt_rows_fused_reg<LOGM, false, true>), grid, block, lds, st, A, B, P, logG);
     else
-        launch((k_ntt_rows_fused_reg<LOGM, false>), grid, block, lds, st, A, B, P, logG);
+        launch((k_ntt_rows_fused_reg<LOGM, false, false>), grid, block, lds, st, A, B, P, logG);
 }
 
 }  // namespace

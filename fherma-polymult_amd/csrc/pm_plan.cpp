@@ -112,7 +112,13 @@ void mul_small_mod(std::vector<uint32_t>& x, uint32_t m, const HostPlan& pl) {
 
 }  // namespace
 
-std::vector<uint32_t> choose_primes(uint32_t N, int qbits) {
+void ntt_geometry(uint32_t logN, uint32_t& logS, uint32_t& logR) {
+    logS = logN <= 7 ? logN : (logN + 1) / 2;
+    logR = logN - logS;
+}
+
+std::vector<uint32_t> choose_primes(uint32_t N, int qbits, int pbits) {
+    if (pbits != 30 && pbits != 31) throw std::runtime_error("the prime bound must be 2^30 or 2^31");
     uint32_t logN = 0;
     while ((1u << logN) < N) ++logN;
     // |c| < N q^2 and the CRT rounding needs |c| / P < 2^-(margin + 1)
@@ -121,9 +127,10 @@ std::vector<uint32_t> choose_primes(uint32_t N, int qbits) {
     double have_bits = 0;
     const uint32_t step = std::max(2 * N, 1u << 16);
     std::vector<uint32_t> primes;
-    for (uint32_t c = ((1u << 31) - 1) / step * step + 1; have_bits <= need_bits; c -= step) {
+    for (uint32_t c = ((1u << pbits) - 1) / step * step + 1; have_bits <= need_bits; c -= step) {
         if (c < step) {
-            throw std::runtime_error("not enough primes p < 2^31 with p = 1 (mod " + std::to_string(step) +
+            throw std::runtime_error("not enough primes p < 2^" + std::to_string(pbits) + " with p = 1 (mod " +
+                                     std::to_string(step) +
                                      ") for N = " + std::to_string(N) + ", q of " + std::to_string(qbits) + " bits");
         }
         if (!is_prime(c)) continue;
@@ -136,7 +143,7 @@ std::vector<uint32_t> choose_primes(uint32_t N, int qbits) {
     return primes;
 }
 
-HostPlan make_plan(uint32_t N, uint32_t W, const uint32_t* q, int L) {
+HostPlan make_plan(uint32_t N, uint32_t W, const uint32_t* q, int L, int pbits) {
     if (N < 2 || (N & (N - 1)) || N > (1u << kMaxLogN)) {
         throw std::runtime_error("N must be a power of two in [2, 2^" + std::to_string(kMaxLogN) + "]; got " +
                                  std::to_string(N));
@@ -155,13 +162,11 @@ HostPlan make_plan(uint32_t N, uint32_t W, const uint32_t* q, int L) {
     pl.q_drop = L > 3 ? L - 3 : 0;
     for (int i = L - 1; i >= pl.q_drop; --i) pl.q_top = pl.q_top * 4294967296.0 + pl.q[i];
 
-    pl.p = choose_primes(N, pl.qbits);
+    pl.p = choose_primes(N, pl.qbits, pbits);
+    pl.pbits = pbits;
     const int K = pl.K = static_cast<int>(pl.p.size());
 
-    // Two-pass geometry: rows of S = 2^ceil(logN / 2), columns of R = N / S
-    // (one pass when N <= 128).
-    pl.logS = pl.logN <= 7 ? pl.logN : (pl.logN + 1) / 2;
-    pl.logR = pl.logN - pl.logS;
+    ntt_geometry(pl.logN, pl.logS, pl.logR);
 
     pl.pneg.resize(K);
     pl.psi.resize(K);

@@ -2,9 +2,12 @@
 // kernels read. Built once per point, outside any timed region.
 //
 // The prime and K rule is the reference's (rns_plan.h:138-155): primes
-// p < 2^31 with p = 1 (mod max(2N, 2^16)), taken downward from the largest,
-// until sum log2 p exceeds 2 * bits(q) + log2 N + 1 + 20. Keeping the same rule
-// keeps the same K, so the product is a drop-in for the reference's plan.
+// p < 2^pbits with p = 1 (mod max(2N, 2^16)), taken downward from the largest,
+// until sum log2 p exceeds 2 * bits(q) + log2 N + 1 + 20. The reference's bound
+// is pbits = 31, which gives its K. pbits = 30 runs the same walk below 2^30:
+// about 3 % more primes, but a factor 4 of headroom in a 32-bit word, which the
+// NTT kernels use to keep residues lazily reduced. The product c is the same
+// for either prime set.
 #pragma once
 
 #include <stdint.h>
@@ -17,6 +20,7 @@ namespace pm {
 struct HostPlan {
     uint32_t N = 0, logN = 0, W = 0;
     int L = 0, K = 0, qbits = 0;
+    int pbits = 31;                   // every prime is below 2^pbits
     std::vector<uint32_t> q;          // L limbs
 
     // q_top: the top three limbs of q as a double (relative to 2^(32 q_drop)),
@@ -46,10 +50,14 @@ struct HostPlan {
 
 // Throws std::runtime_error with the reference's messages for invalid points
 // (rns_plan.h:120-155).
-HostPlan make_plan(uint32_t N, uint32_t W, const uint32_t* q, int L);
+HostPlan make_plan(uint32_t N, uint32_t W, const uint32_t* q, int L, int pbits = 31);
 
-// The prime walk alone: K primes for (N, bits(q)), or an exception.
-std::vector<uint32_t> choose_primes(uint32_t N, int qbits);
+// Two-pass NTT geometry: rows of S = 2^ceil(logN / 2), columns of R = N / S
+// (one pass, R = 1, when N <= 128).
+void ntt_geometry(uint32_t logN, uint32_t& logS, uint32_t& logR);
+
+// The prime walk alone: K primes for (N, bits(q)) below 2^pbits (30 or 31), or an exception.
+std::vector<uint32_t> choose_primes(uint32_t N, int qbits, int pbits = 31);
 
 // Twiddles psi^bitrev_N(i) (forward) or psi^-bitrev_N(i) (inverse), i < N,
 // as (w, floor(w 2^32 / p)) pairs, for primes [j0, j1) of the plan.

@@ -17,6 +17,10 @@ polymult = pytest.importorskip("polymult")
 pytestmark = pytest.mark.gpu
 
 FORMS = [("plain", 0), ("mfma", 1)]
+# prime bases: the default choice (primes below 2^30 on the register-kernel
+# geometries, N = 2^8 .. 2^16, with lazily reduced NTT butterflies) and the
+# reference's primes below 2^31 with canonical butterflies
+BASES = [("auto", 0), ("p31", 31)]
 
 
 def say(*args):
@@ -43,11 +47,11 @@ PATTERNS = ["pattern-%s-n%d-w%d" % (p, n, w) for p in ("all_q_minus_1", "zero", 
             for n, w in ((4, 64), (64, 100), (256, 1747), (4096, 256), (32768, 868), (65536, 2048))]
 
 
-def run_case(golden, name, form):
+def run_case(golden, name, form, basis=0):
     case = golden[name]
     q, a, b = pointgen.operands(case)
-    say("case", name, "N", case["N"], "L", case["L"], "form", form)
-    with polymult.Plan(case["N"], case["W"], q, form=form) as plan:
+    say("case", name, "N", case["N"], "L", case["L"], "form", form, "basis", basis)
+    with polymult.Plan(case["N"], case["W"], q, form=form, basis=basis) as plan:
         c = plan.multiply(a, b)
     got = oracle.digest(c)
     if got != case["c_sha"]:
@@ -56,27 +60,45 @@ def run_case(golden, name, form):
         pytest.fail("%s: %d of %d coefficients differ, first %s" % (name, bad.size, case["N"], bad[:5]))
 
 
+@pytest.mark.parametrize("basis", [b for _, b in BASES], ids=[n for n, _ in BASES])
 @pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
 @pytest.mark.parametrize("name", SMALL + CONFIGS)
-def test_product_matches_gmp(golden, name, form):
-    run_case(golden, name, form)
+def test_product_matches_gmp(golden, name, form, basis):
+    run_case(golden, name, form, basis)
 
 
+@pytest.mark.parametrize("basis", [b for _, b in BASES], ids=[n for n, _ in BASES])
 @pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
 @pytest.mark.parametrize("name", PATTERNS)
-def test_patterns_match_gmp(golden, name, form):
-    run_case(golden, name, form)
+def test_patterns_match_gmp(golden, name, form, basis):
+    run_case(golden, name, form, basis)
 
 
+@pytest.mark.parametrize("N,W,want", [(64, 100, 31), (256, 1747, 30), (4096, 256, 30), (65536, 2048, 30),
+                                      (131072, 512, 31)])
+def test_default_basis_choice(N, W, want):
+    """Primes below 2^30 where every NTT pass has a register kernel (N = 2^8 ..
+    2^16), the reference's below 2^31 elsewhere; an explicit basis is kept."""
+    q, _, _ = oracle.gen_point(11, N, W)
+    with polymult.Plan(N, W, q) as plan:
+        assert plan.basis == want
+        assert np.array_equal(plan.primes, oracle.choose_primes(N, _qbits(q), want))
+    for basis in (polymult.BASIS_P30, polymult.BASIS_P31):
+        with polymult.Plan(N, W, q, basis=basis) as plan:
+            assert plan.basis == basis and plan.K == len(oracle.choose_primes(N, _qbits(q), basis))
+
+
+@pytest.mark.parametrize("pbits", [30, 31])
 @pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
 @pytest.mark.parametrize("N,W,seed", [(64, 100, 3), (4096, 256, 5), (8192, 868, 6), (32768, 868, 7),
                                       (65536, 2048, 8)])
-def test_stages_match_oracle(N, W, seed, form):
-    """Each of the five steps, device to device, against the oracle's step."""
-    say("stages", N, W, seed, "form", form)
+def test_stages_match_oracle(N, W, seed, form, pbits):
+    """Each of the five steps, device to device, against the oracle's step, in
+    both prime bases (the stage entry points leave canonical residues)."""
+    say("stages", N, W, seed, "form", form, "pbits", pbits)
     q, a, b = oracle.gen_point(seed, N, W)
-    ref = oracle.Plan(N, q)
-    with polymult.Plan(N, W, q, form=form) as plan:
+    ref = oracle.Plan(N, q, pbits)
+    with polymult.Plan(N, W, q, form=form, basis=pbits) as plan:
         assert plan.K == ref.K and np.array_equal(plan.primes, ref.primes)
         K, L = plan.K, plan.L
         da, db = polymult.DeviceBuffer.from_array(a), polymult.DeviceBuffer.from_array(b)
@@ -168,8 +190,9 @@ def _qbits(q):
     return int(oracle.int_of(q)).bit_length()
 
 
+@pytest.mark.parametrize("basis", [b for _, b in BASES], ids=[n for n, _ in BASES])
 @pytest.mark.parametrize("N,W", [(4096, 256), (32768, 868), (65536, 2048)])
-def test_phase_batched_graph_matches_gmp(golden, N, W):
+def test_phase_batched_graph_matches_gmp(golden, N, W, basis):
     """Products that share N, L and the primes are captured as five phase-batched
     launches; two replays, eager products in between, every answer vs GMP."""
     cases = sorted((c for c in golden.values() if c["N"] == N and c["W"] == W), key=lambda c: c["name"])
@@ -178,7 +201,7 @@ def test_phase_batched_graph_matches_gmp(golden, N, W):
         q, a, b = pointgen.operands(case)
         if _qbits(q) != W:   # other q sizes have other primes: not one batch
             continue
-        plans.append(polymult.Plan(N, W, q, form=polymult.FORM_MFMA))
+        plans.append(polymult.Plan(N, W, q, form=polymult.FORM_MFMA, basis=basis))
         bufs.append((polymult.DeviceBuffer.from_array(a), polymult.DeviceBuffer.from_array(b),
                      polymult.DeviceBuffer(a.nbytes)))
         keep.append(case)
@@ -243,10 +266,10 @@ def test_row_twiddle_sources_match_gmp(golden, name):
     give the GMP product (the reference's fuzz runs tw-gen and tw-table)."""
     case = golden[name]
     q, a, b = pointgen.operands(case)
-    for form in (polymult.FORM_PLAIN, polymult.FORM_MFMA):
-        with polymult.Plan(case["N"], case["W"], q, form=form) as plan:
+    for form, basis in ((polymult.FORM_PLAIN, 0), (polymult.FORM_MFMA, 0), (polymult.FORM_MFMA, 31)):
+        with polymult.Plan(case["N"], case["W"], q, form=form, basis=basis) as plan:
             for source in (polymult.TWIDDLES_TABLE, polymult.TWIDDLES_GEN):
                 plan.set_twiddles(source)
                 assert plan.twiddles == source
-                say("twiddles", name, "form", form, "source", source)
+                say("twiddles", name, "form", form, "basis", plan.basis, "source", source)
                 assert oracle.digest(plan.multiply(a, b)) == case["c_sha"], (name, form, source)

@@ -45,10 +45,24 @@ def test_fherma_cpp_entry_points_are_exported():
         assert mangled in syms, mangled
 
 
+@pytest.mark.parametrize("basis", [31, 30])
 @pytest.mark.parametrize("N,W", [(2, 3), (64, 2048), (4096, 256), (32768, 868), (65536, 1747), (65536, 2048),
                                  (1 << 17, 512), (1 << 22, 32)])
-def test_host_prime_rule_matches_oracle(N, W):
-    assert np.array_equal(polymult.choose_primes(N, W), oracle.choose_primes(N, W))
+def test_host_prime_rule_matches_oracle(N, W, basis):
+    assert np.array_equal(polymult.choose_primes(N, W, basis), oracle.choose_primes(N, W, basis))
+
+
+def test_prime_basis_arguments():
+    # the reference's basis is the default of pm_choose_primes
+    assert np.array_equal(polymult.choose_primes(4096, 256), polymult.choose_primes(4096, 256, polymult.BASIS_P31))
+    # N = 2^22, 128-bit q: enough primes below 2^31, too few below 2^30
+    assert len(polymult.choose_primes(1 << 22, 128)) > 0
+    with pytest.raises(polymult.PolymulError) as e:
+        polymult.choose_primes(1 << 22, 128, polymult.BASIS_P30)
+    assert e.value.code == polymult.PM_EINVAL and "2^30" in str(e.value)
+    with pytest.raises(polymult.PolymulError) as e:
+        polymult.choose_primes(4096, 256, 29)
+    assert e.value.code == polymult.PM_EINVAL
 
 
 def test_invalid_points_are_refused_before_any_device_work():
@@ -88,6 +102,19 @@ def test_driver_refuses_out_of_range_points(tmp_path):
         r = subprocess.run([solution, str(d)], capture_output=True, text=True, timeout=60)
         assert r.returncode != 0
         assert not (d / "out" / "000000").exists()
+
+
+def test_invalid_basis_is_refused_before_device_work():
+    """PM_BASIS other than p30, p31 or auto fails plan creation with PM_EINVAL."""
+    code = ("import sys; sys.path.insert(0, %r); import numpy as np, polymult\n"
+            "try:\n"
+            "    polymult.Plan(64, 64, np.array([7, 1], dtype=np.uint32))\n"
+            "except polymult.PolymulError as e:\n"
+            "    print(e.code, str(e))\n" % os.path.join(ROOT, "fherma-polymult_amd"))
+    env = dict(os.environ, PM_BASIS="p29")
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert out.stdout.startswith("%d " % polymult.PM_EINVAL), out.stdout + out.stderr
+    assert "PM_BASIS" in out.stdout
 
 
 def test_invalid_twiddle_policy_is_refused_before_device_work():

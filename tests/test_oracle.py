@@ -116,12 +116,19 @@ def test_closed_forms_against_brute_force():
             assert oracle.py_negacyclic(pa, pb, q) == pc, (N, name)
 
 
-def test_prime_rule():
-    for N, W, K in ((4096, 256, 18), (32768, 868, 58), (65536, 1747, 115), (65536, 2048, 134)):
-        p = oracle.choose_primes(N, W)
+@pytest.mark.parametrize("pbits,Ks", [(31, (18, 58, 115, 134)), (30, (19, 60, 119, 139))])
+def test_prime_rule(pbits, Ks):
+    """The reference's walk (pbits 31) and the same walk below 2^30: every
+    candidate p = 1 (mod step) below the bound is tested downward, none skipped,
+    until the bits suffice."""
+    for (N, W), K in zip(((4096, 256), (32768, 868), (65536, 1747), (65536, 2048)), Ks):
+        p = oracle.choose_primes(N, W, pbits)
         assert len(p) == K
         step = max(2 * N, 1 << 16)
-        assert all(sympy.isprime(int(x)) and int(x) % step == 1 and x < 2 ** 31 for x in p)
+        assert all(sympy.isprime(int(x)) and int(x) % step == 1 and x < 2 ** pbits for x in p)
+        # no prime of that form lies above the first chosen one
+        for c in range(int(p[0]) + step, 2 ** pbits, step):
+            assert not sympy.isprime(c)
         assert all(p[i] > p[i + 1] for i in range(len(p) - 1))
         # every prime of that form between consecutive chosen ones is composite
         for hi, lo in zip(p[:-1], p[1:]):
