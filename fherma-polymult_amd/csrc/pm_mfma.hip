@@ -252,30 +252,32 @@ constexpr int kTabLoads = 11;           // 16-byte loads per thread for the tabl
 constexpr int kSegMax = 17;             // limbs per segment: 4 x 17 >= L + 2 for L <= 64
 
 // Digit table words, column-major: word (J, R) at J kTabRows + R holds byte R - 3
-// of the balanced digits of M_4J .. M_4J+3 (zero outside 0 .. 4L); whole 16-byte units.
-__host__ __device__ constexpr uint32_t crt_tab_words(int K) {
-    return (static_cast<uint32_t>((K + 3) / 4) * kTabRows + 3u) & ~3u;
+// of the balanced digits of M_4J .. M_4J+3 (zero outside 0 .. 4L and for primes
+// past K), for the 4 KS prime groups that KS K-steps read; whole 16-byte units.
+__host__ __device__ constexpr uint32_t crt_tab_words(int KS) {
+    return 4u * static_cast<uint32_t>(KS) * kTabRows;
 }
 
 // LDS words, three regions:
 //   R0  y digits [4 KS][64] as 16-byte words (phases 1-2), then the limb
-//       partials (int64, [4 ST][64]), then the output stage (64 x (L + 1));
+//       partials (int64, [68][64], rows past 4 ST zero), then the output stage
+//       (64 x kStage);
 //   R1  the digit table, loaded once and kept for every tile of the workgroup;
 //   aux FP64 partial sums (1024 words), reused in phase 3 for the segment
 //       exports and comparisons, and (v, Ql, Qh) per coefficient (192 words).
-// A-fragment reads for prime groups past the table (J < 4 KS) stay inside
-// R1 + aux: 3 kTabRows < 1216.
-__host__ __device__ constexpr uint32_t crt_r0_words(int KS, int ST) {
-    return static_cast<uint32_t>(1024 * KS) > static_cast<uint32_t>(512 * ST) ? static_cast<uint32_t>(1024 * KS)
-                                                                                : static_cast<uint32_t>(512 * ST);
+// Every bound the tile loop tests is a compile-time one (KS) or a fixed size:
+// uniform runtime bounds became lane masks that the compiler hoisted out of
+// the loop and spilled.
+constexpr int kPartRows = 4 * kSegMax;   // 68 limbs: partial rows and phase 3's segments
+constexpr int kStage = kPartRows + 1;    // output stage row stride (words)
+__host__ __device__ constexpr uint32_t crt_r0_words(int KS) {
+    return static_cast<uint32_t>(1024 * KS) > static_cast<uint32_t>(2 * 64 * kPartRows)
+               ? static_cast<uint32_t>(1024 * KS)
+               : static_cast<uint32_t>(2 * 64 * kPartRows);
 }
 
-__host__ __device__ constexpr uint32_t crt_r1_words(int K) {
-    return crt_tab_words(K);
-}
-
-__host__ __device__ constexpr uint32_t crt_lds_words(int KS, int ST, int K) {
-    return crt_r0_words(KS, ST) + crt_r1_words(K) + 1024 + 192;
+__host__ __device__ constexpr uint32_t crt_lds_words(int KS) {
+    return crt_r0_words(KS) + crt_tab_words(KS) + 1024 + 192;
 }
 
 // d[a] = byte a of w0, w1, w2, w3 (in that byte order), by byte permutes:
@@ -315,15 +317,27 @@ __device__ __forceinline__ int seg_carry(int C, int cout, int lo_s, bool uz, boo
     return cout + adj;
 }
 
+// A pointer into the constant address space (scalar loads for uniform
+// addresses) whose value the compiler cannot see through: loads from it stay
+// where they are written instead of being hoisted out of a loop.
+template <class T>
+using kconst = const T __attribute__((address_space(4)))*;
+template <class T>
+__device__ __forceinline__ kconst<T> laundered(const T* p) {
+    kconst<T> k = (kconst<T>)p;
+    asm volatile("" : "+s"(k));
+    return k;
+}
+
+template <int KS>
 __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, const DevPlan& P,
                                           uint32_t k0, uint32_t count, uint32_t tile0, uint32_t tstep) {
     extern __shared__ uint32_t smem[];
-    const int K = P.K, L = P.L, KS = P.crt_ksteps, ST = P.crt_stiles;
-    const int KJ = (K + 3) >> 2;
-    const uint32_t r0 = crt_r0_words(KS, ST), r1 = crt_r1_words(K);
+    const int K = P.K, L = P.L;
+    constexpr uint32_t r0 = crt_r0_words(KS), r1 = crt_tab_words(KS);
     uint4* yq = reinterpret_cast<uint4*>(smem);                                // [4 KS][64]
     long long* part = reinterpret_cast<long long*>(smem);                      // after phase 2: [limb][64]
-    uint32_t* stage = smem;                                                    // after phase 3: 64 x (L + 1)
+    uint32_t* stage = smem;                                                    // after phase 3: 64 x kStage
     uint32_t* tab = smem + r0;                                                 // [KJ][kTabRows]
     double* fr = reinterpret_cast<double*>(tab + r1);                          // 4 x 64
     double* fq = fr + 4 * 64;                                                  // 4 x 64
@@ -340,7 +354,7 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
     // 1. the digit table, once per workgroup, and the first tile's y words
     {
         const uint4* tab4 = reinterpret_cast<const uint4*>(P.crt_tab);
-        const uint32_t tab4n = crt_tab_words(K) / 4;
+        constexpr uint32_t tab4n = crt_tab_words(KS) / 4;
         uint4 tv[kTabLoads];
 #pragma unroll
         for (int r = 0; r < kTabLoads; ++r) tv[r] = tab4[min(threadIdx.x + r * blockDim.x, tab4n - 1)];
@@ -349,16 +363,26 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
 #pragma unroll
         for (int r = 0; r < kTabLoads; ++r) tab4s[min(threadIdx.x + r * blockDim.x, tab4n - 1)] = tv[r];
     }
-    // prime groups J = wave + 4 i, i < KS: primes 4 J + r; all loads of a tile in flight at once
-    constexpr int kRows = 4 * kCrtMaxKSteps;
+    // prime groups J = wave + 4 i, i < KS: primes 4 J + r; all loads of a tile in
+    // flight at once, through a buffer over the K x N residues: row j at the
+    // scalar offset 4 j N, the lane's column at 4 col; rows past K read 0
+    // (4 K N < 2^31: the launchers check it).
+    constexpr int kRows = 4 * KS;
     uint32_t y[kRows];
+    const __amdgpu_buffer_rsrc_t ybuf = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t*>(Y), 0, static_cast<int>(4u * static_cast<uint32_t>(K) * P.N), 0x00020000);
     auto load_y = [&](uint32_t tile) {
         const uint32_t nb = k0 + tile * 64, nk = min(64u, k0 + count - nb);
-        const uint32_t col = nb + min(static_cast<uint32_t>(lane), nk - 1);
+        const uint32_t voff = 4u * (nb + min(static_cast<uint32_t>(lane), nk - 1));
+        // the row stride is reloaded per tile: loop-invariant row offsets hoisted
+        // out of the tile loop would not fit in scalar registers
+        uint32_t stride = 4u * P.N;
+        asm volatile("" : "+s"(stride));
+        uint32_t soff = 4u * static_cast<uint32_t>(wave) * stride;
 #pragma unroll
         for (int i = 0; i < kRows; ++i) {
-            const int j = 4 * (wave + 4 * (i / 4)) + (i % 4);
-            y[i] = Y[static_cast<size_t>(min(j, K - 1)) * P.N + col];
+            y[i] = __builtin_amdgcn_raw_buffer_load_b32(ybuf, voff, soff, 0);
+            soff += (i % 4 == 3) ? 13u * stride : stride;   // next prime, or the next group J + 4
         }
     };
     if (tile0 < ntiles) load_y(tile0);
@@ -369,26 +393,28 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
     const uint32_t nk = min(64u, k0 + count - nb);
     {
         const bool lane_in = static_cast<uint32_t>(lane) < nk;
+        // loaded per tile (scalar loads), not hoisted out of the tile loop:
+        // 2 x 4 KS doubles would not fit in scalar registers
+        const kconst<double> fc = laundered(P.crt_fc);
         double f = 0, f2 = 0;
 #pragma unroll
-        for (int i = 0; i < kCrtMaxKSteps; ++i) {
-            if (i >= KS) continue;
+        for (int i = 0; i < KS; ++i) {
             const int J = wave + 4 * i;
             uint32_t w[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int j = 4 * J + r;   // < 160: the padded constants are zero past K
-                w[r] = (j < K && lane_in) ? y[4 * i + r] : 0u;
-                f = fma(static_cast<double>(w[r]), P.crt_fc[j], f);
-                f2 = fma(static_cast<double>(w[r]), P.crt_fc[4 * 4 * kCrtMaxKSteps + j], f2);
+                // y rows past K read 0; the constants (160 entries) are zero past K
+                const int j = 4 * J + r;
+                w[r] = lane_in ? y[4 * i + r] : 0u;
+                f = fma(static_cast<double>(w[r]), fc[j], f);
+                f2 = fma(static_cast<double>(w[r]), fc[4 * 4 * kCrtMaxKSteps + j], f2);
             }
             uint32_t d[4];
             transpose_bytes(w[0], w[1], w[2], w[3], d);
-            // B word k is byte 3 - k (the A word k is digit row s - 3 + k); groups
-            // past the last prime hold zeros, so any A word times them is 0
-            const uint32_t keep = J < KJ ? 0xffffffffu : 0u, bias = keep & 0x80808080u;
-            yq[J * 64 + lane] = make_uint4((d[3] & keep) ^ bias, (d[2] & keep) ^ bias, (d[1] & keep) ^ bias,
-                                           (d[0] & keep) ^ bias);
+            // B word k is byte 3 - k (the A word k is digit row s - 3 + k); the
+            // table is zero for primes past K, so their digits add nothing
+            yq[J * 64 + lane] = make_uint4(d[3] ^ 0x80808080u, d[2] ^ 0x80808080u, d[1] ^ 0x80808080u,
+                                           d[0] ^ 0x80808080u);
         }
         fr[wave * 64 + lane] = f;
         fq[wave * 64 + lane] = f2;
@@ -407,8 +433,8 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
     if (tile + tstep < ntiles) load_y(tile + tstep);
 
     // 2. limb partials: tiles st = wave + 4 i (i < 4) for all four column tiles,
-    // and the 17th limb tile (st = 16) for column tile t = wave. Tiles past ST
-    // read zero table rows.
+    // and the 17th limb tile (st = 16) for column tile t = wave. Tiles past
+    // ceil((4 L + 4) / 16) read zero table rows, so their partials are 0.
     const int g = lane >> 4, cl = lane & 15;
     long long p0[4][kTiles], p1;
     {
@@ -423,8 +449,7 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
         const uint4* yx = yq + g * 64 + 16 * wave + cl;
         // unrolled: the accumulators keep fixed registers, the reads immediate offsets
 #pragma unroll
-        for (int ks = 0; ks < kCrtMaxKSteps; ++ks) {
-            if (ks >= KS) continue;
+        for (int ks = 0; ks < KS; ++ks) {
             const uint32_t* tk = ta + 4 * ks * kTabRows;
             const uint32_t* tk4 = ta4 + 4 * ks * kTabRows;
             const uint4* yk = yb + 4 * ks * 64;
@@ -461,23 +486,22 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int st = wave + 4 * i;
-        if (st < ST) {
 #pragma unroll
-            for (int t = 0; t < kTiles; ++t) part[(4 * st + g) * 64 + 16 * t + cl] = p0[i][t];
-        }
+        for (int t = 0; t < kTiles; ++t) part[(4 * st + g) * 64 + 16 * t + cl] = p0[i][t];
     }
-    if (ST > 16) part[(64 + g) * 64 + 16 * wave + cl] = p1;
+    part[(64 + g) * 64 + 16 * wave + cl] = p1;
     __syncthreads();
 
     // 3. one segment of 17 limbs per wave, one coefficient per lane; limbs past
     // L + 2 are zero and only carry R's sign along
-    const int NL = 4 * ST;   // limbs with partials
     const int l0 = kSegMax * wave;
     const uint32_t v = vq[lane], Ql = vq[64 + lane], Qh = vq[128 + lane];
-    // limb l of q, -P mod q and the offset correction at l + 2 (l >= -2), zero outside
-    const uint32_t* lq = P.crt_lc + 2;
-    const uint32_t* lpn = P.crt_lc + 72 + 2;
-    const uint32_t* lcorr = P.crt_lc + 144 + 2;
+    // limb l of q, -P mod q and the offset correction at l + 2 (l >= -2), zero
+    // outside; loaded per tile (scalar loads), not hoisted out of the tile loop
+    const kconst<uint32_t> lc = laundered(P.crt_lc);
+    const kconst<uint32_t> lq = lc + 2;
+    const kconst<uint32_t> lpn = lc + 72 + 2;
+    const kconst<uint32_t> lcorr = lc + 144 + 2;
     auto qlimb = [&](int l) -> uint32_t { return lq[l]; };
     auto pnlimb = [&](int l) -> uint32_t { return lpn[l]; };
     uint32_t x[kSegMax];
@@ -490,9 +514,9 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
 #pragma unroll
         for (int u = 0; u < kSegMax; ++u) {
             const int l = l0 + u;
-            const long long pv = part[min(l, NL - 1) * 64 + lane];
+            const long long pv = part[l * 64 + lane];
             const uint64_t vp = static_cast<uint64_t>(v) * pnlimb(l);
-            long long s = carry + (l < NL ? pv : 0) + static_cast<long long>(lcorr[l]) +
+            long long s = carry + pv + static_cast<long long>(lcorr[l]) +
                           static_cast<long long>(static_cast<uint32_t>(vp)) + static_cast<long long>(c1 >> 32) -
                           q_times_limb(Ql, Qh, qlimb(l), a1, b1, b2);
             c1 = vp;
@@ -582,27 +606,29 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
             C = t >> 32;
         }
     }
-    // 4. limbs below L to the stage (the partials are dead), then whole-line stores
+    // 4. the limbs to the stage (the partials are dead), then whole-line stores
+    // of the L limbs
 #pragma unroll
-    for (int u = 0; u < kSegMax; ++u)
-        if (l0 + u < L) stage[lane * (L + 1) + l0 + u] = x[u];
+    for (int u = 0; u < kSegMax; ++u) stage[lane * kStage + l0 + u] = x[u];
     __syncthreads();
     // one coefficient's L words per wave and step (lane = limb)
     uint32_t* dst = c + static_cast<size_t>(nb) * L;
     if (lane < L) {
 #pragma unroll 4
-        for (uint32_t r = wave; r < nk; r += kWaves) dst[r * L + lane] = stage[r * (L + 1) + lane];
+        for (uint32_t r = wave; r < nk; r += kWaves) dst[r * L + lane] = stage[r * kStage + lane];
     }
     __syncthreads();   // the stores read the stage (region 0) before the next tile's digits overwrite it
     }
 }
 
+template <int KS>
 __global__ void __launch_bounds__(64 * kWaves)
 k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, uint32_t k0, uint32_t count) {
-    crt_tiles(Y, c, P, k0, count, blockIdx.x, gridDim.x);
+    crt_tiles<KS>(Y, c, P, k0, count, blockIdx.x, gridDim.x);
 }
 
 // A batch: blockIdx.z = product, with that product's q-dependent tables.
+template <int KS>
 __global__ void __launch_bounds__(64 * kWaves) k_crt_mfma_batch(DevPlan P, BatchArgs ba) {
     const int i = blockIdx.z;
     DevPlan Q = P;
@@ -610,7 +636,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_crt_mfma_batch(DevPlan P, Batch
     Q.crt_fc = ba.crt_fc[i];
     Q.nq = ba.nq[i];
     Q.crt_lc = ba.crt_lc[i];
-    crt_tiles(ba.A[i], ba.c[i], Q, 0, P.N, blockIdx.x, gridDim.x);
+    crt_tiles<KS>(ba.A[i], ba.c[i], Q, 0, P.N, blockIdx.x, gridDim.x);
 }
 
 // Balanced base-256 digits of a signed value |x| < 2^31.
@@ -676,8 +702,8 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
     for (int j = 0; j < K; ++j) balanced_digits(h.M.data() + static_cast<size_t>(j) * L, L, m.data() + static_cast<size_t>(j) * nd, nd);
     if (ST > 17) throw std::logic_error("CRT limb tiles exceed the kernel's 17");
     const int KJ = (K + 3) / 4;
-    if (crt_tab_words(K) > 4u * kTabLoads * 64 * kWaves) throw std::logic_error("CRT digit table too large");
-    std::vector<uint32_t> tab(crt_tab_words(K), 0);
+    if (crt_tab_words(KS) > 4u * kTabLoads * 64 * kWaves) throw std::logic_error("CRT digit table too large");
+    std::vector<uint32_t> tab(crt_tab_words(KS), 0);
     for (int J = 0; J < KJ; ++J)
         for (int R = 3; R < kTabRows && R - 3 < nd; ++R)
             for (int r = 0; r < 4 && 4 * J + r < K; ++r)
@@ -762,13 +788,52 @@ uint32_t crt_slots(size_t lds) {
     return static_cast<uint32_t>(per_cu * static_cast<size_t>(cus));
 }
 
+namespace {
+
+// The residues are read through a buffer whose byte range must fit in 31 bits.
+void check_crt_range(const DevPlan& P) {
+    if (!P.crt_tab) throw std::runtime_error("matrix-core tables not prepared");
+    if (4ull * static_cast<uint64_t>(P.K) * P.N >= (1ull << 31)) throw std::runtime_error("K x N too large for the CRT");
+    if (P.crt_ksteps < 1 || P.crt_ksteps > kCrtMaxKSteps) throw std::runtime_error("K out of range for the CRT");
+}
+
+template <int KS>
+void crt_launch(const DevPlan& P, const uint32_t* Y, uint32_t* c, uint32_t k0, uint32_t count, hipStream_t st,
+                const BatchArgs* ba, int nprod) {
+    const size_t lds = sizeof(uint32_t) * crt_lds_words(KS);
+    if (ba) {
+        const uint32_t per_product = std::max<uint32_t>(1, crt_slots(lds) / static_cast<uint32_t>(nprod));
+        const dim3 grid(std::min((P.N + 63) / 64, per_product), 1, nprod);
+        launch(k_crt_mfma_batch<KS>, grid, dim3(64 * kWaves), lds, st, P, *ba);
+    } else {
+        const dim3 grid(std::min((count + 63) / 64, crt_slots(lds)));
+        launch(k_crt_mfma<KS>, grid, dim3(64 * kWaves), lds, st, Y, c, P, k0, count);
+    }
+}
+
+void crt_dispatch(const DevPlan& P, const uint32_t* Y, uint32_t* c, uint32_t k0, uint32_t count, hipStream_t st,
+                  const BatchArgs* ba = nullptr, int nprod = 0) {
+    check_crt_range(P);
+    switch (P.crt_ksteps) {
+        case 1: crt_launch<1>(P, Y, c, k0, count, st, ba, nprod); return;
+        case 2: crt_launch<2>(P, Y, c, k0, count, st, ba, nprod); return;
+        case 3: crt_launch<3>(P, Y, c, k0, count, st, ba, nprod); return;
+        case 4: crt_launch<4>(P, Y, c, k0, count, st, ba, nprod); return;
+        case 5: crt_launch<5>(P, Y, c, k0, count, st, ba, nprod); return;
+        case 6: crt_launch<6>(P, Y, c, k0, count, st, ba, nprod); return;
+        case 7: crt_launch<7>(P, Y, c, k0, count, st, ba, nprod); return;
+        case 8: crt_launch<8>(P, Y, c, k0, count, st, ba, nprod); return;
+        case 9: crt_launch<9>(P, Y, c, k0, count, st, ba, nprod); return;
+        default: crt_launch<10>(P, Y, c, k0, count, st, ba, nprod); return;
+    }
+}
+
+}  // namespace
+
 void launch_crt_mfma_range(const DevPlan& P, const uint32_t* Y, uint32_t* c, uint32_t k0, uint32_t count,
                            hipStream_t st) {
-    if (!P.crt_tab) throw std::runtime_error("matrix-core tables not prepared");
     if (count == 0) return;
-    const size_t lds = sizeof(uint32_t) * crt_lds_words(P.crt_ksteps, P.crt_stiles, P.K);
-    const dim3 grid(std::min((count + 63) / 64, crt_slots(lds)));
-    launch(k_crt_mfma, grid, dim3(64 * kWaves), lds, st, Y, c, P, k0, count);
+    crt_dispatch(P, Y, c, k0, count, st);
 }
 
 void launch_decompose_mfma_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st) {
@@ -785,11 +850,7 @@ void launch_decompose_mfma_batch(const DevPlan& P, const BatchArgs& ba, int coun
 }
 
 void launch_crt_mfma_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st) {
-    if (!P.crt_tab) throw std::runtime_error("matrix-core tables not prepared");
-    const size_t lds = sizeof(uint32_t) * crt_lds_words(P.crt_ksteps, P.crt_stiles, P.K);
-    const uint32_t per_product = std::max<uint32_t>(1, crt_slots(lds) / static_cast<uint32_t>(count));
-    const dim3 grid(std::min((P.N + 63) / 64, per_product), 1, count);
-    launch(k_crt_mfma_batch, grid, dim3(64 * kWaves), lds, st, P, ba);
+    crt_dispatch(P, nullptr, nullptr, 0, P.N, st, &ba, count);
 }
 
 }  // namespace pm
