@@ -26,11 +26,14 @@
 //     G[s][n] = sum_{(j,a)} m_{j,s-a} (u_{j,n,a} - 128)   |G| <= 4K 2^14 < 2^24
 //     sum_j y_j M_j = sum_s 256^s G[s][n] + 0x80808080 sum_j M_j   (exactly)
 //   MFMA rows are byte positions s, columns coefficients, K-dim (prime, byte).
-//   A lane's four rows are the four bytes of one 32-bit limb, so it folds them
-//   into one signed 64-bit limb partial; the same kernel then runs the carry
-//   chain, adds v (-P mod q) and reduces mod q (k_crt_mfma below).
-//   v = round(sum_j y_j / p_j) and the FP64 quotient estimate's sum are
-//   accumulated while the y bytes are loaded.
+//   v = round(sum_j y_j / p_j) and the FP64 quotient estimate Q are formed
+//   from sums accumulated while the y bytes are loaded; one more K-dim group,
+//   the quotient group, holds the balanced digits of Q and v against table
+//   columns -q, -q 2^32 and -P mod q, so the contraction itself yields
+//   R = sum_j y_j M_j + v (-P mod q) - Q q. A lane's four rows are the four
+//   bytes of one 32-bit limb, so it folds them into one signed 64-bit limb
+//   partial; the same kernel then runs the carry chain and reduces mod q
+//   (k_crt_mfma below).
 //
 // MFMA operand layout (16x16x64 i8): lane l supplies row (A) / column (B)
 // l % 16 and the 16 K-elements 16 (l / 16) + [0, 16) of each 64-wide K step;
@@ -54,7 +57,7 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr int kWaves = 4;          // waves per workgroup
 constexpr int kTiles = 4;          // 16-coefficient tiles per wave: 64 coefficients
 constexpr int kDecMaxKSteps = 4;   // 4L <= 256 bytes
-constexpr int kCrtMaxKSteps = 10;  // 4K <= 640
+constexpr int kCrtMaxKSteps = 11;  // 16 (ceil(K / 4) + 1) <= 704: the primes and the quotient group
 
 __device__ __forceinline__ v4i mfma16(const v4i& a, const v4i& b, const v4i& c) {
     return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
@@ -225,17 +228,18 @@ __global__ void __launch_bounds__(64 * kWaves) k_decompose_mfma_batch(DevPlan P,
 //   1. all threads: y[j][nb .. nb + 64) for every prime, byte-transposed per
 //      group of four primes (word a of group J holds byte a of y_4J .. y_4J+3),
 //      into LDS as int8 digits, with partial sums of y_j / p_j and y_j M_j / q;
-//      the M digit table is copied to LDS alongside. Wave 0 then forms
-//      v = round(sum y_j / p_j) and the quotient estimate
-//      Q = floor(sum y_j M_j / q + v (-P mod q) / q) per coefficient.
+//      the M digit table is copied to LDS alongside (once per workgroup). Each
+//      wave then forms v = round(sum y_j / p_j) and the quotient estimate
+//      Q = floor(sum y_j M_j / q + v (-P mod q) / q) per coefficient and
+//      writes their digits as the quotient group's B word.
 //   2. MFMA: wave w owns the limb tiles st = w, w + 4, w + 8, w + 12 (wave 0
 //      also st = 16) for all four 16-coefficient tiles, a 4 x 4 block of
 //      accumulators, so every A and B fragment read from LDS feeds four MFMAs.
 //      Each lane folds its four rows (the bytes of one limb) into a signed
 //      64-bit limb partial; after a barrier they go to LDS as [limb][coefficient].
-//   3. R = sum_l partial_l 2^(32 l) + offset correction + v (-P mod q) - Q q is
-//      congruent to the product mod q and lies in [-q, 2q) (Q is exact or one
-//      off). The L + 2 limbs are cut into four segments, one per wave, lane =
+//   3. R = sum_l partial_l 2^(32 l) + offset correction (the partials include
+//      v (-P mod q) - Q q) is congruent to the product mod q and lies in
+//      [-q, 2q) (Q is exact or one off). The L + 2 limbs are cut into four segments, one per wave, lane =
 //      coefficient: each wave runs its segment's carry chain with carry-in 0,
 //      then the carries between segments are resolved from each segment's
 //      carry-out, low 64 bits and all-zero / all-ones flags (a carry-in can
@@ -248,12 +252,14 @@ __global__ void __launch_bounds__(64 * kWaves) k_decompose_mfma_batch(DevPlan P,
 // 4 (4 ks + g) + r, byte a of y; the A word for row s is then word J = 4 ks + g
 // of digit-table row s - a.
 constexpr int kTabRows = 16 * 17 + 3;   // digit rows -3 .. 271 of the transposed table
-constexpr int kTabLoads = 11;           // 16-byte loads per thread for the table: 40 x 275 words
+constexpr int kTabLoads = 12;           // 16-byte loads per thread for the table: 44 x 275 words
 constexpr int kSegMax = 17;             // limbs per segment: 4 x 17 >= L + 2 for L <= 64
 
 // Digit table words, column-major: word (J, R) at J kTabRows + R holds byte R - 3
 // of the balanced digits of M_4J .. M_4J+3 (zero outside 0 .. 4L and for primes
-// past K), for the 4 KS prime groups that KS K-steps read; whole 16-byte units.
+// past K); column ceil(K / 4) those of -q, -q 2^32, -P mod q and 0 (the
+// quotient group); 4 KS columns in all, the ones that KS K-steps read; whole
+// 16-byte units.
 __host__ __device__ constexpr uint32_t crt_tab_words(int KS) {
     return 4u * static_cast<uint32_t>(KS) * kTabRows;
 }
@@ -264,7 +270,7 @@ __host__ __device__ constexpr uint32_t crt_tab_words(int KS) {
 //       (64 x kStage);
 //   R1  the digit table, loaded once and kept for every tile of the workgroup;
 //   aux FP64 partial sums (1024 words), reused in phase 3 for the segment
-//       exports and comparisons, and (v, Ql, Qh) per coefficient (192 words).
+//       exports and comparisons.
 // Every bound the tile loop tests is a compile-time one (KS) or a fixed size:
 // uniform runtime bounds became lane masks that the compiler hoisted out of
 // the loop and spilled.
@@ -277,7 +283,7 @@ __host__ __device__ constexpr uint32_t crt_r0_words(int KS) {
 }
 
 __host__ __device__ constexpr uint32_t crt_lds_words(int KS) {
-    return crt_r0_words(KS) + crt_tab_words(KS) + 1024 + 192;
+    return crt_r0_words(KS) + crt_tab_words(KS) + 1024;
 }
 
 // d[a] = byte a of w0, w1, w2, w3 (in that byte order), by byte permutes:
@@ -346,7 +352,6 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
     int* ex_s = ex_c + 256;                          // low-64-bit summary
     uint32_t* ex_f = reinterpret_cast<uint32_t*>(ex_c + 512);   // all-zero / all-ones flags
     int* cmpw = ex_c + 768;                          // comparison with q
-    uint32_t* vq = reinterpret_cast<uint32_t*>(fq + 4 * 64);                   // 3 x 64: v, Ql, Qh
     const uint32_t ntiles = (count + 63) / 64;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: scalar limb indices
@@ -420,15 +425,33 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
         fq[wave * 64 + lane] = f2;
     }
     __syncthreads();
-    if (wave == 0) {
+    {
+        // v and Q per coefficient, in every wave (no barrier before phase 2: all
+        // waves write the same quotient-group word, each before reading it)
         const double frac = fr[lane] + fr[64 + lane] + fr[128 + lane] + fr[192 + lane];
         const int v = crt_round(frac, K);
         // -P mod q is negvP row 1; v (-P mod q) / q is v nq[1]
         const uint64_t Q = quotient_estimate(fq[lane] + fq[64 + lane] + fq[128 + lane] + fq[192 + lane] +
                                              static_cast<double>(v) * P.nq[1]);
-        vq[lane] = static_cast<uint32_t>(v);
-        vq[64 + lane] = static_cast<uint32_t>(Q);
-        vq[128 + lane] = static_cast<uint32_t>(Q >> 32);
+        // the quotient group: "prime" 0 holds Q's balanced digits 0-3 (table
+        // column -q), 1 digits 4-5 (-q 2^32), 2 v's two digits (-P mod q), as
+        // signed bytes without the bias
+        long long xq = static_cast<long long>(Q), xv = v;
+        uint32_t qd[6], vd[2];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const int d = static_cast<int>((xq + 128) & 255) - 128;
+            qd[k] = static_cast<uint32_t>(d) & 255u;
+            xq = (xq - d) >> 8;
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int d = static_cast<int>((xv + 128) & 255) - 128;
+            vd[k] = static_cast<uint32_t>(d) & 255u;
+            xv = (xv - d) >> 8;
+        }
+        const uint32_t d0 = qd[0] | (qd[4] << 8) | (vd[0] << 16), d1 = qd[1] | (qd[5] << 8) | (vd[1] << 16);
+        yq[(K + 3) / 4 * 64 + lane] = make_uint4(qd[3], qd[2], d1, d0);
     }
     if (tile + tstep < ntiles) load_y(tile + tstep);
 
@@ -493,36 +516,23 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
     __syncthreads();
 
     // 3. one segment of 17 limbs per wave, one coefficient per lane; limbs past
-    // L + 2 are zero and only carry R's sign along
+    // L + 2 are zero and only carry R's sign along. The partials already hold
+    // v (-P mod q) - Q q (the quotient group); the offset correction remains.
     const int l0 = kSegMax * wave;
-    const uint32_t v = vq[lane], Ql = vq[64 + lane], Qh = vq[128 + lane];
-    // limb l of q, -P mod q and the offset correction at l + 2 (l >= -2), zero
+    // limb l of q and of the offset correction at l + 2 (l >= -2), zero
     // outside; loaded per tile (scalar loads), not hoisted out of the tile loop
     const kconst<uint32_t> lc = laundered(P.crt_lc);
     const kconst<uint32_t> lq = lc + 2;
-    const kconst<uint32_t> lpn = lc + 72 + 2;
     const kconst<uint32_t> lcorr = lc + 144 + 2;
     auto qlimb = [&](int l) -> uint32_t { return lq[l]; };
-    auto pnlimb = [&](int l) -> uint32_t { return lpn[l]; };
     uint32_t x[kSegMax];
     long long carry = 0;
-    {
-        uint64_t a1 = static_cast<uint64_t>(Ql) * qlimb(l0 - 1);
-        uint64_t b1 = static_cast<uint64_t>(Qh) * qlimb(l0 - 1);
-        uint64_t b2 = static_cast<uint64_t>(Qh) * qlimb(l0 - 2);
-        uint64_t c1 = static_cast<uint64_t>(v) * pnlimb(l0 - 1);
 #pragma unroll
-        for (int u = 0; u < kSegMax; ++u) {
-            const int l = l0 + u;
-            const long long pv = part[l * 64 + lane];
-            const uint64_t vp = static_cast<uint64_t>(v) * pnlimb(l);
-            long long s = carry + pv + static_cast<long long>(lcorr[l]) +
-                          static_cast<long long>(static_cast<uint32_t>(vp)) + static_cast<long long>(c1 >> 32) -
-                          q_times_limb(Ql, Qh, qlimb(l), a1, b1, b2);
-            c1 = vp;
-            x[u] = static_cast<uint32_t>(s);
-            carry = s >> 32;
-        }
+    for (int u = 0; u < kSegMax; ++u) {
+        const int l = l0 + u;
+        const long long s = carry + part[l * 64 + lane] + static_cast<long long>(lcorr[l]);
+        x[u] = static_cast<uint32_t>(s);
+        carry = s >> 32;
     }
     {
         bool uz = true, uo = true;
@@ -653,7 +663,7 @@ void balanced4(long long x, int8_t out[4]) {
 
 void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const void*, size_t)>& upload) {
     const int K = h.K, L = h.L;
-    if (L > 4 * kDecMaxKSteps * 4 || (4 * K + 63) / 64 > kCrtMaxKSteps) {
+    if (L > 4 * kDecMaxKSteps * 4 || ((K + 3) / 4 + 1 + 3) / 4 > kCrtMaxKSteps) {
         throw std::runtime_error("point too large for the matrix-core tables");
     }
     // Decompose tables: T_{j,t} = f_j 256^t mod p_j with f_j = 1 (the plain
@@ -694,8 +704,8 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
     // used only for L >= 8, so L / 4 >= 2 and the magic fits 32 bits
     d.dec_magic = static_cast<uint32_t>((uint64_t(1) << 32) / static_cast<uint64_t>(std::max(L / 4, 2)) + 1);
 
-    // CRT tables.
-    const int KS = (4 * K + 63) / 64;
+    // CRT tables: the prime groups, then the quotient group (column KJ).
+    const int KS = ((K + 3) / 4 + 1 + 3) / 4;
     const int ST = (4 * L + 4 + 15) / 16;
     const int nd = 4 * L + 1;
     std::vector<int8_t> m(static_cast<size_t>(K) * nd);
@@ -710,6 +720,31 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
                 tab[static_cast<size_t>(J) * kTabRows + R] |=
                     static_cast<uint32_t>(static_cast<uint8_t>(m[static_cast<size_t>(4 * J + r) * nd + (R - 3)]))
                     << (8 * r);
+    {
+        // -q as balanced digits: those of 2^(32 (L + 1)) - q over 4 (L + 1) + 1
+        // digits, less 256^(4 (L + 1)) at the top (a top digit of 0 or 1 becomes -1 or 0)
+        std::vector<uint32_t> tc(L + 1, 0);
+        uint64_t borrow = 1;   // ~q + 1 over L + 1 limbs
+        for (int l = 0; l <= L; ++l) {
+            const uint64_t s = static_cast<uint64_t>(~(l < L ? h.q[l] : 0u)) + borrow;
+            tc[l] = static_cast<uint32_t>(s);
+            borrow = s >> 32;
+        }
+        const int nq = 4 * (L + 1) + 1;
+        std::vector<int8_t> negq(nq), pn(nd);
+        balanced_digits(tc.data(), L + 1, negq.data(), nq);
+        negq[nq - 1] = static_cast<int8_t>(negq[nq - 1] - 1);
+        balanced_digits(h.negvP.data() + L, L, pn.data(), nd);   // row v = 1: -P mod q
+        auto put = [&](int digit, int r, int8_t v) {
+            if (digit + 3 >= kTabRows) throw std::logic_error("CRT quotient digits exceed the table");
+            tab[static_cast<size_t>(KJ) * kTabRows + digit + 3] |= static_cast<uint32_t>(static_cast<uint8_t>(v)) << (8 * r);
+        };
+        for (int s = 0; s < nq; ++s) {
+            put(s, 0, negq[s]);       // Q's digits 0-3 times -q
+            put(s + 4, 1, negq[s]);   // digits 4-5 times -q 2^32
+        }
+        for (int s = 0; s < nd; ++s) put(s, 2, pn[s]);   // v times -P mod q
+    }
     // 0x80808080 * sum_j M_j, exactly (L + 2 limbs).
     std::vector<uint64_t> acc(L + 2, 0);
     for (int j = 0; j < K; ++j)
@@ -824,7 +859,8 @@ void crt_dispatch(const DevPlan& P, const uint32_t* Y, uint32_t* c, uint32_t k0,
         case 7: crt_launch<7>(P, Y, c, k0, count, st, ba, nprod); return;
         case 8: crt_launch<8>(P, Y, c, k0, count, st, ba, nprod); return;
         case 9: crt_launch<9>(P, Y, c, k0, count, st, ba, nprod); return;
-        default: crt_launch<10>(P, Y, c, k0, count, st, ba, nprod); return;
+        case 10: crt_launch<10>(P, Y, c, k0, count, st, ba, nprod); return;
+        default: crt_launch<11>(P, Y, c, k0, count, st, ba, nprod); return;
     }
 }
 
