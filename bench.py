@@ -97,21 +97,30 @@ def phase_bytes(K, L):
             "ntt_rows_fused": 3 * kn, "ntt_cols_inv": 2 * kn, "crt": kn + nl}
 
 
-# Kernel names (as rocprofv3 reports them) of each phase, for the traffic table.
-PHASE_KERNELS = {"decompose_a": "k_decompose_mfma<4>", "decompose_b": "k_decompose_mfma<4>",
-                 "ntt_cols_fwd_a": "k_ntt_col_reg<8,true>", "ntt_cols_fwd_b": "k_ntt_col_reg<8,true>",
-                 "ntt_rows_fused": "k_ntt_rows_fused_reg<8>", "ntt_cols_inv": "k_ntt_col_reg<8,false>",
-                 "crt": "k_crt_mfma"}
+def phase_kernels(plan):
+    """Kernel names (as rocprofv3 reports them, spaces removed) of each phase of
+    the N = 65536 one-product schedule: the register NTT kernels take the twiddle
+    source and the basis (lazily reduced butterflies below 2^30) as template
+    arguments, the CRT its K-steps (one group per four primes plus the quotient
+    group, 16 groups per K-step)."""
+    lazy = "true" if plan.basis == 30 else "false"
+    gen = "true" if plan.twiddles == 1 else "false"
+    ks = ((plan.K + 3) // 4 + 1 + 3) // 4
+    col = "k_ntt_col_reg<8,%s,5," + lazy + ">"
+    return {"decompose_a": "k_decompose_mfma<4>", "decompose_b": "k_decompose_mfma<4>",
+            "ntt_cols_fwd_a": col % "true", "ntt_cols_fwd_b": col % "true",
+            "ntt_rows_fused": "k_ntt_rows_fused_reg<8,%s,%s>" % (gen, lazy), "ntt_cols_inv": col % "false",
+            "crt": "k_crt_mfma<%d>" % ks}
 
 
-def measured_traffic(phase):
-    """HBM bytes per launch of a phase's kernel from the committed counter table
+def measured_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed counter table
     (profiles/hbm_traffic_n65536_w2048.txt: kernel, scaled FETCH_SIZE + WRITE_SIZE
     bytes per dispatch), or None when the table or the kernel is absent."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_n65536_w2048.txt")
     if not os.path.exists(path):
         return None
-    want = PHASE_KERNELS.get(phase)
+    want = kernel
     with open(path) as f:
         for line in f:
             parts = line.split()
@@ -278,9 +287,10 @@ def main():
     nbytes = phase_bytes(K, L)
     dom = max(phases, key=phases.get)
     achieved = nbytes[dom] / (phases[dom] * 1e-3) / 1e9
+    kname = phase_kernels(plans[0])[dom]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom), "kernel": dom,
-                "kernel_name": PHASE_KERNELS[dom], "kernel_ms": round(phases[dom], 5),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kname), "kernel": dom,
+                "kernel_name": kname, "kernel_ms": round(phases[dom], 5),
                 "algorithmic_bytes": nbytes[dom]}
     prod_bytes = 12 * N * L + 56 * K * N
     cpu = None if args.no_cpu else cpu_baseline(points[:1])
