@@ -139,49 +139,35 @@ int row_twiddle_policy(const pm::HostPlan& h) {
     return 16.0 * h.K * h.N > 64e6 ? 1 : 0;
 }
 
-// The prime basis of a new plan. AUTO (after PM_BASIS=p30|p31|auto, anything
-// else refused): primes below 2^30 when the geometry has register kernels for
-// every NTT pass, the only kernels with lazily reduced butterflies, and enough
-// such primes exist; otherwise the reference's primes below 2^31.
-pm::HostPlan plan_in_basis(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, int basis, bool& lazy) {
+// The prime basis of a new plan: the reference's rule (primes below 2^31)
+// unless PM_BASIS=p30 (PM_BASIS=p31 and auto name the default) or an explicit
+// PM_BASIS_P30 asks for the same walk below 2^30. `kernels`: every NTT pass of
+// the geometry has a register kernel, whose faster butterfly forms the plan
+// then uses.
+pm::HostPlan plan_in_basis(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, int basis, bool& kernels) {
     if (basis == PM_BASIS_AUTO) {
+        basis = PM_BASIS_P31;
         if (const char* v = std::getenv("PM_BASIS")) {
             if (!std::strcmp(v, "p30"))
                 basis = PM_BASIS_P30;
-            else if (!std::strcmp(v, "p31"))
-                basis = PM_BASIS_P31;
             else
-                need(!std::strcmp(v, "auto"), "PM_BASIS must be p30, p31 or auto");
+                need(!std::strcmp(v, "p31") || !std::strcmp(v, "auto"), "PM_BASIS must be p30, p31 or auto");
         }
     }
-    need(basis == PM_BASIS_AUTO || basis == PM_BASIS_P30 || basis == PM_BASIS_P31,
-         "basis must be PM_BASIS_AUTO, PM_BASIS_P30 or PM_BASIS_P31");
-    uint32_t logN = 0;
-    while (logN < 31 && (1u << logN) < N) ++logN;
+    need(basis == PM_BASIS_P30 || basis == PM_BASIS_P31, "basis must be PM_BASIS_AUTO, PM_BASIS_P30 or PM_BASIS_P31");
+    pm::HostPlan h = pm::make_plan(N, W, q, static_cast<int>(L), basis == PM_BASIS_P30 ? 30 : 31);
     pm::DevPlan g{};
-    pm::ntt_geometry(logN, g.logS, g.logR);
-    const bool kernels = pm::batch_kernels_ok(g);
-    pm::HostPlan h;
-    if (basis == PM_BASIS_AUTO && kernels) {
-        try {
-            h = pm::make_plan(N, W, q, static_cast<int>(L), 30);
-        } catch (const std::runtime_error&) {
-            // too few primes below 2^30 (or an invalid point, which the
-            // reference's basis refuses again with the reference's message)
-            h = pm::make_plan(N, W, q, static_cast<int>(L), 31);
-        }
-    } else {
-        h = pm::make_plan(N, W, q, static_cast<int>(L), basis == PM_BASIS_P30 ? 30 : 31);
-    }
-    lazy = kernels && h.pbits == 30;
+    g.logS = h.logS;
+    g.logR = h.logR;
+    kernels = pm::batch_kernels_ok(g);
     return h;
 }
 
 pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, int basis) {
     need(q != nullptr, "q is null");
     std::unique_ptr<pm_plan> pl(new pm_plan());
-    bool lazy = false;
-    pl->host = plan_in_basis(N, W, L, q, basis, lazy);
+    bool kernels = false;
+    pl->host = plan_in_basis(N, W, L, q, basis, kernels);
     const pm::HostPlan& h = pl->host;
     const int K = h.K;
     const int row_gen = row_twiddle_policy(h);   // before any device work
@@ -211,7 +197,7 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q,
     d.L = h.L;
     d.K = K;
     d.row_gen = row_gen;
-    d.lazy = lazy ? 1 : 0;
+    d.lazy = kernels ? (h.pbits == 30 ? 2 : 1) : 0;
     d.q_drop = h.q_drop;
     d.q_top = h.q_top;
     d.pc = pl->upload(pc);

@@ -26,8 +26,10 @@ struct DevPlan {
     uint32_t N, logN, logS, logR;
     int L, K, q_drop;
     int row_gen;              // row-pass twiddles: 1 generated (rz, rom), 0 from the full tables
-    // 1: every prime is below 2^30 and every pass has a register kernel, which
-    // then keep NTT-domain residues lazily reduced (pm_ntt.hip); 0: canonical.
+    // Butterfly form of the register NTT kernels (pm_ntt.hip), used when every
+    // pass has one: 1 forward residues in [0, 2p) (any p < 2^31), 2 forward and
+    // inverse lazily reduced (every prime below 2^30); 0 canonical (the generic
+    // kernels, or a geometry without register kernels for every pass).
     int lazy;
     double q_top;
     const PrimeConst* pc;     // K

@@ -64,13 +64,33 @@ __device__ __forceinline__ void gs_lazy(uint32_t& x, uint32_t& y, const uint2 w,
     y = d * w.x - mulhi32(d, w.y) * p;
 }
 
-template <bool kFwd, bool kLazy>
+// The forward butterfly for any p < 2^31 with residues in [0, 2p): x comes
+// down to [0, p); the lazy Shoup remainder v of y w (in [0, 2p)) is formed as
+// m = -v - 1 mod 2^32, and max(m, m + p) = -v_r - 1 with v_r = v mod p in
+// [0, p) (the max picks m + p exactly when it does not wrap, that is when
+// v >= p + 1, and otherwise m, whose v is then p or below; v = p itself gives
+// m + p = 2^32 - 1). Then x + v_r = ~M + x and x - v_r + p = x + M + p + 1
+// take one instruction each: 10 VALU ops against 12 for the canonical form.
+__device__ __forceinline__ void ct_half(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
+    const uint32_t m = mulhi32(y, w.y) * p - y * w.x - 1u;
+    const uint32_t M = max(m, m + p);
+    const uint32_t xr = csub(x, p);
+    x = ~M + xr;
+    y = xr + M + (p + 1u);
+}
+
+// Butterfly forms (kMode): 0 canonical, residues in [0, p); 1 forward for any
+// p < 2^31 with residues in [0, 2p) (ct_half), inverse canonical; 2 both
+// lazily reduced for p < 2^30 (ct_lazy, gs_lazy).
+template <bool kFwd, int kMode>
 __device__ __forceinline__ void bfly(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    if (kFwd && kLazy)
+    if (kFwd && kMode == 2)
         ct_lazy(x, y, w, p);
+    else if (kFwd && kMode == 1)
+        ct_half(x, y, w, p);
     else if (kFwd)
         ct(x, y, w, p);
-    else if (kLazy)
+    else if (kMode == 2)
         gs_lazy(x, y, w, p);
     else
         gs(x, y, w, p);
@@ -142,7 +162,7 @@ __device__ __forceinline__ auto row_tw(const DevPlan& P, int j, uint32_t rho, bo
 }
 
 // Phase A on one or two register sets (kTwo: x and y share the twiddles).
-template <int LOGM, bool kFwd, bool kTwo, bool kLazy, class TW>
+template <int LOGM, bool kFwd, bool kTwo, int kMode, class TW>
 __device__ __forceinline__ void phase_a(uint32_t (&x)[16], uint32_t (&y)[16], const TW& tw, uint32_t p) {
     constexpr int GA = 1 << (LOGM - 4), NG = 16 / GA;
 #pragma unroll
@@ -156,15 +176,15 @@ __device__ __forceinline__ void phase_a(uint32_t (&x)[16], uint32_t (&y)[16], co
             for (int i = 0; i < NG; ++i) {
 #pragma unroll
                 for (int k = blk * 2 * d; k < blk * 2 * d + d; ++k) {
-                    bfly<kFwd, kLazy>(x[i * GA + k], x[i * GA + k + d], w, p);
-                    if (kTwo) bfly<kFwd, kLazy>(y[i * GA + k], y[i * GA + k + d], w, p);
+                    bfly<kFwd, kMode>(x[i * GA + k], x[i * GA + k + d], w, p);
+                    if (kTwo) bfly<kFwd, kMode>(y[i * GA + k], y[i * GA + k + d], w, p);
                 }
             }
         }
     }
 }
 
-template <bool kFwd, bool kTwo, bool kLazy, class TW>
+template <bool kFwd, bool kTwo, int kMode, class TW>
 __device__ __forceinline__ void phase_b(uint32_t (&x)[16], uint32_t (&y)[16], const TW& tw, uint32_t t,
                                         uint32_t p) {
 #pragma unroll
@@ -176,8 +196,8 @@ __device__ __forceinline__ void phase_b(uint32_t (&x)[16], uint32_t (&y)[16], co
             const uint2 w = tw.b(e, t, blk);
 #pragma unroll
             for (int k = blk * 2 * d; k < blk * 2 * d + d; ++k) {
-                bfly<kFwd, kLazy>(x[k], x[k + d], w, p);
-                if (kTwo) bfly<kFwd, kLazy>(y[k], y[k + d], w, p);
+                bfly<kFwd, kMode>(x[k], x[k + d], w, p);
+                if (kTwo) bfly<kFwd, kMode>(y[k], y[k + d], w, p);
             }
         }
     }
@@ -236,9 +256,10 @@ struct BufTw {
 // slot padded(u) of column c is at padded(u) C + c; with t < TPS and
 // TPS NG = 16, padded(t + TPS i + 16 k) = t + TPS i + 17 k and
 // padded(16 t + k) = 17 t + k, so every slot is a lane base plus a constant.
-// kLazy (p < 2^30): the forward pass leaves its output in [0, 4p), the inverse
-// pass takes input in [0, 2p) and leaves it canonical.
-template <int LOGM, bool kFwd, int LOGC, bool kLazy>
+// kMode 1: the forward pass leaves its output in [0, 2p); kMode 2 (p < 2^30):
+// the forward pass leaves [0, 4p), the inverse pass takes input in [0, 2p) and
+// leaves it canonical.
+template <int LOGM, bool kFwd, int LOGC, int kMode>
 __device__ __forceinline__ void col_pass(uint32_t* __restrict__ X, const DevPlan& P) {
     extern __shared__ uint32_t lds[];
     constexpr uint32_t M = 1u << LOGM, TPS = M / 16, GA = TPS, NG = 16 / GA, C = 1u << LOGC;
@@ -259,7 +280,7 @@ __device__ __forceinline__ void col_pass(uint32_t* __restrict__ X, const DevPlan
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
             for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = x.ld(va, 4u * (TPS * i + 16 * k) * S);
-        phase_a<LOGM, true, false, kLazy>(r, r, tw, p);
+        phase_a<LOGM, true, false, kMode>(r, r, tw, p);
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -267,13 +288,13 @@ __device__ __forceinline__ void col_pass(uint32_t* __restrict__ X, const DevPlan
         __syncthreads();
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) r[k] = lb[k * C];
-        phase_b<true, false, kLazy>(r, r, tw, t, p);
+        phase_b<true, false, kMode>(r, r, tw, t, p);
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) x.st(r[k], vb, 4u * k * S);
     } else {
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) r[k] = x.ld(vb, 4u * k * S);
-        phase_b<false, false, kLazy>(r, r, tw, t, p);
+        phase_b<false, false, kMode>(r, r, tw, t, p);
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) lb[k * C] = r[k];
         __syncthreads();
@@ -281,8 +302,8 @@ __device__ __forceinline__ void col_pass(uint32_t* __restrict__ X, const DevPlan
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
             for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = la[(TPS * i + 17 * k) * C];
-        phase_a<LOGM, false, false, kLazy>(r, r, tw, p);
-        if (kLazy) {
+        phase_a<LOGM, false, false, kMode>(r, r, tw, p);
+        if (kMode == 2) {
 #pragma unroll
             for (uint32_t k = 0; k < 16; ++k) r[k] = csub(r[k], p);
         }
@@ -293,23 +314,24 @@ __device__ __forceinline__ void col_pass(uint32_t* __restrict__ X, const DevPlan
     }
 }
 
-template <int LOGM, bool kFwd, int LOGC, bool kLazy>
+template <int LOGM, bool kFwd, int LOGC, int kMode>
 __global__ void __launch_bounds__(512) k_ntt_col_reg(uint32_t* __restrict__ X, DevPlan P) {
-    col_pass<LOGM, kFwd, LOGC, kLazy>(X, P);
+    col_pass<LOGM, kFwd, LOGC, kMode>(X, P);
 }
 
 // A batch: forward, blockIdx.z = 2 (product) + (0: A, 1: B); inverse, blockIdx.z = product (A).
-template <int LOGM, bool kFwd, int LOGC, bool kLazy>
+template <int LOGM, bool kFwd, int LOGC, int kMode>
 __global__ void __launch_bounds__(512) k_ntt_col_batch(DevPlan P, BatchArgs ba) {
     const int i = kFwd ? blockIdx.z >> 1 : blockIdx.z;
-    col_pass<LOGM, kFwd, LOGC, kLazy>(kFwd && (blockIdx.z & 1) ? ba.B[i] : ba.A[i], P);
+    col_pass<LOGM, kFwd, LOGC, kMode>(kFwd && (blockIdx.z & 1) ? ba.B[i] : ba.A[i], P);
 }
 
 // Row pass alone (forward or inverse): G = 2^logG rows of M = S elements.
 // Twiddles are generated per row (kGen) or read from the full table; phase
 // A's are staged once per row into LDS by the row's first threads. The output
-// is canonical; kLazy (p < 2^30) accepts forward input in [0, 4p).
-template <int LOGM, bool kFwd, bool kGen, bool kLazy>
+// is canonical; the forward pass accepts input in [0, 2p) (kMode 1) or [0, 4p)
+// (kMode 2).
+template <int LOGM, bool kFwd, bool kGen, int kMode>
 __global__ void __launch_bounds__(512)
 k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
     extern __shared__ uint32_t lds[];
@@ -332,7 +354,7 @@ k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
 #pragma unroll
             for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = x[t + TPS * i + 16 * k];
         __syncthreads();
-        phase_a<LOGM, true, false, kLazy>(r, r, ta, pc.p);
+        phase_a<LOGM, true, false, kMode>(r, r, ta, pc.p);
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -340,10 +362,13 @@ k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
         __syncthreads();
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) r[k] = l[padded(16 * t + k)];
-        phase_b<true, false, kLazy>(r, r, gen, t, pc.p);
-        if (kLazy) {
+        phase_b<true, false, kMode>(r, r, gen, t, pc.p);
+        if (kMode == 2) {
 #pragma unroll
             for (uint32_t k = 0; k < 16; ++k) r[k] = csub(csub(r[k], 2 * pc.p), pc.p);
+        } else if (kMode == 1) {
+#pragma unroll
+            for (uint32_t k = 0; k < 16; ++k) r[k] = csub(r[k], pc.p);
         }
         // phase B layout: 16 consecutive words per thread
 #pragma unroll
@@ -351,7 +376,7 @@ k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
     } else {
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) r[k] = x[16 * t + k];
-        phase_b<false, false, kLazy>(r, r, gen, t, pc.p);
+        phase_b<false, false, kMode>(r, r, gen, t, pc.p);
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) l[padded(16 * t + k)] = r[k];
         __syncthreads();
@@ -359,8 +384,8 @@ k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
             for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = l[padded(t + TPS * i + 16 * k)];
-        phase_a<LOGM, false, false, kLazy>(r, r, ta, pc.p);
-        if (kLazy) {
+        phase_a<LOGM, false, false, kMode>(r, r, ta, pc.p);
+        if (kMode == 2) {
 #pragma unroll
             for (uint32_t k = 0; k < 16; ++k) r[k] = csub(r[k], pc.p);
         }
@@ -379,11 +404,12 @@ __device__ __forceinline__ uint32_t pointwise_mul(uint32_t a, uint32_t b, const 
 }
 
 // Row pass fused: forward rows of A and B (sharing twiddle loads), the pointwise
-// product with N^-1 (P/p_j)^-1 into A, inverse rows of A. kLazy (p < 2^30):
-// input in [0, 4p); both factors of the pointwise product are brought to
-// [0, 2p), which keeps a b below 2^32 p and the Montgomery product in [0, 2p),
-// the inverse stages' input range; the output is left in [0, 2p).
-template <int LOGM, bool kGen, bool kLazy>
+// product with N^-1 (P/p_j)^-1 into A, inverse rows of A. kMode 1: input in
+// [0, 2p), both factors brought to [0, p) for the canonical Montgomery product.
+// kMode 2 (p < 2^30): input in [0, 4p); both factors are brought to [0, 2p),
+// which keeps a b below 2^32 p and the Montgomery product in [0, 2p), the
+// inverse stages' input range; the output is left in [0, 2p).
+template <int LOGM, bool kGen, int kMode>
 __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, const DevPlan& P,
                                            uint32_t logG) {
     extern __shared__ uint32_t lds[];
@@ -411,7 +437,7 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
             b[i * GA + k] = Bm[row + t + TPS * i + 16 * k];
         }
     wave_sync();
-    phase_a<LOGM, true, true, kLazy>(a, b, fa, pc.p);
+    phase_a<LOGM, true, true, kMode>(a, b, fa, pc.p);
 #pragma unroll
     for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -427,16 +453,19 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
     wave_sync();
 #pragma unroll
     for (uint32_t k = 0; k < 16; ++k) b[k] = la[padded(16 * t + k)];
-    phase_b<true, true, kLazy>(a, b, fw, t, pc.p);
-    if (kLazy) {
+    phase_b<true, true, kMode>(a, b, fw, t, pc.p);
+    if (kMode == 2) {
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k)
             a[k] = mont_mul_lazy(csub(a[k], 2 * pc.p), csub(b[k], 2 * pc.p), pc.p, pc.pneg);
+    } else if (kMode == 1) {
+#pragma unroll
+        for (uint32_t k = 0; k < 16; ++k) a[k] = pointwise_mul(csub(a[k], pc.p), csub(b[k], pc.p), pc);
     } else {
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) a[k] = pointwise_mul(a[k], b[k], pc);
     }
-    phase_b<false, false, kLazy>(a, a, iw, t, pc.p);
+    phase_b<false, false, kMode>(a, a, iw, t, pc.p);
     // Each thread rewrites only the LDS words it read above (b's, phase B layout).
 #pragma unroll
     for (uint32_t k = 0; k < 16; ++k) la[padded(16 * t + k)] = a[k];
@@ -445,23 +474,23 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
     for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
         for (uint32_t k = 0; k < GA; ++k) a[i * GA + k] = la[padded(t + TPS * i + 16 * k)];
-    phase_a<LOGM, false, false, kLazy>(a, a, ia, pc.p);
+    phase_a<LOGM, false, false, kMode>(a, a, ia, pc.p);
 #pragma unroll
     for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
         for (uint32_t k = 0; k < GA; ++k) A[row + t + TPS * i + 16 * k] = a[i * GA + k];
 }
 
-template <int LOGM, bool kGen, bool kLazy>
+template <int LOGM, bool kGen, int kMode>
 __global__ void __launch_bounds__(512)
 k_ntt_rows_fused_reg(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, DevPlan P, uint32_t logG) {
-    rows_fused<LOGM, kGen, kLazy>(A, Bm, P, logG);
+    rows_fused<LOGM, kGen, kMode>(A, Bm, P, logG);
 }
 
 // A batch: blockIdx.z = product.
-template <int LOGM, bool kGen, bool kLazy>
+template <int LOGM, bool kGen, int kMode>
 __global__ void __launch_bounds__(512) k_ntt_rows_fused_batch(DevPlan P, BatchArgs ba, uint32_t logG) {
-    rows_fused<LOGM, kGen, kLazy>(ba.A[blockIdx.z], ba.B[blockIdx.z], P, logG);
+    rows_fused<LOGM, kGen, kMode>(ba.A[blockIdx.z], ba.B[blockIdx.z], P, logG);
 }
 
 bool reg_ok(uint32_t logm) { return logm >= 4 && logm <= 8; }
@@ -478,18 +507,18 @@ uint32_t row_logG(const DevPlan& P) {
     return want < P.logR ? want : P.logR;
 }
 
-template <int LOGM, int LOGC, bool kLazy>
+template <int LOGM, int LOGC, int kMode>
 void col_launch3(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st, dim3 grid, dim3 block, size_t lds,
                  const BatchArgs* ba) {
     if (ba) {
         if (forward)
-            launch((k_ntt_col_batch<LOGM, true, LOGC, kLazy>), grid, block, lds, st, P, *ba);
+            launch((k_ntt_col_batch<LOGM, true, LOGC, kMode>), grid, block, lds, st, P, *ba);
         else
-            launch((k_ntt_col_batch<LOGM, false, LOGC, kLazy>), grid, block, lds, st, P, *ba);
+            launch((k_ntt_col_batch<LOGM, false, LOGC, kMode>), grid, block, lds, st, P, *ba);
     } else if (forward) {
-        launch((k_ntt_col_reg<LOGM, true, LOGC, kLazy>), grid, block, lds, st, X, P);
+        launch((k_ntt_col_reg<LOGM, true, LOGC, kMode>), grid, block, lds, st, X, P);
     } else {
-        launch((k_ntt_col_reg<LOGM, false, LOGC, kLazy>), grid, block, lds, st, X, P);
+        launch((k_ntt_col_reg<LOGM, false, LOGC, kMode>), grid, block, lds, st, X, P);
     }
 }
 
@@ -499,10 +528,12 @@ void col_launch2(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st, co
     const dim3 grid(1u << (P.logS - LOGC), P.K, ba ? (forward ? 2 * count : count) : 1);
     const dim3 block(1u << (LOGC + LOGM - 4));
     const size_t lds = sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << LOGC;
-    if (P.lazy)
-        col_launch3<LOGM, LOGC, true>(P, X, forward, st, grid, block, lds, ba);
+    if (P.lazy == 2)
+        col_launch3<LOGM, LOGC, 2>(P, X, forward, st, grid, block, lds, ba);
+    else if (P.lazy == 1)
+        col_launch3<LOGM, LOGC, 1>(P, X, forward, st, grid, block, lds, ba);
     else
-        col_launch3<LOGM, LOGC, false>(P, X, forward, st, grid, block, lds, ba);
+        col_launch3<LOGM, LOGC, 0>(P, X, forward, st, grid, block, lds, ba);
 }
 
 // (R, C) shape of the column pass as one switch key, or -1 if only the generic kernels apply.
@@ -533,17 +564,17 @@ void col_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     }
 }
 
-template <int LOGM, bool kLazy>
+template <int LOGM, int kMode>
 void row_launch3(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st, dim3 grid, dim3 block, size_t lds,
                  uint32_t logG) {
     if (forward && P.row_gen)
-        launch((k_ntt_row_reg<LOGM, true, true, kLazy>), grid, block, lds, st, X, P, logG);
+        launch((k_ntt_row_reg<LOGM, true, true, kMode>), grid, block, lds, st, X, P, logG);
     else if (forward)
-        launch((k_ntt_row_reg<LOGM, true, false, kLazy>), grid, block, lds, st, X, P, logG);
+        launch((k_ntt_row_reg<LOGM, true, false, kMode>), grid, block, lds, st, X, P, logG);
     else if (P.row_gen)
-        launch((k_ntt_row_reg<LOGM, false, true, kLazy>), grid, block, lds, st, X, P, logG);
+        launch((k_ntt_row_reg<LOGM, false, true, kMode>), grid, block, lds, st, X, P, logG);
     else
-        launch((k_ntt_row_reg<LOGM, false, false, kLazy>), grid, block, lds, st, X, P, logG);
+        launch((k_ntt_row_reg<LOGM, false, false, kMode>), grid, block, lds, st, X, P, logG);
 }
 
 template <int LOGM>
@@ -553,10 +584,12 @@ void row_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     const dim3 block(1u << (logG + LOGM - 4));
     const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (sizeof(uint2) << (LOGM - 4 + logG));
-    if (P.lazy)
-        row_launch3<LOGM, true>(P, X, forward, st, grid, block, lds, logG);
+    if (P.lazy == 2)
+        row_launch3<LOGM, 2>(P, X, forward, st, grid, block, lds, logG);
+    else if (P.lazy == 1)
+        row_launch3<LOGM, 1>(P, X, forward, st, grid, block, lds, logG);
     else
-        row_launch3<LOGM, false>(P, X, forward, st, grid, block, lds, logG);
+        row_launch3<LOGM, 0>(P, X, forward, st, grid, block, lds, logG);
 }
 
 template <int LOGM>
@@ -566,14 +599,14 @@ void fused_launch_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStr
     const dim3 block(1u << (logG + LOGM - 4));
     const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (2 * sizeof(uint2) << (LOGM - 4 + logG));
-    if (P.row_gen && P.lazy)
-        launch((k_ntt_rows_fused_batch<LOGM, true, true>), grid, block, lds, st, P, ba, logG);
-    else if (P.row_gen)
-        launch((k_ntt_rows_fused_batch<LOGM, true, false>), grid, block, lds, st, P, ba, logG);
-    else if (P.lazy)
-        launch((k_ntt_rows_fused_batch<LOGM, false, true>), grid, block, lds, st, P, ba, logG);
-    else
-        launch((k_ntt_rows_fused_batch<LOGM, false, false>), grid, block, lds, st, P, ba, logG);
+    switch (P.lazy * 2 + (P.row_gen ? 1 : 0)) {
+        case 5: launch((k_ntt_rows_fused_batch<LOGM, true, 2>), grid, block, lds, st, P, ba, logG); break;
+        case 4: launch((k_ntt_rows_fused_batch<LOGM, false, 2>), grid, block, lds, st, P, ba, logG); break;
+        case 3: launch((k_ntt_rows_fused_batch<LOGM, true, 1>), grid, block, lds, st, P, ba, logG); break;
+        case 2: launch((k_ntt_rows_fused_batch<LOGM, false, 1>), grid, block, lds, st, P, ba, logG); break;
+        case 1: launch((k_ntt_rows_fused_batch<LOGM, true, 0>), grid, block, lds, st, P, ba, logG); break;
+        default: launch((k_ntt_rows_fused_batch<LOGM, false, 0>), grid, block, lds, st, P, ba, logG); break;
+    }
 }
 
 template <int LOGM>
@@ -583,14 +616,14 @@ void fused_launch(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t 
     const dim3 block(1u << (logG + LOGM - 4));
     const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (2 * sizeof(uint2) << (LOGM - 4 + logG));
-    if (P.row_gen && P.lazy)
-        launch((k_ntt_rows_fused_reg<LOGM, true, true>), grid, block, lds, st, A, B, P, logG);
-    else if (P.row_gen)
-        launch((k_ntt_rows_fused_reg<LOGM, true, false>), grid, block, lds, st, A, B, P, logG);
-    else if (P.lazy)
-        launch((k_ntt_rows_fused_reg<LOGM, false, true>), grid, block, lds, st, A, B, P, logG);
-    else
-        launch((k_ntt_rows_fused_reg<LOGM, false, false>), grid, block, lds, st, A, B, P, logG);
+    switch (P.lazy * 2 + (P.row_gen ? 1 : 0)) {
+        case 5: launch((k_ntt_rows_fused_reg<LOGM, true, 2>), grid, block, lds, st, A, B, P, logG); break;
+        case 4: launch((k_ntt_rows_fused_reg<LOGM, false, 2>), grid, block, lds, st, A, B, P, logG); break;
+        case 3: launch((k_ntt_rows_fused_reg<LOGM, true, 1>), grid, block, lds, st, A, B, P, logG); break;
+        case 2: launch((k_ntt_rows_fused_reg<LOGM, false, 1>), grid, block, lds, st, A, B, P, logG); break;
+        case 1: launch((k_ntt_rows_fused_reg<LOGM, true, 0>), grid, block, lds, st, A, B, P, logG); break;
+        default: launch((k_ntt_rows_fused_reg<LOGM, false, 0>), grid, block, lds, st, A, B, P, logG); break;
+    }
 }
 
 }  // namespace

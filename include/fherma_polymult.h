@@ -49,14 +49,15 @@ const char* pm_last_error(void);
  * power of two, L outside [1, 64], q whose top limb is zero, and points that
  * need more primes than exist. W is recorded, not checked (as in the reference).
  *
- * Prime bases. PM_BASIS_P31 is the reference's rule above. PM_BASIS_P30 runs
- * the same walk below 2^30: about 3 % more primes, but 4p < 2^32 lets the NTT
- * kernels keep residues lazily reduced, with a third fewer instructions per
- * forward butterfly. The product c is identical in either basis.
- * pm_plan_create uses PM_BASIS_AUTO: P30 when every NTT pass of the geometry
- * has a register kernel (N = 2^8 .. 2^16) and enough such primes exist, else
- * P31; the environment variable PM_BASIS=p30|p31|auto overrides it (any other
- * value makes plan creation fail with PM_EINVAL). */
+ * Prime bases. PM_BASIS_P31 is the reference's rule above, and the default
+ * (PM_BASIS_AUTO, pm_plan_create). PM_BASIS_P30 runs the same walk below 2^30:
+ * about 4 % more primes, but 4p < 2^32 lets the NTT kernels keep residues lazily
+ * reduced in both directions (8 and 9 instructions per butterfly against 10 and
+ * 11 in the default basis, 12 and 11 canonical). The product c is identical in
+ * either basis; a point with too few primes below 2^30 is refused.
+ * The environment variable PM_BASIS=p30 makes pm_plan_create use PM_BASIS_P30
+ * (p31 or auto: the default; any other value makes plan creation fail with
+ * PM_EINVAL). */
 typedef enum {
     PM_BASIS_AUTO = 0,
     PM_BASIS_P30 = 30,

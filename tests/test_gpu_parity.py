@@ -17,10 +17,10 @@ polymult = pytest.importorskip("polymult")
 pytestmark = pytest.mark.gpu
 
 FORMS = [("plain", 0), ("mfma", 1)]
-# prime bases: the default choice (primes below 2^30 on the register-kernel
-# geometries, N = 2^8 .. 2^16, with lazily reduced NTT butterflies) and the
-# reference's primes below 2^31 with canonical butterflies
-BASES = [("auto", 0), ("p31", 31)]
+# prime bases: the default (the reference's primes below 2^31; forward
+# residues in [0, 2p) on the register-kernel geometries, N = 2^8 .. 2^16) and
+# the opt-in primes below 2^30 (lazily reduced butterflies both ways)
+BASES = [("default", 0), ("p30", 30)]
 
 
 def say(*args):
@@ -74,15 +74,14 @@ def test_patterns_match_gmp(golden, name, form, basis):
     run_case(golden, name, form, basis)
 
 
-@pytest.mark.parametrize("N,W,want", [(64, 100, 31), (256, 1747, 30), (4096, 256, 30), (65536, 2048, 30),
-                                      (131072, 512, 31)])
-def test_default_basis_choice(N, W, want):
-    """Primes below 2^30 where every NTT pass has a register kernel (N = 2^8 ..
-    2^16), the reference's below 2^31 elsewhere; an explicit basis is kept."""
+@pytest.mark.parametrize("N,W", [(64, 100), (256, 1747), (4096, 256), (65536, 2048), (131072, 512)])
+def test_basis_choice(N, W):
+    """The default plan takes the reference's primes (the reference's K); an
+    explicit basis is kept."""
     q, _, _ = oracle.gen_point(11, N, W)
     with polymult.Plan(N, W, q) as plan:
-        assert plan.basis == want
-        assert np.array_equal(plan.primes, oracle.choose_primes(N, _qbits(q), want))
+        assert plan.basis == 31
+        assert np.array_equal(plan.primes, oracle.choose_primes(N, _qbits(q)))
     for basis in (polymult.BASIS_P30, polymult.BASIS_P31):
         with polymult.Plan(N, W, q, basis=basis) as plan:
             assert plan.basis == basis and plan.K == len(oracle.choose_primes(N, _qbits(q), basis))
@@ -266,7 +265,7 @@ def test_row_twiddle_sources_match_gmp(golden, name):
     give the GMP product (the reference's fuzz runs tw-gen and tw-table)."""
     case = golden[name]
     q, a, b = pointgen.operands(case)
-    for form, basis in ((polymult.FORM_PLAIN, 0), (polymult.FORM_MFMA, 0), (polymult.FORM_MFMA, 31)):
+    for form, basis in ((polymult.FORM_PLAIN, 0), (polymult.FORM_MFMA, 0), (polymult.FORM_MFMA, 30)):
         with polymult.Plan(case["N"], case["W"], q, form=form, basis=basis) as plan:
             for source in (polymult.TWIDDLES_TABLE, polymult.TWIDDLES_GEN):
                 plan.set_twiddles(source)
