@@ -290,9 +290,9 @@ void run_phase(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, h
     const pm::DevPlan& P = pl->dev;
     const bool mfma = pl->form == PM_FORM_MFMA;
     switch (i) {
-        case 0: if (mfma) launch_decompose_mfma(P, a, pl->A, st); else launch_decompose_valu(P, a, pl->A, st); break;
+        case 0: if (mfma) launch_decompose_mfma(P, a, pl->A, st, true); else launch_decompose_valu(P, a, pl->A, st); break;
         case 1:   // b scaled by the pointwise constant (b_side)
-            if (mfma) launch_decompose_mfma(b_side(P), b, pl->B, st); else launch_decompose_valu(b_side(P), b, pl->B, st);
+            if (mfma) launch_decompose_mfma(b_side(P), b, pl->B, st, true); else launch_decompose_valu(b_side(P), b, pl->B, st);
             break;
         case 2: launch_ntt_cols(P, pl->A, true, st); break;
         case 3: launch_ntt_cols(P, pl->B, true, st); break;

@@ -115,8 +115,10 @@ void launch_ntt_cols_generic(const DevPlan& P, uint32_t* X, bool forward, hipStr
 void launch_ntt_rows_generic(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st);
 void launch_ntt_rows_fused_generic(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st);
 
-// Matrix-core form (int8 MFMA, exact).
-void launch_decompose_mfma(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st);
+// Matrix-core form (int8 MFMA, exact). lazy_out: residues may be left in
+// [0, 2p) (for the forward NTT of a product when P.lazy is set), else canonical.
+void launch_decompose_mfma(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st,
+                           bool lazy_out = false);
 // Batches of up to kBatchMax products that share N, the primes and L (so
 // every grid and every prime table): one plan P for all, and per product its
 // operands a, b, answer c, residue buffers A, B and the tables that depend on
@@ -147,7 +149,7 @@ void launch_copy16(void* dst, const void* src, size_t bytes, hipStream_t st);
 void launch_crt_mfma(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream_t st);
 // The same over coefficients [k0, k0 + count) only (chunked host pipeline).
 void launch_decompose_mfma_range(const DevPlan& P, const uint32_t* limbs, uint32_t* res, uint32_t k0,
-                                 uint32_t count, hipStream_t st);
+                                 uint32_t count, hipStream_t st, bool lazy_out = false);
 void launch_crt_mfma_range(const DevPlan& P, const uint32_t* Y, uint32_t* c, uint32_t k0, uint32_t count,
                            hipStream_t st);
 

@@ -205,7 +205,7 @@ HostPipeline& pipeline(pm_plan* pl) {
 void decompose_range(pm_plan* pl, const uint32_t* limbs, uint32_t* res, const Span& s, hipStream_t st) {
     const DevPlan P = res == pl->B ? b_side(pl->dev) : pl->dev;
     if (pl->form == PM_FORM_MFMA)
-        launch_decompose_mfma_range(P, limbs, res, s.k0, s.count, st);
+        launch_decompose_mfma_range(P, limbs, res, s.k0, s.count, st, true);
     else
         launch_decompose_valu_range(P, limbs, res, s.k0, s.count, st);
 }

@@ -66,19 +66,20 @@ __device__ __forceinline__ v4i mfma16(const v4i& a, const v4i& b, const v4i& c) 
 // r mod p for an integer-valued double 0 <= r < 2^50: the estimate
 // floor(r / p - 1/2) (the FMA's error is below 2^-30) is floor(r / p) or one
 // less, so the exact FMA remainder lies in [0, 2p), below 2^32; it is converted
-// and brought to [0, p) with one unsigned min.
+// and (kLazy false) brought to [0, p) with one unsigned min.
+template <bool kLazy = false>
 __device__ __forceinline__ uint32_t mod_p_f64(double r, uint32_t p, double pinv) {
     const double qd = floor(fma(r, pinv, -0.5));
     const uint32_t v = static_cast<uint32_t>(fma(-qd, static_cast<double>(p), r));
-    return min(v, v - p);
+    return kLazy ? v : min(v, v - p);
 }
 
 // Decompose, one workgroup per 64 coefficients (4 tiles of 16). Their limbs are
 // copied to LDS with coalesced loads (rows of 16 KT + 4 words, zero beyond L,
 // stored as int8 digits u ^ 0x80), every wave takes its B fragments from there,
 // and the four waves split the prime groups. A fragments of the next group are
-// loaded before the current group's MFMAs.
-template <int KT>
+// loaded before the current group's MFMAs. kLazyOut: residues left in [0, 2p).
+template <int KT, bool kLazyOut>
 __device__ __forceinline__ void decompose_tile(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
                                                const DevPlan& P, uint32_t k0, uint32_t count) {
     extern __shared__ uint32_t xs[];
@@ -199,29 +200,29 @@ __device__ __forceinline__ void decompose_tile(const uint32_t* __restrict__ limb
                 const int lo = acc[t][0] + acc[t][1] * 256;
                 const int hi = acc[t][2] + acc[t][3] * 256;
                 const double r = fma(65536.0, static_cast<double>(hi), static_cast<double>(lo) + corr);
-                const uint32_t v = mod_p_f64(r, p, pinv);
+                const uint32_t v = mod_p_f64<kLazyOut>(r, p, pinv);
                 if (static_cast<uint32_t>(16 * t + cl) < nk) out[16 * t + cl] = v;
             }
         }
     }
 }
 
-template <int KT>
+template <int KT, bool kLazyOut>
 __global__ void __launch_bounds__(64 * kWaves)
 k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res, DevPlan P, uint32_t k0,
                  uint32_t count) {
-    decompose_tile<KT>(limbs, res, P, k0, count);
+    decompose_tile<KT, kLazyOut>(limbs, res, P, k0, count);
 }
 
 // A batch: blockIdx.z = 2 (product) + (0: a into A, 1: b into B).
-template <int KT>
+template <int KT, bool kLazyOut>
 __global__ void __launch_bounds__(64 * kWaves) k_decompose_mfma_batch(DevPlan P, BatchArgs ba) {
     const int i = blockIdx.z >> 1;
     const bool second = (blockIdx.z & 1) != 0;   // b: scaled by the pointwise constant
     DevPlan Q = P;
     Q.dec_digits = second ? P.dec_digits_b : P.dec_digits;
     Q.dec_corr = second ? P.dec_corr_b : P.dec_corr;
-    decompose_tile<KT>(second ? ba.b[i] : ba.a[i], second ? ba.B[i] : ba.A[i], Q, 0, P.N);
+    decompose_tile<KT, kLazyOut>(second ? ba.b[i] : ba.a[i], second ? ba.B[i] : ba.A[i], Q, 0, P.N);
 }
 
 // CRT, one workgroup (four waves) per 64 coefficients, in one kernel:
@@ -784,23 +785,50 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
     d.crt_stiles = ST;
 }
 
-void launch_decompose_mfma(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st) {
-    launch_decompose_mfma_range(P, limbs, res, 0, P.N, st);
+void launch_decompose_mfma(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st, bool lazy_out) {
+    launch_decompose_mfma_range(P, limbs, res, 0, P.N, st, lazy_out);
 }
 
-void launch_decompose_mfma_range(const DevPlan& P, const uint32_t* limbs, uint32_t* res, uint32_t k0,
-                                 uint32_t count, hipStream_t st) {
+namespace {
+
+template <int KT>
+void dec_launch(const DevPlan& P, const uint32_t* limbs, uint32_t* res, uint32_t k0, uint32_t count, hipStream_t st,
+                bool lazy_out, const BatchArgs* ba, int nprod) {
+    const size_t lds = sizeof(uint32_t) * 16 * kTiles * (16 * KT + 4) + 20 * static_cast<size_t>(P.K);
+    const dim3 block(64 * kWaves);
+    if (ba) {
+        const dim3 grid((P.N + 16 * kTiles - 1) / (16 * kTiles), 1, 2 * nprod);
+        if (lazy_out)
+            launch(k_decompose_mfma_batch<KT, true>, grid, block, lds, st, P, *ba);
+        else
+            launch(k_decompose_mfma_batch<KT, false>, grid, block, lds, st, P, *ba);
+    } else {
+        const dim3 grid((count + 16 * kTiles - 1) / (16 * kTiles));
+        if (lazy_out)
+            launch(k_decompose_mfma<KT, true>, grid, block, lds, st, limbs, res, P, k0, count);
+        else
+            launch(k_decompose_mfma<KT, false>, grid, block, lds, st, limbs, res, P, k0, count);
+    }
+}
+
+void dec_dispatch(const DevPlan& P, const uint32_t* limbs, uint32_t* res, uint32_t k0, uint32_t count, hipStream_t st,
+                  bool lazy_out, const BatchArgs* ba = nullptr, int nprod = 0) {
     if (!P.dec_digits) throw std::runtime_error("matrix-core tables not prepared");
-    if (count == 0) return;
-    const dim3 grid((count + 16 * kTiles - 1) / (16 * kTiles));
-    const size_t lds = sizeof(uint32_t) * 16 * kTiles * (16 * P.dec_ksteps + 4) + 20 * static_cast<size_t>(P.K);
     switch (P.dec_ksteps) {
-        case 1: launch(k_decompose_mfma<1>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
-        case 2: launch(k_decompose_mfma<2>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
-        case 3: launch(k_decompose_mfma<3>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
-        case 4: launch(k_decompose_mfma<4>, grid, dim3(64 * kWaves), lds, st, limbs, res, P, k0, count); break;
+        case 1: dec_launch<1>(P, limbs, res, k0, count, st, lazy_out, ba, nprod); return;
+        case 2: dec_launch<2>(P, limbs, res, k0, count, st, lazy_out, ba, nprod); return;
+        case 3: dec_launch<3>(P, limbs, res, k0, count, st, lazy_out, ba, nprod); return;
+        case 4: dec_launch<4>(P, limbs, res, k0, count, st, lazy_out, ba, nprod); return;
         default: throw std::runtime_error("L out of range");
     }
+}
+
+}  // namespace
+
+void launch_decompose_mfma_range(const DevPlan& P, const uint32_t* limbs, uint32_t* res, uint32_t k0,
+                                 uint32_t count, hipStream_t st, bool lazy_out) {
+    if (count == 0) return;
+    dec_dispatch(P, limbs, res, k0, count, st, lazy_out && P.lazy != 0);
 }
 
 void launch_crt_mfma(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream_t st) {
@@ -872,17 +900,10 @@ void launch_crt_mfma_range(const DevPlan& P, const uint32_t* Y, uint32_t* c, uin
     crt_dispatch(P, Y, c, k0, count, st);
 }
 
+// Always a product's decomposition: the forward NTT follows, so the residues
+// may stay in [0, 2p) when the plan's butterflies accept that.
 void launch_decompose_mfma_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st) {
-    if (!P.dec_digits) throw std::runtime_error("matrix-core tables not prepared");
-    const dim3 grid((P.N + 16 * kTiles - 1) / (16 * kTiles), 1, 2 * count);
-    const size_t lds = sizeof(uint32_t) * 16 * kTiles * (16 * P.dec_ksteps + 4) + 20 * static_cast<size_t>(P.K);
-    switch (P.dec_ksteps) {
-        case 1: launch(k_decompose_mfma_batch<1>, grid, dim3(64 * kWaves), lds, st, P, ba); break;
-        case 2: launch(k_decompose_mfma_batch<2>, grid, dim3(64 * kWaves), lds, st, P, ba); break;
-        case 3: launch(k_decompose_mfma_batch<3>, grid, dim3(64 * kWaves), lds, st, P, ba); break;
-        case 4: launch(k_decompose_mfma_batch<4>, grid, dim3(64 * kWaves), lds, st, P, ba); break;
-        default: throw std::runtime_error("L out of range");
-    }
+    dec_dispatch(P, nullptr, nullptr, 0, P.N, st, P.lazy != 0, &ba, count);
 }
 
 void launch_crt_mfma_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st) {

@@ -4,6 +4,7 @@ Every product is compared bit for bit: with the committed sha256 of the exact
 product (tests/golden/golden.json, computed with GMP) and, step by step, with
 the oracle's C restatement of the reference's algorithm on the same inputs.
 """
+import random
 import sys
 
 import numpy as np
@@ -72,6 +73,22 @@ def test_product_matches_gmp(golden, name, form, basis):
 @pytest.mark.parametrize("name", PATTERNS)
 def test_patterns_match_gmp(golden, name, form, basis):
     run_case(golden, name, form, basis)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_fuzz_matches_oracle(seed):
+    """Random points in the spirit of the reference's fuzz runs (fuzz.py): N a
+    random power of two up to 2^14, q a random odd integer of 2 .. 2048 bits,
+    a and b uniform below q; the plain form, the matrix-core form and the
+    matrix-core form in the basis below 2^30 against the oracle."""
+    rng = random.Random(91000 + seed)
+    N, W = 1 << rng.randint(1, 14), rng.randint(2, 2048)
+    q, a, b = oracle.gen_point(91000 + seed, N, W)
+    say("fuzz", seed, "N", N, "W", W)
+    ref = oracle.Plan(N, q).multiply(a, b)
+    for form, basis in ((polymult.FORM_PLAIN, 0), (polymult.FORM_MFMA, 0), (polymult.FORM_MFMA, 30)):
+        with polymult.Plan(N, W, q, form=form, basis=basis) as plan:
+            assert np.array_equal(plan.multiply(a, b), ref), (seed, N, W, form, basis)
 
 
 @pytest.mark.parametrize("N,W", [(64, 100), (256, 1747), (4096, 256), (65536, 2048), (131072, 512)])
