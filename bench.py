@@ -108,7 +108,7 @@ def phase_kernels(plan):
     gen = "true" if plan.twiddles == 1 else "false"
     ks = ((plan.K + 3) // 4 + 1 + 3) // 4
     col = "k_ntt_col_reg<8,%s,5," + lazy + ">"
-    return {"decompose_a": "k_decompose_mfma<4>", "decompose_b": "k_decompose_mfma<4>",
+    return {"decompose_a": "k_decompose_mfma<4,true>", "decompose_b": "k_decompose_mfma<4,true>",
             "ntt_cols_fwd_a": col % "true", "ntt_cols_fwd_b": col % "true",
             "ntt_rows_fused": "k_ntt_rows_fused_reg<8,%s,%s>" % (gen, lazy), "ntt_cols_inv": col % "false",
             "crt": "k_crt_mfma<%d>" % ks}
