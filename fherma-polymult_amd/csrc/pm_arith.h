@@ -68,7 +68,7 @@ PM_HD uint32_t mont_mul_lazy(uint32_t a, uint32_t b, uint32_t p, uint32_t pneg) 
     return hi + mulhi32(m, p) + (lo != 0u);
 }
 
-// The same reduced to [0, p), for a, b < p.
+// The same reduced to [0, p), for a b < 2^32 p (a, b < p, or a < p and b < 2p).
 PM_HD uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t p, uint32_t pneg) {
     return csub(mont_mul_lazy(a, b, p, pneg), p);
 }

@@ -405,7 +405,8 @@ __device__ __forceinline__ uint32_t pointwise_mul(uint32_t a, uint32_t b, const 
 
 // Row pass fused: forward rows of A and B (sharing twiddle loads), the pointwise
 // product with N^-1 (P/p_j)^-1 into A, inverse rows of A. kMode 1: input in
-// [0, 2p), both factors brought to [0, p) for the canonical Montgomery product.
+// [0, 2p); a brought to [0, p) keeps a b below 2 p^2 < 2^32 p, which the
+// canonical Montgomery product needs.
 // kMode 2 (p < 2^30): input in [0, 4p); both factors are brought to [0, 2p),
 // which keeps a b below 2^32 p and the Montgomery product in [0, 2p), the
 // inverse stages' input range; the output is left in [0, 2p).
@@ -460,7 +461,7 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
             a[k] = mont_mul_lazy(csub(a[k], 2 * pc.p), csub(b[k], 2 * pc.p), pc.p, pc.pneg);
     } else if (kMode == 1) {
 #pragma unroll
-        for (uint32_t k = 0; k < 16; ++k) a[k] = pointwise_mul(csub(a[k], pc.p), csub(b[k], pc.p), pc);
+        for (uint32_t k = 0; k < 16; ++k) a[k] = pointwise_mul(csub(a[k], pc.p), b[k], pc);
     } else {
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) a[k] = pointwise_mul(a[k], b[k], pc);
