@@ -183,9 +183,6 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q,
         pc[j].one_s = pm::shoup_companion(1, p);
         pc[j].scale = h.scale[j];
         pc[j].scale_s = h.scale_s[j];
-        const unsigned __int128 mu = (static_cast<unsigned __int128>(1) << 64) / p;
-        pc[j].mu_k = static_cast<uint32_t>(mu >> 32);
-        pc[j].mu_lo = static_cast<uint32_t>(mu);
     }
     pl->tw = twiddles_for(h);
 

@@ -18,8 +18,6 @@ struct PrimeConst {
     uint32_t one_s;   // Shoup companion of 1: floor(2^32 / p)
     uint32_t scale;   // N^-1 (P/p_j)^-1 2^32 mod p
     uint32_t scale_s; // its Shoup companion
-    uint32_t mu_k;    // floor(2^64 / p) = mu_k 2^32 + mu_lo: Shoup companions on the fly
-    uint32_t mu_lo;
 };
 
 struct DevPlan {
@@ -45,7 +43,7 @@ struct DevPlan {
     const uint2* itw;         // K x N: (psi^-bitrev(i), Shoup companion)
     // Row-pass twiddles, generated: the twiddle of row rho, stage distance 2^e,
     // block b is Z[rho][e] * Om[b] (pm_ntt.hip).
-    const uint2* rz;          // K x R x logS: (Z, Shoup companion), forward
+    const uint2* rz;          // K x R x logS: (Z 2^32 mod p, Shoup companion), forward
     const uint2* rzi;         // inverse
     const uint32_t* rom;      // K x S/2: Om, forward
     const uint32_t* romi;     // inverse

@@ -79,27 +79,106 @@ __device__ __forceinline__ void ct_half(uint32_t& x, uint32_t& y, const uint2 w,
     y = xr + M + (p + 1u);
 }
 
+// The same butterflies with a Montgomery twiddle pair (w.x = w' = w 2^32 mod p,
+// canonical, and w.y = w' p^-1 mod 2^32), the form the row pass generates for
+// 7 VALU ops against 13 for a Shoup pair. With m = y w.y mod 2^32, m p and y w'
+// agree in their low words, so (y w' - m p) / 2^32 = hi(y w') - hi(m p) exactly,
+// congruent to y w mod p; for any y < 2^32, y w' < 2^32 p puts both high words
+// in [0, p) and their difference d in (-p, p). For a signed y (the inverse
+// butterflies' x - y) the signed high words give the same identity, with
+// |y w'| < 2^31 p keeping d in (-p, p).
+__device__ __forceinline__ uint32_t mont_tw_diff(uint32_t y, const uint2 w, uint32_t p) {
+    return mulhi32(y, w.x) - mulhi32(y * w.y, p);
+}
+
+__device__ __forceinline__ uint32_t mont_tw_diff_signed(uint32_t d, const uint2 w, uint32_t p) {
+    const int hi_dw = __mulhi(static_cast<int>(d), static_cast<int>(w.x));
+    const int hi_mp = __mulhi(static_cast<int>(d * w.y), static_cast<int>(p));
+    return static_cast<uint32_t>(hi_dw - hi_mp);
+}
+
+// Canonical CT: x, y in [0, p); v = d mod p = min(d, d + p) as unsigned words.
+__device__ __forceinline__ void ct_m(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
+    const uint32_t d = mont_tw_diff(y, w, p);
+    const uint32_t v = min(d, d + p);
+    y = sub_mod(x, v, p);
+    x = add_mod(x, v, p);
+}
+
+// ct_half's form (p < 2^31, x in [0, 2p)): a = h2 + ~h1 = -d - 1, and
+// max(a, a - p) = -v - 1 with v = d mod p (for d >= 0, a >= 2^32 - p and a - p
+// does not wrap; for d < 0, a < p and a - p wraps above it). 10 VALU ops, as
+// ct_half.
+__device__ __forceinline__ void ct_half_m(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
+    const uint32_t h1 = mulhi32(y, w.x), h2 = mulhi32(y * w.y, p);
+    const uint32_t a = h2 + ~h1;
+    const uint32_t M = max(a, a - p);
+    const uint32_t xr = csub(x, p);
+    x = ~M + xr;
+    y = xr + M + (p + 1u);
+}
+
+// ct_lazy's ranges (p < 2^30, x in [0, 4p)): x comes down to [0, 2p), and
+// xr + d + p, xr - d + p lie in (0, 4p).
+__device__ __forceinline__ void ct_lazy_m(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
+    const uint32_t nd = mulhi32(y * w.y, p) - mulhi32(y, w.x);
+    const uint32_t xp = csub(x, p + p) + p;
+    x = xp - nd;
+    y = xp + nd;
+}
+
+// Canonical GS: x, y in [0, p); x - y in (-p, p) as a signed word,
+// |(x - y) w'| < p^2.
+__device__ __forceinline__ void gs_m(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
+    const uint32_t r = mont_tw_diff_signed(x - y, w, p);
+    x = add_mod(x, y, p);
+    y = min(r, r + p);
+}
+
+// gs_lazy's ranges (p < 2^30, x, y in [0, 2p)): x - y in (-2p, 2p),
+// |(x - y) w'| < 2 p^2 < 2^31 p; the product d + p is left in (0, 2p).
+__device__ __forceinline__ void gs_lazy_m(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
+    const uint32_t r = mont_tw_diff_signed(x - y, w, p);
+    x = csub(x + y, p + p);
+    y = r + p;
+}
+
 // Butterfly forms (kMode): 0 canonical, residues in [0, p); 1 forward for any
 // p < 2^31 with residues in [0, 2p) (ct_half), inverse canonical; 2 both
-// lazily reduced for p < 2^30 (ct_lazy, gs_lazy).
-template <bool kFwd, int kMode>
+// lazily reduced for p < 2^30 (ct_lazy, gs_lazy). kMont: the twiddle is a
+// Montgomery pair (generated row twiddles), else a Shoup pair (tables).
+template <bool kFwd, int kMode, bool kMont>
 __device__ __forceinline__ void bfly(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    if (kFwd && kMode == 2)
-        ct_lazy(x, y, w, p);
-    else if (kFwd && kMode == 1)
-        ct_half(x, y, w, p);
-    else if (kFwd)
-        ct(x, y, w, p);
-    else if (kMode == 2)
-        gs_lazy(x, y, w, p);
-    else
-        gs(x, y, w, p);
+    if constexpr (kMont) {
+        if (kFwd && kMode == 2)
+            ct_lazy_m(x, y, w, p);
+        else if (kFwd && kMode == 1)
+            ct_half_m(x, y, w, p);
+        else if (kFwd)
+            ct_m(x, y, w, p);
+        else if (kMode == 2)
+            gs_lazy_m(x, y, w, p);
+        else
+            gs_m(x, y, w, p);
+    } else {
+        if (kFwd && kMode == 2)
+            ct_lazy(x, y, w, p);
+        else if (kFwd && kMode == 1)
+            ct_half(x, y, w, p);
+        else if (kFwd)
+            ct(x, y, w, p);
+        else if (kMode == 2)
+            gs_lazy(x, y, w, p);
+        else
+            gs(x, y, w, p);
+    }
 }
 
 // Twiddle sources: tw(e, b) is the twiddle of block b (counted from the start
 // of the sub-transform) of the stage with distance 2^e.
 // tw.b(e, t, blk): phase B's twiddle for thread t, block blk of its 16 elements.
 struct TableTw {   // the full K x N table: index (B >> (e + 1)) + b
+    static constexpr bool kMont = false;
     const uint2* __restrict__ tab;
     uint32_t B;
     __device__ __forceinline__ uint2 operator()(int e, uint32_t b) const { return tab[(B >> (e + 1)) + b]; }
@@ -108,21 +187,19 @@ struct TableTw {   // the full K x N table: index (B >> (e + 1)) + b
     }
 };
 
-// w = Z * Om mod p and its Shoup companion floor(w 2^32 / p), from
-// floor(2^64 / p) = mu_k 2^32 + mu_lo: the estimate mu_k w + mulhi(w, mu_lo) is
-// exact or one short, and the remainder -q p mod 2^32 tells which.
-__device__ __forceinline__ uint2 make_twiddle(uint32_t om, uint2 z, const PrimeConst& pc) {
-    const uint32_t w = mul_shoup(om, z.x, z.y, pc.p);
-    uint32_t q = pc.mu_k * w + mulhi32(w, pc.mu_lo);
-    q += (0u - q * pc.p) >= pc.p;
-    return make_uint2(w, q);
-}
-
-struct GenTw {     // row rho: Z[rho][e] * Om[b]
+// Row rho's twiddle of stage e, block b, as a Montgomery pair: Z[rho][e] is
+// stored as Z 2^32 mod p with its Shoup companion, so its Shoup product with
+// Om[b] is w' = Z Om 2^32 mod p, canonical, and w' pinv (pinv = p^-1 mod 2^32)
+// completes the pair.
+struct GenTw {
+    static constexpr bool kMont = true;
     const uint2* __restrict__ z;
     const uint32_t* __restrict__ om;
-    PrimeConst pc;
-    __device__ __forceinline__ uint2 operator()(int e, uint32_t b) const { return make_twiddle(om[b], z[e], pc); }
+    uint32_t p, pinv;
+    __device__ __forceinline__ uint2 operator()(int e, uint32_t b) const {
+        const uint32_t w = mul_shoup(om[b], z[e].x, z[e].y, p);
+        return make_uint2(w, w * pinv);
+    }
     __device__ __forceinline__ uint2 b(int e, uint32_t t, int blk) const {
         return (*this)(e, (t << (3 - e)) + blk);
     }
@@ -130,8 +207,9 @@ struct GenTw {     // row rho: Z[rho][e] * Om[b]
 
 // Phase A twiddles of one row, precomputed in LDS: stage e, block b at
 // 2^(m-1-e) - 1 + b (2^(m-4) - 1 entries).
-template <int LOGM>
+template <int LOGM, bool kMontTw>
 struct LdsTwA {
+    static constexpr bool kMont = kMontTw;
     const uint2* tab;
     __device__ __forceinline__ uint2 operator()(int e, uint32_t b) const {
         return tab[(1u << (LOGM - 1 - e)) - 1 + b];
@@ -155,7 +233,7 @@ __device__ __forceinline__ auto row_tw(const DevPlan& P, int j, uint32_t rho, bo
     if constexpr (kGen) {
         const uint32_t R = 1u << P.logR;
         return GenTw{(fwd ? P.rz : P.rzi) + (static_cast<size_t>(j) * R + rho) * LOGM,
-                     (fwd ? P.rom : P.romi) + static_cast<size_t>(j) * ((1u << LOGM) / 2), pc};
+                     (fwd ? P.rom : P.romi) + static_cast<size_t>(j) * ((1u << LOGM) / 2), pc.p, 0u - pc.pneg};
     } else {
         return TableTw{(fwd ? P.tw : P.itw) + static_cast<size_t>(j) * P.N, P.N + (rho << LOGM)};
     }
@@ -176,8 +254,8 @@ __device__ __forceinline__ void phase_a(uint32_t (&x)[16], uint32_t (&y)[16], co
             for (int i = 0; i < NG; ++i) {
 #pragma unroll
                 for (int k = blk * 2 * d; k < blk * 2 * d + d; ++k) {
-                    bfly<kFwd, kMode>(x[i * GA + k], x[i * GA + k + d], w, p);
-                    if (kTwo) bfly<kFwd, kMode>(y[i * GA + k], y[i * GA + k + d], w, p);
+                    bfly<kFwd, kMode, TW::kMont>(x[i * GA + k], x[i * GA + k + d], w, p);
+                    if (kTwo) bfly<kFwd, kMode, TW::kMont>(y[i * GA + k], y[i * GA + k + d], w, p);
                 }
             }
         }
@@ -196,8 +274,8 @@ __device__ __forceinline__ void phase_b(uint32_t (&x)[16], uint32_t (&y)[16], co
             const uint2 w = tw.b(e, t, blk);
 #pragma unroll
             for (int k = blk * 2 * d; k < blk * 2 * d + d; ++k) {
-                bfly<kFwd, kMode>(x[k], x[k + d], w, p);
-                if (kTwo) bfly<kFwd, kMode>(y[k], y[k + d], w, p);
+                bfly<kFwd, kMode, TW::kMont>(x[k], x[k + d], w, p);
+                if (kTwo) bfly<kFwd, kMode, TW::kMont>(y[k], y[k + d], w, p);
             }
         }
     }
@@ -238,6 +316,7 @@ struct Buf {
 // splits into a lane part (t << (3 - e), four values fixed per thread) and a
 // uniform part.
 struct BufTw {
+    static constexpr bool kMont = false;
     const uint2* __restrict__ tab;
     Buf buf;
     uint32_t B, vt[4];
@@ -346,7 +425,7 @@ k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
     uint32_t* l = lds + rl * ROW;
     uint2* twa = reinterpret_cast<uint2*>(lds + G * ROW) + rl * GA;
     fill_row_phase_a<LOGM>(twa, gen, t);
-    const LdsTwA<LOGM> ta{twa};
+    const LdsTwA<LOGM, decltype(gen)::kMont> ta{twa};
     uint32_t r[16];
     if (kFwd) {
 #pragma unroll
@@ -428,7 +507,7 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
     uint2* twa = reinterpret_cast<uint2*>(lds + G * ROW) + rl * 2 * GA;
     fill_row_phase_a<LOGM>(twa, fw, t);
     fill_row_phase_a<LOGM>(twa + GA, iw, t);
-    const LdsTwA<LOGM> fa{twa}, ia{twa + GA};
+    const LdsTwA<LOGM, decltype(fw)::kMont> fa{twa}, ia{twa + GA};
     uint32_t a[16], b[16];
 #pragma unroll
     for (uint32_t i = 0; i < NG; ++i)

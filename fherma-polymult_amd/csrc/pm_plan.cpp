@@ -275,7 +275,9 @@ void build_row_twiddles(const HostPlan& pl, bool forward, int j0, int j1, uint32
         for (uint32_t rho = 0; rho < R; ++rho) {
             const uint64_t base = 2ull * (pl.logR ? bitrev32(rho, pl.logR) : 0) + 1;
             for (uint32_t e = 0; e < pl.logS; ++e) {
-                const uint32_t w = pow_mod(r, (base << e) % (2ull * pl.N), p);
+                // Montgomery form: the kernels' product with Om is then a Montgomery twiddle
+                const uint32_t w = static_cast<uint32_t>(
+                    (static_cast<uint64_t>(pow_mod(r, (base << e) % (2ull * pl.N), p)) << 32) % p);
                 z[2 * (rho * pl.logS + e)] = w;
                 z[2 * (rho * pl.logS + e) + 1] = shoup_companion(w, p);
             }

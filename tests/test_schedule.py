@@ -159,18 +159,3 @@ def test_register_schedule_matches_oracle_ntt(N):
         itw = np.array([pow(ipsi, e, p) for e in brv], dtype=object)
         assert emulate(res[j], tw, itw, N, p, True, psi) == fwd[j].tolist()
         assert emulate(fwd[j], tw, itw, N, p, False, psi) == inv[j].tolist()
-
-
-def test_shoup_companion_on_the_fly():
-    """make_twiddle's companion: mu_k w + mulhi(w, mu_lo), plus one if the
-    remainder -q p mod 2^32 is at least p, equals floor(w 2^32 / p)."""
-    import random
-    rng = random.Random(1)
-    for p in list(oracle.choose_primes(65536, 2048)[:8]) + list(oracle.choose_primes(4096, 256)[-4:]):
-        p = int(p)
-        mu = (1 << 64) // p
-        k, lo = mu >> 32, mu & 0xFFFFFFFF
-        for w in [1, 2, p - 1, p - 2] + [rng.randrange(1, p) for _ in range(20000)]:
-            q = (k * w + ((w * lo) >> 32)) & 0xFFFFFFFF
-            q += 1 if ((-q * p) & 0xFFFFFFFF) >= p else 0
-            assert q == (w << 32) // p
