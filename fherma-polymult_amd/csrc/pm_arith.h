@@ -73,6 +73,39 @@ PM_HD uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t p, uint32_t pneg) {
     return csub(mont_mul_lazy(a, b, p, pneg), p);
 }
 
+// High word of the signed 64-bit product.
+PM_HD int32_t mulhi_i32(int32_t a, int32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __mulhi(a, b);
+#else
+    return static_cast<int32_t>((static_cast<int64_t>(a) * b) >> 32);
+#endif
+}
+
+// Twiddle products with a Montgomery pair (wm = w 2^32 mod p, canonical, and
+// wq = wm p^-1 mod 2^32). With m = y wq mod 2^32, m p and y wm agree in their
+// low words, so (y wm - m p) / 2^32 = hi(y wm) - hi(m p) exactly, congruent to
+// y w mod p. For any y < 2^32, y wm < 2^32 p puts both high words in [0, p) and
+// the result, as a signed word, in (-p, p).
+PM_HD uint32_t mont_tw_diff(uint32_t y, uint32_t wm, uint32_t wq, uint32_t p) {
+    return mulhi32(y, wm) - mulhi32(y * wq, p);
+}
+
+// The same for a signed d (the inverse butterflies' x - y) with |d wm| < 2^31 p,
+// from the signed high words: again in (-p, p).
+PM_HD uint32_t mont_tw_diff_signed(uint32_t d, uint32_t wm, uint32_t wq, uint32_t p) {
+    const int32_t hi_dw = mulhi_i32(static_cast<int32_t>(d), static_cast<int32_t>(wm));
+    const int32_t hi_mp = mulhi_i32(static_cast<int32_t>(d * wq), static_cast<int32_t>(p));
+    return static_cast<uint32_t>(hi_dw - hi_mp);
+}
+
+// r in (-p, p) as a signed word, brought to [0, p): r + p falls below r (it
+// wraps) exactly when r is negative.
+PM_HD uint32_t canon_signed(uint32_t r, uint32_t p) {
+    const uint32_t s = r + p;
+    return s < r ? s : r;
+}
+
 PM_HD uint32_t shoup_companion(uint32_t w, uint32_t p) {
     return static_cast<uint32_t>((static_cast<uint64_t>(w) << 32) / p);
 }

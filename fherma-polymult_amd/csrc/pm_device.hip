@@ -118,8 +118,9 @@ __device__ __forceinline__ void tile_stages(uint32_t* tile, const uint2* __restr
                 tile[lo] = add_mod(x, v, p);
                 tile[hi] = sub_mod(x, v, p);
             } else {
+                // inverse tables hold Montgomery pairs; x - y in (-p, p) as a signed word
                 tile[lo] = add_mod(x, y, p);
-                tile[hi] = mul_shoup(x - y + p, w.x, w.y, p);   // (0, 2p): no reduction needed first
+                tile[hi] = canon_signed(mont_tw_diff_signed(x - y, w.x, w.y, p), p);
             }
         }
         __syncthreads();

@@ -80,27 +80,14 @@ __device__ __forceinline__ void ct_half(uint32_t& x, uint32_t& y, const uint2 w,
 }
 
 // The same butterflies with a Montgomery twiddle pair (w.x = w' = w 2^32 mod p,
-// canonical, and w.y = w' p^-1 mod 2^32), the form the row pass generates for
-// 7 VALU ops against 13 for a Shoup pair. With m = y w.y mod 2^32, m p and y w'
-// agree in their low words, so (y w' - m p) / 2^32 = hi(y w') - hi(m p) exactly,
-// congruent to y w mod p; for any y < 2^32, y w' < 2^32 p puts both high words
-// in [0, p) and their difference d in (-p, p). For a signed y (the inverse
-// butterflies' x - y) the signed high words give the same identity, with
-// |y w'| < 2^31 p keeping d in (-p, p).
-__device__ __forceinline__ uint32_t mont_tw_diff(uint32_t y, const uint2 w, uint32_t p) {
-    return mulhi32(y, w.x) - mulhi32(y * w.y, p);
-}
+// canonical, and w.y = w' p^-1 mod 2^32; mont_tw_diff in pm_arith.h): the
+// inverse tables hold this form, which saves one VALU op per inverse
+// butterfly, and the row pass generates it for 7 VALU ops against 13 for a
+// Shoup pair. The product's difference of high words d lies in (-p, p).
 
-__device__ __forceinline__ uint32_t mont_tw_diff_signed(uint32_t d, const uint2 w, uint32_t p) {
-    const int hi_dw = __mulhi(static_cast<int>(d), static_cast<int>(w.x));
-    const int hi_mp = __mulhi(static_cast<int>(d * w.y), static_cast<int>(p));
-    return static_cast<uint32_t>(hi_dw - hi_mp);
-}
-
-// Canonical CT: x, y in [0, p); v = d mod p = min(d, d + p) as unsigned words.
+// Canonical CT: x, y in [0, p); v = d mod p.
 __device__ __forceinline__ void ct_m(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    const uint32_t d = mont_tw_diff(y, w, p);
-    const uint32_t v = min(d, d + p);
+    const uint32_t v = canon_signed(mont_tw_diff(y, w.x, w.y, p), p);
     y = sub_mod(x, v, p);
     x = add_mod(x, v, p);
 }
@@ -130,15 +117,15 @@ __device__ __forceinline__ void ct_lazy_m(uint32_t& x, uint32_t& y, const uint2 
 // Canonical GS: x, y in [0, p); x - y in (-p, p) as a signed word,
 // |(x - y) w'| < p^2.
 __device__ __forceinline__ void gs_m(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    const uint32_t r = mont_tw_diff_signed(x - y, w, p);
+    const uint32_t r = mont_tw_diff_signed(x - y, w.x, w.y, p);
     x = add_mod(x, y, p);
-    y = min(r, r + p);
+    y = canon_signed(r, p);
 }
 
 // gs_lazy's ranges (p < 2^30, x, y in [0, 2p)): x - y in (-2p, 2p),
 // |(x - y) w'| < 2 p^2 < 2^31 p; the product d + p is left in (0, 2p).
 __device__ __forceinline__ void gs_lazy_m(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    const uint32_t r = mont_tw_diff_signed(x - y, w, p);
+    const uint32_t r = mont_tw_diff_signed(x - y, w.x, w.y, p);
     x = csub(x + y, p + p);
     y = r + p;
 }
@@ -177,8 +164,10 @@ __device__ __forceinline__ void bfly(uint32_t& x, uint32_t& y, const uint2 w, ui
 // Twiddle sources: tw(e, b) is the twiddle of block b (counted from the start
 // of the sub-transform) of the stage with distance 2^e.
 // tw.b(e, t, blk): phase B's twiddle for thread t, block blk of its 16 elements.
+// The full K x N tables hold Shoup pairs (forward) or Montgomery pairs (inverse).
+template <bool kMontTw>
 struct TableTw {   // the full K x N table: index (B >> (e + 1)) + b
-    static constexpr bool kMont = false;
+    static constexpr bool kMont = kMontTw;
     const uint2* __restrict__ tab;
     uint32_t B;
     __device__ __forceinline__ uint2 operator()(int e, uint32_t b) const { return tab[(B >> (e + 1)) + b]; }
@@ -228,14 +217,14 @@ __device__ __forceinline__ void fill_row_phase_a(uint2* tab, const TW& g, uint32
 // The row pass's twiddles for row rho of prime j: generated from the per-row
 // and per-prime factors (kGen), or read from the full table, whose index for
 // row rho is ((N + rho S) >> (e + 1)) + b.
-template <int LOGM, bool kGen>
-__device__ __forceinline__ auto row_tw(const DevPlan& P, int j, uint32_t rho, bool fwd, const PrimeConst& pc) {
+template <int LOGM, bool kGen, bool fwd>
+__device__ __forceinline__ auto row_tw(const DevPlan& P, int j, uint32_t rho, const PrimeConst& pc) {
     if constexpr (kGen) {
         const uint32_t R = 1u << P.logR;
         return GenTw{(fwd ? P.rz : P.rzi) + (static_cast<size_t>(j) * R + rho) * LOGM,
                      (fwd ? P.rom : P.romi) + static_cast<size_t>(j) * ((1u << LOGM) / 2), pc.p, 0u - pc.pneg};
     } else {
-        return TableTw{(fwd ? P.tw : P.itw) + static_cast<size_t>(j) * P.N, P.N + (rho << LOGM)};
+        return TableTw<!fwd>{(fwd ? P.tw : P.itw) + static_cast<size_t>(j) * P.N, P.N + (rho << LOGM)};
     }
 }
 
@@ -315,8 +304,9 @@ struct Buf {
 // wave-uniform (scalar loads), phase B's index (B >> (e + 1)) + (t << (3 - e)) + blk
 // splits into a lane part (t << (3 - e), four values fixed per thread) and a
 // uniform part.
+template <bool kMontTw>
 struct BufTw {
-    static constexpr bool kMont = false;
+    static constexpr bool kMont = kMontTw;
     const uint2* __restrict__ tab;
     Buf buf;
     uint32_t B, vt[4];
@@ -349,7 +339,7 @@ __device__ __forceinline__ void col_pass(uint32_t* __restrict__ X, const DevPlan
     const Buf x(X + static_cast<size_t>(j) * P.N, 4u * P.N);
     const uint32_t col = (blockIdx.x << LOGC) + c;
     const uint32_t va = 4u * (t * S + col), vb = 4u * (16u * t * S + col);   // phase A / phase B rows
-    const BufTw tw((kFwd ? P.tw : P.itw) + static_cast<size_t>(j) * P.N, P.N, M, t);
+    const BufTw<!kFwd> tw((kFwd ? P.tw : P.itw) + static_cast<size_t>(j) * P.N, P.N, M, t);
     uint32_t* la = lds + t * C + c;          // phase A slots: + (TPS i + 17 k) C
     uint32_t* lb = lds + 17u * t * C + c;    // phase B slots: + k C
     const uint32_t p = P.pc[j].p;
@@ -421,7 +411,7 @@ k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
     const int j = blockIdx.y;
     uint32_t* x = X + static_cast<size_t>(j) * P.N + static_cast<size_t>(rho) * M;
     const PrimeConst pc = P.pc[j];
-    const auto gen = row_tw<LOGM, kGen>(P, j, rho, kFwd, pc);
+    const auto gen = row_tw<LOGM, kGen, kFwd>(P, j, rho, pc);
     uint32_t* l = lds + rl * ROW;
     uint2* twa = reinterpret_cast<uint2*>(lds + G * ROW) + rl * GA;
     fill_row_phase_a<LOGM>(twa, gen, t);
@@ -500,14 +490,15 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
     const int j = blockIdx.y;
     const size_t row = static_cast<size_t>(j) * P.N + static_cast<size_t>(rho) * M;
     const PrimeConst pc = P.pc[j];
-    const auto fw = row_tw<LOGM, kGen>(P, j, rho, true, pc);
-    const auto iw = row_tw<LOGM, kGen>(P, j, rho, false, pc);
+    const auto fw = row_tw<LOGM, kGen, true>(P, j, rho, pc);
+    const auto iw = row_tw<LOGM, kGen, false>(P, j, rho, pc);
     // one exchange buffer per row, used by a and then b
     uint32_t* la = lds + rl * ROW;
     uint2* twa = reinterpret_cast<uint2*>(lds + G * ROW) + rl * 2 * GA;
     fill_row_phase_a<LOGM>(twa, fw, t);
     fill_row_phase_a<LOGM>(twa + GA, iw, t);
-    const LdsTwA<LOGM, decltype(fw)::kMont> fa{twa}, ia{twa + GA};
+    const LdsTwA<LOGM, decltype(fw)::kMont> fa{twa};
+    const LdsTwA<LOGM, decltype(iw)::kMont> ia{twa + GA};
     uint32_t a[16], b[16];
 #pragma unroll
     for (uint32_t i = 0; i < NG; ++i)

@@ -258,10 +258,17 @@ void build_twiddles(const HostPlan& pl, bool forward, int j0, int j1, uint32_t* 
             x = x * r % p;
         }
         uint32_t* o = out + 2 * static_cast<size_t>(j - j0) * N;
+        const uint32_t pinv = 0u - pl.pneg[j];
         for (uint32_t i = 0; i < N; ++i) {
             const uint32_t w = pw[pl.logN ? bitrev32(i, pl.logN) : 0];
-            o[2 * i] = w;
-            o[2 * i + 1] = shoup_companion(w, p);
+            if (forward) {
+                o[2 * i] = w;
+                o[2 * i + 1] = shoup_companion(w, p);
+            } else {
+                const uint32_t wm = static_cast<uint32_t>((static_cast<uint64_t>(w) << 32) % p);
+                o[2 * i] = wm;
+                o[2 * i + 1] = wm * pinv;
+            }
         }
     }
 }

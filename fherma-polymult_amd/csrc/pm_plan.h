@@ -59,8 +59,10 @@ void ntt_geometry(uint32_t logN, uint32_t& logS, uint32_t& logR);
 // The prime walk alone: K primes for (N, bits(q)) below 2^pbits (30 or 31), or an exception.
 std::vector<uint32_t> choose_primes(uint32_t N, int qbits, int pbits = 31);
 
-// Twiddles psi^bitrev_N(i) (forward) or psi^-bitrev_N(i) (inverse), i < N,
-// as (w, floor(w 2^32 / p)) pairs, for primes [j0, j1) of the plan.
+// Twiddles w = psi^bitrev_N(i) (forward) or psi^-bitrev_N(i) (inverse), i < N,
+// for primes [j0, j1) of the plan: forward as Shoup pairs (w, floor(w 2^32 / p)),
+// inverse as Montgomery pairs (w 2^32 mod p, that times p^-1 mod 2^32), which
+// the inverse butterflies take for one VALU op less.
 void build_twiddles(const HostPlan& pl, bool forward, int j0, int j1, uint32_t* out_pairs);
 
 // Row-pass twiddle factors for primes [j0, j1): the twiddle psi^bitrev_N(idx)
