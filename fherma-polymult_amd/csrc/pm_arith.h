@@ -1,11 +1,10 @@
 // Word arithmetic modulo a prime p < 2^31, shared by the host plan and the
 // gfx950 kernels.
 //
-// With the reference's primes (just below 2^31) every residue is kept canonical
-// in [0, p): there is no headroom in a 32-bit word for lazily reduced sums
-// (2p + 2p > 2^32), so each butterfly reduces its sum and difference with one
-// unsigned min each. Primes below 2^30 leave that headroom; the NTT kernels'
-// lazily reduced butterflies for them are in pm_ntt.hip.
+// With the reference's primes (just below 2^31) a 32-bit word holds residues
+// up to 2p but not 4p, so sums are reduced with one unsigned min each; the NTT
+// kernels' butterfly forms built on these (residues in [0, 2p) forward, and
+// lazily reduced ones for primes below 2^30) are in pm_ntt.hip.
 //
 // Reference counterparts: rns.cuh:46-93 (mulhi, add_mod, sub_mod, mul_shoup,
 // mont_mul, bitrev_bits).
