@@ -56,6 +56,13 @@ def make_point(seed, n, w):
     q[L - 1] &= mask
     q[L - 1] |= np.uint32(1 << ((w - 1) % 32))
     q[0] |= np.uint32(1)
+    return make_operands(seed, n, w, q)
+
+
+def make_operands(seed, n, w, q):
+    """(q, a, b): a and b uniform in [0, q) from the seed's streams 1 and 2."""
+    L = (w + 31) // 32
+    mask = np.uint32((1 << (w % 32)) - 1 if w % 32 else 0xFFFFFFFF)
 
     def operand(stream):
         key = _mix_int(seed * G + stream * H)
@@ -234,6 +241,101 @@ def p30_rate(args, points, da, db, dc, gold, log):
     return out
 
 
+def uneven_rate(args, log):
+    """The 64-product batch of uneven shapes (N in 4096 .. 65536, q of 128 .. 2019
+    bits, seeds 4000 .. 4063 as in the golden fixtures), device-resident, replayed
+    as one hipGraph: products/s and mismatches against the GMP digests. A second
+    workload beside the headline, never the headline."""
+    import polymult  # noqa: E402
+    path = os.path.join(ROOT, "tests", "golden", "golden.json")
+    with open(path) as f:
+        cases = sorted((c for c in json.load(f)["cases"] if c["name"].startswith("config-uneven-")),
+                       key=lambda c: c["name"])
+    points = [make_point(c["seed"], c["N"], c["W"]) for c in cases]
+    plans = [polymult.Plan(c["N"], c["W"], q, form=polymult.FORM_MFMA) for c, (q, _, _) in zip(cases, points)]
+    da = [polymult.DeviceBuffer.from_array(a) for _, a, _ in points]
+    db = [polymult.DeviceBuffer.from_array(b) for _, _, b in points]
+    dc = [polymult.DeviceBuffer(a.nbytes) for _, a, _ in points]
+    graph = polymult.Batch(plans, da, db, dc)
+    graph.launch()
+    polymult.synchronize()
+    mismatches = sum(digest(x.download((c["N"], c["L"]))) != c["c_sha"] for c, x in zip(cases, dc))
+    for _ in range(args.warmup):
+        graph.launch()
+    polymult.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.launch()
+    polymult.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    out = {"value": round(len(cases) / sec, 2), "unit": "polymuls/s", "products": len(cases),
+           "ms_per_batch": round(sec * 1e3, 4), "mismatches": int(mismatches),
+           "note": "64 products of uneven N (4096-65536) and q width (128-2019 bits), one hipGraph, device-resident"}
+    log("uneven batch: %.3f ms per batch of %d, %.1f products/s, %d mismatches"
+        % (sec * 1e3, len(cases), len(cases) / sec, mismatches))
+    graph.close()
+    for x in da + db + dc:
+        x.close()
+    for p in plans:
+        p.close()
+    return out
+
+
+# The reference's two measured points (its working notes, on an NVIDIA L40S):
+# N=65536 with the spec's 1747-bit prime q, and N=32768 with its 868-bit prime q;
+# end-to-end from host vectors, one product per call.
+REFERENCE_POINTS = [("point1", "matrix-n65536-w1747-prime", 2.30), ("point2", "matrix-n32768-w868-prime", 0.605)]
+
+
+def reference_points(log, reps=11):
+    """The reference's measured points with their golden q: the median end-to-end
+    time of one product from host numpy arrays (fherma_run's path, PCIe included)
+    and the mean device-resident time of one product (no batching), each product
+    checked against its GMP digest."""
+    import polymult  # noqa: E402
+    path = os.path.join(ROOT, "tests", "golden", "golden.json")
+    with open(path) as f:
+        cases = {c["name"]: c for c in json.load(f)["cases"]}
+    out = {}
+    for key, name, ref_ms in REFERENCE_POINTS:
+        c = cases[name]
+        n, w, L = c["N"], c["W"], c["L"]
+        qi = int(c["q_hex"], 16)
+        q = np.array([(qi >> (32 * l)) & 0xFFFFFFFF for l in range(L)], dtype=np.uint32)
+        _, a, b = make_operands(c["seed"], n, w, q)
+        plan = polymult.Plan(n, w, q, form=polymult.FORM_MFMA)
+        mode, _ = plan.tune_host()
+        mism = int(digest(plan.multiply(a, b)) != c["c_sha"])
+        host = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            plan.multiply(a, b)
+            host.append(time.perf_counter() - t0)
+        da, db = polymult.DeviceBuffer.from_array(a), polymult.DeviceBuffer.from_array(b)
+        dc = polymult.DeviceBuffer(a.nbytes)
+        plan.multiply_device(da, db, dc)
+        polymult.synchronize()
+        mism += int(digest(dc.download((n, L))) != c["c_sha"])
+        iters = 20
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            plan.multiply_device(da, db, dc)
+        polymult.synchronize()
+        dev = (time.perf_counter() - t0) / iters
+        out[key] = {"N": n, "W": w, "K": plan.K, "q": "the spec's prime", "mismatches": mism,
+                    "end_to_end_ms": round(float(np.median(host)) * 1e3, 4),
+                    "host_mode": {polymult.HOST_STAGED: "staged", polymult.HOST_MAPPED: "mapped",
+                                  polymult.HOST_DIRECT: "direct"}.get(mode, str(mode)),
+                    "device_resident_ms": round(dev * 1e3, 4),
+                    "reference_end_to_end_ms_l40s": ref_ms}
+        log("%s (N=%d, W=%d): end to end %.3f ms (%s), device-resident %.3f ms, %d mismatches"
+            % (key, n, w, out[key]["end_to_end_ms"], out[key]["host_mode"], dev * 1e3, mism))
+        for x in (da, db, dc):
+            x.close()
+        plan.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,6 +345,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-p30", action="store_true", help="skip the opt-in prime basis below 2^30")
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (PCIe-inclusive) rate")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the 64-product uneven batch")
+    ap.add_argument("--no-points", action="store_true", help="skip the reference's two measured points")
     ap.add_argument("--phase-iters", type=int, default=20,
                     help="back-to-back launches of each phase's kernel in the per-phase timing")
     ap.add_argument("--phases-only", action="store_true",
@@ -291,6 +395,12 @@ def main():
     p30 = None
     if not (args.phases_only or args.no_p30):
         p30 = p30_rate(args, points, da, db, dc, gold, log)
+    uneven = None
+    if not (args.phases_only or args.no_uneven):
+        uneven = uneven_rate(args, log)
+    ref_points = None
+    if not (args.phases_only or args.no_points):
+        ref_points = reference_points(log)
 
     # Host buffers in and out (fherma_run's path: PCIe transfers included), after
     # the timed choice of the host mode on the first plan. Never the headline value.
@@ -345,6 +455,8 @@ def main():
         "phases_ms": {k: round(v, 5) for k, v in phases.items()},
         "host_inclusive": host,
         "p30_basis": p30,
+        "uneven_batch": uneven,
+        "reference_points": ref_points,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
