@@ -11,6 +11,8 @@
 //   mapped  the same chunks, but the decompose reads the pinned buffer through
 //           its device mapping and the CRT writes the answer there: no DMA.
 //   direct  hipMemcpyAsync from and to the caller's pageable memory.
+// The timed choice covers the mode and, for staged and mapped, the number of
+// chunks per operand (1, 2 or 4; PM_HOST_CHUNKS fixes it).
 // XNACK-based access to pageable memory (the reference's hmm mode) is not used:
 // it needs xnack+ code objects and HSA_XNACK=1 on the machine.
 #include "../../include/fherma_polymult.h"
@@ -20,6 +22,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
@@ -146,8 +149,13 @@ std::vector<Span> spans(uint32_t N, int n) {
 
 }  // namespace
 
+constexpr int kMaxChunks = 16;
+
 struct HostPipeline {
     int mode = PM_HOST_STAGED;
+    int chunks = 2;             // coefficient chunks per operand (staged and mapped)
+    bool chunks_fixed = false;  // PM_HOST_CHUNKS given: the timed choice leaves it alone
+    uint32_t N = 0;
     std::vector<Span> in, out;
     size_t words = 0;
     uint32_t *h_a = nullptr, *h_b = nullptr, *h_c = nullptr;   // pinned, mapped
@@ -170,6 +178,12 @@ struct HostPipeline {
 
 void destroy_pipeline(HostPipeline* hp) { delete hp; }
 
+static void set_chunks(HostPipeline& hp, int n) {
+    hp.chunks = std::max(1, std::min(n, kMaxChunks));
+    hp.in = spans(hp.N, hp.chunks);
+    hp.out = spans(hp.N, hp.chunks);
+}
+
 namespace {
 
 HostPipeline& pipeline(pm_plan* pl) {
@@ -178,9 +192,9 @@ HostPipeline& pipeline(pm_plan* pl) {
     hp->words = pl->words();
     const size_t bytes = hp->words * sizeof(uint32_t);
     const char* env = std::getenv("PM_HOST_CHUNKS");
-    const int n = env ? std::max(1, std::atoi(env)) : 4;
-    hp->in = spans(pl->host.N, n);
-    hp->out = spans(pl->host.N, n);
+    hp->N = pl->host.N;
+    hp->chunks_fixed = env != nullptr;
+    set_chunks(*hp, env ? std::atoi(env) : 2);
     for (auto pr : {std::make_pair(&hp->h_a, &hp->m_a), std::make_pair(&hp->h_b, &hp->m_b),
                     std::make_pair(&hp->h_c, &hp->m_c)}) {
         check(hipHostMalloc(reinterpret_cast<void**>(pr.first), bytes, hipHostMallocMapped), "hipHostMalloc");
@@ -192,10 +206,10 @@ HostPipeline& pipeline(pm_plan* pl) {
         v.resize(n);
         for (auto& e : v) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
     };
-    events(hp->ev_a, hp->in.size());
-    events(hp->ev_b, hp->in.size());
-    events(hp->ev_crt, hp->out.size());
-    events(hp->ev_out, hp->out.size());
+    events(hp->ev_a, kMaxChunks);
+    events(hp->ev_b, kMaxChunks);
+    events(hp->ev_crt, kMaxChunks);
+    events(hp->ev_out, kMaxChunks);
     hp->pool.reset(new CopyPool(copy_workers()));
     pl->pipe = hp.release();
     return *pl->pipe;
@@ -303,12 +317,22 @@ double host_autotune(pm_plan* pl) {
         a[i] = static_cast<uint32_t>(i * 2654435761u) >> 1;
         b[i] = static_cast<uint32_t>(i * 2246822519u) >> 1;
     }
-    const int modes[3] = {PM_HOST_STAGED, PM_HOST_MAPPED, PM_HOST_DIRECT};
+    // (mode, chunks) in timing order; the first, staged in the default chunk
+    // count, stands unless another is clearly faster. The chunk count matters
+    // only to staged and mapped; PM_HOST_CHUNKS fixes it.
+    const int chunk0 = hp.chunks;
+    std::vector<std::pair<int, int>> cands{{PM_HOST_STAGED, chunk0}};
+    for (int m : {PM_HOST_STAGED, PM_HOST_MAPPED})
+        for (int n : {1, 2, 4})
+            if (!hp.chunks_fixed && !(m == PM_HOST_STAGED && n == chunk0)) cands.emplace_back(m, n);
+    if (hp.chunks_fixed) cands.emplace_back(PM_HOST_MAPPED, chunk0);
+    cands.emplace_back(PM_HOST_DIRECT, chunk0);
     double best = 1e30;
-    int pick = PM_HOST_STAGED;
-    for (int m : modes) {
-        hp.mode = m;
-        host_multiply(pl, a.data(), b.data(), c.data());   // first use of the mode, untimed
+    std::pair<int, int> pick = cands.front();
+    for (size_t i = 0; i < cands.size(); ++i) {
+        hp.mode = cands[i].first;
+        set_chunks(hp, cands[i].second);
+        host_multiply(pl, a.data(), b.data(), c.data());   // first use of the setting, untimed
         std::vector<double> t;
         for (int r = 0; r < 5; ++r) {
             const auto t0 = std::chrono::steady_clock::now();
@@ -317,13 +341,15 @@ double host_autotune(pm_plan* pl) {
         }
         std::sort(t.begin(), t.end());
         const double med = t[2];
-        // staged (the first) stands unless another is clearly faster
-        if (m == PM_HOST_STAGED ? med < best : med < 0.95 * best) {
+        if (std::getenv("PM_HOST_TUNE_LOG"))
+            std::fprintf(stderr, "host mode %d, %d chunks: %.3f ms (median of 5)\n", cands[i].first, cands[i].second, med);
+        if (i == 0 ? med < best : med < 0.95 * best) {
             best = med;
-            pick = m;
+            pick = cands[i];
         }
     }
-    hp.mode = pick;
+    hp.mode = pick.first;
+    set_chunks(hp, pick.second);
     return best;
 }
 

@@ -39,6 +39,23 @@ def test_host_modes_match_gmp(golden, name, mode, form):
             assert oracle.digest(c) == case["c_sha"], (name, mode)
 
 
+@pytest.mark.parametrize("chunks", [1, 3, 16])
+@pytest.mark.parametrize("mode", [0, 1], ids=["staged", "mapped"])
+def test_host_chunk_counts_match_gmp(golden, chunks, mode, monkeypatch):
+    """PM_HOST_CHUNKS fixes the coefficient chunks per operand; the timed choice
+    then keeps that count and picks only the mode."""
+    case = golden["config-n32768-w868-s1"]
+    q, a, b = pointgen.operands(case)
+    monkeypatch.setenv("PM_HOST_CHUNKS", str(chunks))
+    say("host chunks", chunks, "mode", mode)
+    with polymult.Plan(case["N"], case["W"], q) as plan:
+        plan.set_host_mode(mode)
+        assert oracle.digest(plan.multiply(a, b)) == case["c_sha"]
+        tuned, ms = plan.tune_host()
+        assert tuned in (0, 1, 2) and ms > 0
+        assert oracle.digest(plan.multiply(a, b)) == case["c_sha"]
+
+
 def test_timed_mode_choice(golden):
     case = golden["config-n32768-w868-s2"]
     q, a, b = pointgen.operands(case)
