@@ -137,6 +137,34 @@ def measured_traffic(kernel):
     return None
 
 
+# Issue rate of wave64 VALU instructions per SIMD on gfx950 (integer add, xor,
+# v_mul_lo_u32 and v_mul_hi_u32 alike, measured with a probe kernel), and the
+# MI355X peak engine clock.
+VALU_CYCLES_PER_INST = 4.4
+CLOCK_GHZ = 2.4
+
+
+def valu_issue(kernel, kernel_ms, cus):
+    """The VALU issue time of one launch of a kernel from the committed SQ
+    counter table (profiles/sq_counters_n65536_w2048.txt: waves and VALU
+    instructions per wave), against its measured duration; None when absent."""
+    path = os.path.join(ROOT, "profiles", "sq_counters_n65536_w2048.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 3 and parts[0] == kernel:
+                waves, per_wave = int(parts[1]), float(parts[2])
+                simds = 4 * cus
+                ms = per_wave * waves * VALU_CYCLES_PER_INST / simds / (CLOCK_GHZ * 1e9) * 1e3
+                return {"instructions_per_wave": per_wave, "waves": waves, "cycles_per_instruction": VALU_CYCLES_PER_INST,
+                        "clock_ghz": CLOCK_GHZ, "simds": simds, "issue_ms": round(ms, 5),
+                        "frac": round(ms / kernel_ms, 4),
+                        "note": "integer VALU issue is this kernel's bound; frac = issue time / measured time"}
+    return None
+
+
 def cpu_baseline(points, budget_s=12.0):
     """The reference's CPU path (oracle/_ref) on the box's cores, else the oracle port."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -439,6 +467,7 @@ def main():
     prod_bytes = 12 * N * L + 56 * K * N
     cpu = None if args.no_cpu else cpu_baseline(points[:1])
     name, cus = polymult.device_info()
+    roofline["valu_issue"] = valu_issue(kname, phases[dom], cus)
     out = {
         "metric": "polymuls/s at N=65536, 2048-bit q: batched+MFMA vs plain HIP; % HBM roofline",
         "value": None if value is None else round(value, 2), "unit": "polymuls/s", "n_gpus": 1,
