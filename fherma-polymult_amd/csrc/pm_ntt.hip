@@ -492,13 +492,14 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
     const PrimeConst pc = P.pc[j];
     const auto fw = row_tw<LOGM, kGen, true>(P, j, rho, pc);
     const auto iw = row_tw<LOGM, kGen, false>(P, j, rho, pc);
-    // one exchange buffer per row, used by a and then b
+    // one exchange buffer per row, used by a and then b; one phase-A twiddle
+    // slot set per row, forward first, refilled with the inverse ones before the
+    // inverse phase A (19 KB of LDS per 16 rows: 8 workgroups per CU)
     uint32_t* la = lds + rl * ROW;
-    uint2* twa = reinterpret_cast<uint2*>(lds + G * ROW) + rl * 2 * GA;
+    uint2* twa = reinterpret_cast<uint2*>(lds + G * ROW) + rl * GA;
     fill_row_phase_a<LOGM>(twa, fw, t);
-    fill_row_phase_a<LOGM>(twa + GA, iw, t);
     const LdsTwA<LOGM, decltype(fw)::kMont> fa{twa};
-    const LdsTwA<LOGM, decltype(iw)::kMont> ia{twa + GA};
+    const LdsTwA<LOGM, decltype(iw)::kMont> ia{twa};
     uint32_t a[16], b[16];
 #pragma unroll
     for (uint32_t i = 0; i < NG; ++i)
@@ -540,6 +541,8 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
     // Each thread rewrites only the LDS words it read above (b's, phase B layout).
 #pragma unroll
     for (uint32_t k = 0; k < 16; ++k) la[padded(16 * t + k)] = a[k];
+    // the forward phase A read its twiddle slots before the syncs above
+    fill_row_phase_a<LOGM>(twa, iw, t);
     wave_sync();
 #pragma unroll
     for (uint32_t i = 0; i < NG; ++i)
@@ -669,7 +672,7 @@ void fused_launch_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStr
     const dim3 grid(1u << (P.logR - logG), P.K, count);
     const dim3 block(1u << (logG + LOGM - 4));
     const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
-                       (2 * sizeof(uint2) << (LOGM - 4 + logG));
+                       (sizeof(uint2) << (LOGM - 4 + logG));
     switch (P.lazy * 2 + (P.row_gen ? 1 : 0)) {
         case 5: launch((k_ntt_rows_fused_batch<LOGM, true, 2>), grid, block, lds, st, P, ba, logG); break;
         case 4: launch((k_ntt_rows_fused_batch<LOGM, false, 2>), grid, block, lds, st, P, ba, logG); break;
@@ -686,7 +689,7 @@ void fused_launch(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t 
     const dim3 grid(1u << (P.logR - logG), P.K);
     const dim3 block(1u << (logG + LOGM - 4));
     const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
-                       (2 * sizeof(uint2) << (LOGM - 4 + logG));
+                       (sizeof(uint2) << (LOGM - 4 + logG));
     switch (P.lazy * 2 + (P.row_gen ? 1 : 0)) {
         case 5: launch((k_ntt_rows_fused_reg<LOGM, true, 2>), grid, block, lds, st, A, B, P, logG); break;
         case 4: launch((k_ntt_rows_fused_reg<LOGM, false, 2>), grid, block, lds, st, A, B, P, logG); break;
