@@ -586,6 +586,15 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
         for (int w = 3; w >= 0; --w) cmp = cmp != 0 ? cmp : cmpw[w * 64 + lane];
         sign = Ctop < 0 ? 1 : (cmp >= 0 ? -1 : 0);   // add q, subtract q, or keep
     }
+    // The correction changes a coefficient only when its quotient estimate was
+    // one off (rare); with every sign zero it would add 0 q and carry nothing,
+    // so a workgroup whose lanes all keep R skips it. The vote: one word per
+    // wave in ex_s, whose summaries every wave read before the last barrier;
+    // the ballot runs with the whole wave active, outside the lane-0 branch.
+    const bool wave_needs = __builtin_amdgcn_ballot_w64(sign != 0) != 0;
+    if (lane == 0) ex_s[wave] = wave_needs ? 1 : 0;
+    __syncthreads();
+    if ((ex_s[0] | ex_s[1] | ex_s[2] | ex_s[3]) != 0) {
     {
         long long cc = 0;
         bool az = true, ao = true;
@@ -616,6 +625,7 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
             x[u] = static_cast<uint32_t>(t);
             C = t >> 32;
         }
+    }
     }
     // 4. the limbs to the stage (the partials are dead), then whole-line stores
     // of the L limbs
