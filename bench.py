@@ -385,8 +385,8 @@ def main():
                          "HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
-              file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (the products of a batch are "
+              "independent and would be split whole across GPUs)", file=sys.stderr)
         sys.exit(2)
 
     import polymult  # noqa: E402
