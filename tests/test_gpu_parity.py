@@ -75,6 +75,19 @@ def test_patterns_match_gmp(golden, name, form, basis):
     run_case(golden, name, form, basis)
 
 
+# The reference's test-matrix points above N = 2^17 (run_tests.sh:18-21), up to the
+# largest N = 2^22, with random odd q, the spec's prime q and all-(q-1) operands:
+# the generic one-barrier-per-stage NTT kernels at their largest grids.
+LARGEST = ["matrix-n%d-w%d-%s" % (n, w, kind) for n, w in ((262144, 128), (1048576, 31), (4194304, 32))
+           for kind in ("random", "prime", "prime-allqm1")]
+
+
+@pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
+@pytest.mark.parametrize("name", LARGEST)
+def test_largest_sizes_match_gmp(golden, name, form):
+    run_case(golden, name, form)
+
+
 @pytest.mark.parametrize("seed", range(24))
 def test_fuzz_matches_oracle(seed):
     """Random points in the spirit of the reference's fuzz runs (fuzz.py): N a
