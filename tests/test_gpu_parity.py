@@ -88,6 +88,23 @@ def test_largest_sizes_match_gmp(golden, name, form):
     run_case(golden, name, form)
 
 
+@pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
+def test_whole_test_matrix_matches_gmp(golden, form):
+    """Every point of the reference's test matrix (run_tests.sh:18-21 and the
+    spec's limb edges), each with a random odd q, the spec's prime q and the
+    all-(q-1) operands: 141 products, each against its GMP digest."""
+    names = sorted(n for n in golden if n.startswith("matrix-"))
+    bad = []
+    for name in names:
+        case = golden[name]
+        q, a, b = pointgen.operands(case)
+        say("matrix", name, "form", form)
+        with polymult.Plan(case["N"], case["W"], q, form=form) as plan:
+            if oracle.digest(plan.multiply(a, b)) != case["c_sha"]:
+                bad.append(name)
+    assert len(names) == 141 and not bad, bad
+
+
 @pytest.mark.parametrize("seed", range(24))
 def test_fuzz_matches_oracle(seed):
     """Random points in the spirit of the reference's fuzz runs (fuzz.py): N a
