@@ -301,19 +301,26 @@ void run_phase(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, h
 }
 
 // The phases of one product on device pointers, enqueued on st. Where the
-// register kernels apply, the forward column passes of A and B go out as one
-// launch: one pass alone can leave a partly filled last round of workgroups
-// (8 K column blocks at N = 65536 against 1024 resident workgroups), which two
-// passes in one launch share.
+// register kernels apply, the decompositions of a and b (matrix-core form) and
+// the forward column passes of A and B go out as one launch each: one pass
+// alone can leave a partly filled last round of workgroups (8 K column blocks
+// at N = 65536 against 1024 resident workgroups), which two passes in one
+// launch share.
 void run_device(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st) {
     if (!batch_kernels_ok(pl->dev)) {
         for (int i = 0; i < PM_NPHASES; ++i) run_phase(pl, a, b, c, st, i);
     } else {
-        run_phase(pl, a, b, c, st, 0);
-        run_phase(pl, a, b, c, st, 1);
         BatchArgs ba{};
+        ba.a[0] = a;
+        ba.b[0] = b;
         ba.A[0] = pl->A;
         ba.B[0] = pl->B;
+        if (pl->form == PM_FORM_MFMA) {
+            launch_decompose_mfma_batch(pl->dev, ba, 1, st);   // b with the b-side tables
+        } else {
+            run_phase(pl, a, b, c, st, 0);
+            run_phase(pl, a, b, c, st, 1);
+        }
         launch_ntt_cols_batch(pl->dev, ba, 1, true, st);
         for (int i = 4; i < PM_NPHASES; ++i) run_phase(pl, a, b, c, st, i);
     }
