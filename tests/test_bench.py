@@ -1,0 +1,74 @@
+"""bench.py's host side on the CPU: its seeded generator reproduces the golden
+fixtures' operands (so the bench's products are checked against GMP digests), the
+kernel names it forms match the committed rocprofv3 tables it reads the HBM
+traffic and VALU counts from, and a request for more than one GPU is refused."""
+import json
+import os
+import subprocess
+import sys
+from types import SimpleNamespace
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+import bench  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def cases():
+    with open(os.path.join(ROOT, "tests", "golden", "golden.json")) as f:
+        return {c["name"]: c for c in json.load(f)["cases"]}
+
+
+@pytest.mark.parametrize("name", ["config-n65536-w2048-00", "config-n65536-w2048-15", "config-uneven-00",
+                                  "config-uneven-63"])
+def test_generator_reproduces_golden_operands(cases, name):
+    c = cases[name]
+    q, a, b = bench.make_point(c["seed"], c["N"], c["W"])
+    assert "%x" % sum(int(w) << (32 * i) for i, w in enumerate(q)) == c["q_hex"]
+    assert bench.digest(a) == c["a_sha"] and bench.digest(b) == c["b_sha"]
+
+
+@pytest.mark.parametrize("key,name,_ms", bench.REFERENCE_POINTS)
+def test_reference_points_reproduce_golden_operands(cases, key, name, _ms):
+    import numpy as np
+    c = cases[name]
+    qi = int(c["q_hex"], 16)
+    q = np.array([(qi >> (32 * l)) & 0xFFFFFFFF for l in range(c["L"])], dtype=np.uint32)
+    _, a, b = bench.make_operands(c["seed"], c["N"], c["W"], q)
+    assert bench.digest(a) == c["a_sha"] and bench.digest(b) == c["b_sha"]
+
+
+def test_golden_digests_cover_the_batch():
+    gold = bench.golden_digests()
+    assert all(bench.SEED0 + i in gold for i in range(bench.BATCH))
+
+
+@pytest.mark.parametrize("basis,lazy", [(31, "1"), (30, "2")])
+def test_kernel_names_match_the_committed_profiles(basis, lazy):
+    plan = SimpleNamespace(basis=basis, twiddles=1, K=134 if basis == 31 else 139)
+    names = bench.phase_kernels(plan)
+    assert names["ntt_rows_fused"] == "k_ntt_rows_fused_reg<8,true,%s>" % lazy
+    if basis == 31:   # the profiles are of the default basis
+        for phase, kernel in names.items():
+            assert bench.measured_traffic(kernel) is not None, (phase, kernel)
+            assert bench.valu_issue(kernel, 0.05, 256) is not None, (phase, kernel)
+
+
+def test_roofline_helpers_read_the_profiles():
+    rows = "k_ntt_rows_fused_reg<8,true,1>"
+    traffic = bench.measured_traffic(rows)
+    alg = 12 * 134 * 65536
+    assert alg <= traffic <= 1.2 * alg
+    v = bench.valu_issue(rows, 0.044, 256)
+    assert v["waves"] == 8576 and 2000 < v["instructions_per_wave"] < 3000
+    expect = v["instructions_per_wave"] * v["waves"] * 4.4 / 1024 / 2.4e9 * 1e3
+    assert abs(v["issue_ms"] - expect) < 1e-4 and abs(v["frac"] - expect / 0.044) < 1e-3
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
