@@ -17,6 +17,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -508,7 +509,23 @@ int pm_batch_create(int count, pm_plan* const* plans, const uint32_t* const* a, 
                         run_batched(plans + i0, a + i0, b + i0, c + i0, i1 - i0, branch[g]);
                     }
                 } else {
-                    for (int i = 0; i < count; ++i) run_device(plans[i], a[i], b[i], c[i], branch[i % branch.size()]);
+                    // Products of different shapes, largest first, each on the branch
+                    // with the least work so far, so that the branches end together
+                    // (work: K N (log N + 2) for the NTTs and steps per prime, plus
+                    // N L K / 8 for the contractions).
+                    auto work = [&](int i) {
+                        const pm::HostPlan& h = plans[i]->host;
+                        return static_cast<double>(h.K) * h.N * (h.logN + 2) + static_cast<double>(h.N) * h.L * h.K / 8;
+                    };
+                    std::vector<int> order(count);
+                    for (int i = 0; i < count; ++i) order[i] = i;
+                    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+                    std::vector<double> load(branch.size(), 0.0);
+                    for (int i : order) {
+                        const size_t k = static_cast<size_t>(std::min_element(load.begin(), load.end()) - load.begin());
+                        run_device(plans[i], a[i], b[i], c[i], branch[k]);
+                        load[k] += work(i);
+                    }
                 }
                 for (size_t s = 0; s < branch.size(); ++s) {
                     check(hipEventRecord(done[s], branch[s]), "hipEventRecord");
