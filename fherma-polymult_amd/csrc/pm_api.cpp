@@ -107,13 +107,16 @@ std::shared_ptr<Twiddles> twiddles_for(const pm::HostPlan& pl) {
     std::vector<uint32_t> z(zw), o(ow);
     for (int fwd = 1; fwd >= 0; --fwd) {
         pm::build_row_twiddles(pl, fwd != 0, 0, pl.K, z.data(), o.data());
-        uint2* dz = static_cast<uint2*>(dev_alloc(zw * sizeof(uint32_t)));
-        uint32_t* dom = static_cast<uint32_t*>(dev_alloc(ow * sizeof(uint32_t)));
-        (fwd ? t->rz : t->rzi) = dz;
-        (fwd ? t->rom : t->romi) = dom;
+        // each buffer is owned by t as soon as it exists, so a failed allocation frees the others
+        uint2*& dz = fwd ? t->rz : t->rzi;
+        dz = static_cast<uint2*>(dev_alloc(zw * sizeof(uint32_t)));
+        uint32_t*& dom = fwd ? t->rom : t->romi;
+        dom = static_cast<uint32_t*>(dev_alloc(ow * sizeof(uint32_t)));
         check(hipMemcpy(dz, z.data(), zw * sizeof(uint32_t), hipMemcpyHostToDevice), "upload row twiddles");
         check(hipMemcpy(dom, o.data(), ow * sizeof(uint32_t), hipMemcpyHostToDevice), "upload row twiddles");
     }
+    // entries whose tables every plan has released are dropped here
+    for (auto e = g_tw_cache.begin(); e != g_tw_cache.end();) e = e->second.expired() ? g_tw_cache.erase(e) : std::next(e);
     g_tw_cache[key] = t;
     return t;
 }

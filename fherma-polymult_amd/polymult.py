@@ -143,12 +143,14 @@ class DeviceBuffer:
 
     def upload(self, arr: np.ndarray) -> None:
         arr = np.ascontiguousarray(arr)
-        assert arr.nbytes <= self.nbytes
+        if arr.nbytes > self.nbytes:
+            raise ValueError("array of %d bytes does not fit a buffer of %d" % (arr.nbytes, self.nbytes))
         _check(lib.pm_copy_to_device(_vp(self.ptr), arr.ctypes.data_as(_vp), arr.nbytes))
 
     def download(self, shape, dtype=np.uint32) -> np.ndarray:
         out = np.empty(shape, dtype=dtype)
-        assert out.nbytes <= self.nbytes
+        if out.nbytes > self.nbytes:
+            raise ValueError("%d bytes requested from a buffer of %d" % (out.nbytes, self.nbytes))
         _check(lib.pm_copy_to_host(out.ctypes.data_as(_vp), _vp(self.ptr), out.nbytes))
         return out
 
@@ -182,6 +184,16 @@ class Stream:
 
     def __del__(self):
         self.close()
+
+
+def _fit(buf: DeviceBuffer, nbytes: int, what: str) -> ctypes.c_void_p:
+    """buf's device pointer, after checking that it holds `nbytes`: the kernels
+    trust the plan's sizes, so a short buffer is refused here, on the host."""
+    if not buf.ptr:
+        raise ValueError("%s: buffer is closed" % what)
+    if buf.nbytes < nbytes:
+        raise ValueError("%s: buffer of %d bytes, the plan needs %d" % (what, buf.nbytes, nbytes))
+    return _vp(buf.ptr)
 
 
 def _sv(stream):
@@ -277,33 +289,49 @@ class Plan:
         _check(lib.pm_plan_tune_host(self.handle, ctypes.byref(m), ctypes.byref(ms)))
         return m.value, ms.value
 
+    @property
+    def limb_bytes(self) -> int:
+        """Bytes of one operand or product: N x L words."""
+        return 4 * self.N * self.L
+
+    @property
+    def residue_bytes(self) -> int:
+        """Bytes of one residue array: K x N words."""
+        return 4 * self.K * self.N
+
+    def _abc(self, a: DeviceBuffer, b: DeviceBuffer, c: DeviceBuffer):
+        n = self.limb_bytes
+        return _fit(a, n, "a"), _fit(b, n, "b"), _fit(c, n, "c")
+
     def multiply_device(self, a: DeviceBuffer, b: DeviceBuffer, c: DeviceBuffer, stream=None) -> None:
-        _check(lib.pm_multiply_device(self.handle, _vp(a.ptr), _vp(b.ptr), _vp(c.ptr), _sv(stream)))
+        _check(lib.pm_multiply_device(self.handle, *self._abc(a, b, c), _sv(stream)))
 
     # the five steps, device to device
     def stage_decompose(self, limbs: DeviceBuffer, res: DeviceBuffer, stream=None) -> None:
-        _check(lib.pm_stage_decompose(self.handle, _vp(limbs.ptr), _vp(res.ptr), _sv(stream)))
+        _check(lib.pm_stage_decompose(self.handle, _fit(limbs, self.limb_bytes, "limbs"),
+                                      _fit(res, self.residue_bytes, "res"), _sv(stream)))
 
     def stage_ntt(self, res: DeviceBuffer, forward: bool, stream=None) -> None:
-        _check(lib.pm_stage_ntt(self.handle, _vp(res.ptr), 1 if forward else 0, _sv(stream)))
+        _check(lib.pm_stage_ntt(self.handle, _fit(res, self.residue_bytes, "res"), 1 if forward else 0, _sv(stream)))
 
     def stage_pointwise(self, A: DeviceBuffer, B: DeviceBuffer, stream=None) -> None:
-        _check(lib.pm_stage_pointwise(self.handle, _vp(A.ptr), _vp(B.ptr), _sv(stream)))
+        _check(lib.pm_stage_pointwise(self.handle, _fit(A, self.residue_bytes, "A"), _fit(B, self.residue_bytes, "B"),
+                                      _sv(stream)))
 
     def stage_crt(self, res: DeviceBuffer, c: DeviceBuffer, stream=None) -> None:
-        _check(lib.pm_stage_crt(self.handle, _vp(res.ptr), _vp(c.ptr), _sv(stream)))
+        _check(lib.pm_stage_crt(self.handle, _fit(res, self.residue_bytes, "res"), _fit(c, self.limb_bytes, "c"),
+                                _sv(stream)))
 
     def time_device(self, a: DeviceBuffer, b: DeviceBuffer, c: DeviceBuffer, warmup: int, iters: int,
                     stream=None) -> float:
         ms = ctypes.c_double(0)
-        _check(lib.pm_time_multiply_device(self.handle, _vp(a.ptr), _vp(b.ptr), _vp(c.ptr), int(warmup),
-                                           int(iters), ctypes.byref(ms), _sv(stream)))
+        _check(lib.pm_time_multiply_device(self.handle, *self._abc(a, b, c), int(warmup), int(iters),
+                                           ctypes.byref(ms), _sv(stream)))
         return ms.value
 
     def profile_phases(self, a: DeviceBuffer, b: DeviceBuffer, c: DeviceBuffer, iters: int, stream=None) -> dict:
         out = (ctypes.c_double * NPHASES)()
-        _check(lib.pm_profile_phases(self.handle, _vp(a.ptr), _vp(b.ptr), _vp(c.ptr), int(iters), out,
-                                     _sv(stream)))
+        _check(lib.pm_profile_phases(self.handle, *self._abc(a, b, c), int(iters), out, _sv(stream)))
         return {lib.pm_phase_name(i).decode(): out[i] for i in range(NPHASES)}
 
     def close(self) -> None:
@@ -321,22 +349,26 @@ class Plan:
         self.close()
 
 
-def batch_multiply_device(plans, a_bufs, b_bufs, c_bufs, stream=None) -> None:
+def _batch_args(plans, a_bufs, b_bufs, c_bufs):
     n = len(plans)
-    arr = lambda xs: (_vp * n)(*[_vp(x) for x in xs])  # noqa: E731
-    _check(lib.pm_batch_multiply_device(n, arr([p._h for p in plans]), arr([x.ptr for x in a_bufs]),
-                                        arr([x.ptr for x in b_bufs]), arr([x.ptr for x in c_bufs]), _sv(stream)))
+    if not (len(a_bufs) == len(b_bufs) == len(c_bufs) == n):
+        raise ValueError("plans and buffers differ in number")
+    abc = [p._abc(a, b, c) for p, a, b, c in zip(plans, a_bufs, b_bufs, c_bufs)]
+    arr = lambda xs: (_vp * n)(*xs)  # noqa: E731
+    return (n, arr([p.handle for p in plans]), arr([x[0] for x in abc]), arr([x[1] for x in abc]),
+            arr([x[2] for x in abc]))
+
+
+def batch_multiply_device(plans, a_bufs, b_bufs, c_bufs, stream=None) -> None:
+    _check(lib.pm_batch_multiply_device(*_batch_args(plans, a_bufs, b_bufs, c_bufs), _sv(stream)))
 
 
 class Batch:
     """A batch of products captured once as a hipGraph; launch() replays it."""
 
     def __init__(self, plans, a_bufs, b_bufs, c_bufs):
-        n = len(plans)
-        arr = lambda xs: (_vp * n)(*[_vp(x) for x in xs])  # noqa: E731
         h = _vp()
-        _check(lib.pm_batch_create(n, arr([p._h for p in plans]), arr([x.ptr for x in a_bufs]),
-                                   arr([x.ptr for x in b_bufs]), arr([x.ptr for x in c_bufs]), ctypes.byref(h)))
+        _check(lib.pm_batch_create(*_batch_args(plans, a_bufs, b_bufs, c_bufs), ctypes.byref(h)))
         self._h = h.value
         self._keep = (list(plans), list(a_bufs), list(b_bufs), list(c_bufs))
 

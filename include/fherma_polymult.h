@@ -133,11 +133,14 @@ int pm_batch_multiply_device(int count, pm_plan* const* plans, const uint32_t* c
 
 /* The same batch captured once as a hipGraph and replayed by pm_batch_launch:
  * one launch per batch, no per-kernel host overhead. Products that share N, L
- * and the primes in the matrix-core form are captured as five phase-batched
- * launches per 32 products; other batches as every product's kernel chain
- * forked over several streams. The device pointers and the plans' forms are
- * fixed at creation; each product needs its own plan, and the plans must
- * outlive the batch. */
+ * and the primes in the matrix-core form are split into up to 16 groups of
+ * consecutive products (PM_BATCH_GROUPS), each group captured as five
+ * phase-batched launches on a stream of its own; other batches as every
+ * product's kernel chain, largest first, on the least-loaded of four streams.
+ * The device pointers and the plans' forms and twiddle sources are fixed at
+ * creation; each product needs its own plan, and the plans must outlive the
+ * batch. A batch works in its plans' residue buffers: its launches must not
+ * overlap each other or other calls on those plans. */
 typedef struct pm_batch pm_batch;
 int pm_batch_create(int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
                     uint32_t* const* c, pm_batch** out);

@@ -330,3 +330,27 @@ def test_row_twiddle_sources_match_gmp(golden, name):
                 assert plan.twiddles == source
                 say("twiddles", name, "form", form, "basis", plan.basis, "source", source)
                 assert oracle.digest(plan.multiply(a, b)) == case["c_sha"], (name, form, source)
+
+
+def test_short_device_buffers_are_refused_on_the_host(golden):
+    """A device buffer smaller than the plan's operand, product or residue
+    array is refused with ValueError before any launch (the kernels trust the
+    plan's sizes); the right sizes then give the GMP product."""
+    case = golden["config-n4096-w256-s1"]
+    q, a, b = pointgen.operands(case)
+    with polymult.Plan(case["N"], case["W"], q, form=polymult.FORM_MFMA) as plan:
+        n, r = plan.limb_bytes, plan.residue_bytes
+        da, db = polymult.DeviceBuffer.from_array(a), polymult.DeviceBuffer.from_array(b)
+        short, C, R = polymult.DeviceBuffer(n - 4), polymult.DeviceBuffer(n), polymult.DeviceBuffer(r)
+        for call in (lambda: plan.multiply_device(da, db, short), lambda: plan.multiply_device(short, db, C),
+                     lambda: plan.stage_decompose(da, short), lambda: plan.stage_ntt(short, True),
+                     lambda: plan.stage_crt(R, short), lambda: polymult.batch_multiply_device([plan], [da], [db], [short]),
+                     lambda: polymult.batch_multiply_device([plan], [da, da], [db], [C]),
+                     lambda: short.upload(a)):
+            with pytest.raises(ValueError):
+                call()
+        plan.multiply_device(da, db, C)
+        polymult.synchronize()
+        assert oracle.digest(C.download((plan.N, plan.L))) == case["c_sha"]
+        for buf in (da, db, short, C, R):
+            buf.close()
