@@ -39,13 +39,19 @@ std::string read_file(const fs::path& p) {
     return s.str();
 }
 
-// The manifest holds four numbers; a key lookup is all it needs.
-double manifest_value(const std::string& text, const char* key) {
+// The manifest holds four non-negative integers; a key lookup is all it needs.
+uint64_t manifest_value(const std::string& text, const char* key) {
     const std::string k = std::string("\"") + key + "\"";
     const size_t at = text.find(k);
     if (at == std::string::npos) throw std::runtime_error(std::string("manifest.json has no ") + key);
-    const size_t colon = text.find(':', at + k.size());
-    return std::strtod(text.c_str() + colon + 1, nullptr);
+    const size_t colon = text.find_first_not_of(" \t\r\n", at + k.size());
+    if (colon == std::string::npos || text[colon] != ':')
+        throw std::runtime_error(std::string("manifest.json: no value for ") + key);
+    char* end = nullptr;
+    const double v = std::strtod(text.c_str() + colon + 1, &end);
+    if (end == text.c_str() + colon + 1 || !(v >= 0) || v > 4294967295.0 || v != static_cast<double>(static_cast<uint64_t>(v)))
+        throw std::runtime_error(std::string("manifest.json: ") + key + " is not an integer in [0, 2^32)");
+    return static_cast<uint64_t>(v);
 }
 
 fherma::Tensor<uint32_t> read_tensor(const fs::path& p, std::vector<int64_t> shape) {
@@ -81,7 +87,7 @@ std::string escape(const std::string& s) {
     std::string o;
     for (char c : s.substr(0, 200)) {
         if (c == '"' || c == '\\') o += '\\';
-        o += c == '\n' ? ' ' : c;
+        o += static_cast<unsigned char>(c) < 0x20 ? ' ' : c;   // no control characters in a JSON string
     }
     return o;
 }
@@ -94,15 +100,21 @@ int main(int argc, char** argv) {
         return 2;
     }
     const fs::path root = argv[1];
-    const std::string manifest = read_file(root / "manifest.json");
-    fherma::Point p;
-    p.N = static_cast<uint32_t>(manifest_value(manifest, "N"));
-    p.W = static_cast<uint32_t>(manifest_value(manifest, "W"));
-    p.L = static_cast<uint32_t>(manifest_value(manifest, "L"));
-    p.q = read_tensor(root / "point" / "q.bin", {static_cast<int64_t>(p.L)});
-    const size_t total = static_cast<size_t>(manifest_value(manifest, "cases"));
     const fs::path out = root / "out";
-    fs::create_directories(out);
+    fherma::Point p;
+    size_t total = 0;
+    try {   // an unreadable point directory: a message and status 2, no answers
+        const std::string manifest = read_file(root / "manifest.json");
+        p.N = static_cast<uint32_t>(manifest_value(manifest, "N"));
+        p.W = static_cast<uint32_t>(manifest_value(manifest, "W"));
+        p.L = static_cast<uint32_t>(manifest_value(manifest, "L"));
+        p.q = read_tensor(root / "point" / "q.bin", {static_cast<int64_t>(p.L)});
+        total = static_cast<size_t>(manifest_value(manifest, "cases"));
+        fs::create_directories(out);
+    } catch (const std::exception& e) {
+        std::cerr << "solution: " << e.what() << "\n";
+        return 2;
+    }
 
     const auto t0 = std::chrono::steady_clock::now();
     void* state = nullptr;

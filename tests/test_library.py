@@ -104,6 +104,28 @@ def test_driver_refuses_out_of_range_points(tmp_path):
         assert not (d / "out" / "000000").exists()
 
 
+def test_driver_refuses_unreadable_point_directories(tmp_path):
+    """A missing manifest, a missing or malformed value, or a q.bin of the wrong
+    size ends the driver with status 2 and a message, before fherma_init."""
+    import json
+    solution = os.path.join(ROOT, "fherma-polymult_amd", "solution")
+    good = {"N": 64, "W": 32, "L": 1, "cases": 1}
+    bad = [None, {"N": 64, "W": 32, "L": 1}, dict(good, N=-64), dict(good, cases=1.5), "{\"N\" 64}"]
+    for i, manifest in enumerate(bad + ["short-q"]):
+        d = tmp_path / ("d%d" % i)
+        (d / "point").mkdir(parents=True)
+        np.ones(1, dtype="<u4")[:0 if manifest == "short-q" else 1].tofile(str(d / "point" / "q.bin"))
+        if manifest == "short-q":
+            manifest = good
+        if isinstance(manifest, dict):
+            (d / "manifest.json").write_text(json.dumps(manifest))
+        elif manifest is not None:
+            (d / "manifest.json").write_text(manifest)
+        r = subprocess.run([solution, str(d)], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and r.stderr.startswith("solution: "), (manifest, r.returncode, r.stderr)
+        assert not (d / "out" / "results.json").exists()
+
+
 def test_invalid_basis_is_refused_before_device_work():
     """PM_BASIS other than p30, p31 or auto fails plan creation with PM_EINVAL."""
     code = ("import sys; sys.path.insert(0, %r); import numpy as np, polymult\n"
