@@ -38,7 +38,7 @@ struct DevPlan {
     // dec_digits_b / dec_corr_b are laid out as dec_digits / dec_corr.
     const uint32_t* pow2b;
     const int8_t* dec_digits_b;
-    const double* dec_corr_b;
+    const uint64_t* dec_corr_b;
     const uint2* tw;          // K x N: (psi^bitrev(i), Shoup companion)
     const uint2* itw;         // K x N: (psi^-bitrev(i), Shoup companion)
     // Row-pass twiddles, generated: the twiddle of row rho, stage distance 2^e,
@@ -58,9 +58,10 @@ struct DevPlan {
     const double* nq;         // K + 1: negvP_v / q
     // int8 matrix-core forms (pm_mfma.hip); null until prepared.
     const int8_t* dec_digits; // [ceil(K/4)][dec_ksteps][64 lanes][16]: MFMA A fragments
-    // K: 128 sum_t (256^t mod p_j) mod p_j plus the least multiple of p_j that
-    // is >= 2^47, so that every decompose sum is positive before its reduction
-    const double* dec_corr;
+    // K: 128 sum_t T_{j,t} mod p_j (T the table entries, 256^t 2^32 mod p_j
+    // times the side's factor) plus the least multiple of p_j that is >= 2^47,
+    // so that every decompose sum is positive before its reduction
+    const uint64_t* dec_corr;
     // CRT digit table, column-major: word J kTabRows + R (pm_mfma.hip) holds
     // byte R - 3 of the balanced digits of M_j for primes j = 4J .. 4J + 3
     // (zero for j >= K and outside digit rows 0 .. 4L).

@@ -12,13 +12,16 @@
 // values become balanced base-256 digits in [-128, 127], built on the host.
 //
 // Decompose (reference rns.cuh:110-128: x mod p_j = sum_l limb_l 2^(32 l) mod p_j):
-//   x_n = sum_t u_{n,t} 256^t over its 4L bytes; T_{j,t} = 256^t mod p_j, centred
-//   into (-p/2, p/2] and split into 4 balanced digits e_{j,t,b}:
+//   x_n = sum_t u_{n,t} 256^t over its 4L bytes; T_{j,t} = 256^t 2^32 mod p_j
+//   (Montgomery form), centred into (-p/2, p/2] and split into 4 balanced
+//   digits e_{j,t,b}:
 //     G_b[j][n] = sum_t e_{j,t,b} (u_{n,t} - 128)        |G_b| <= 4L 2^14 <= 2^22
-//     x_n mod p_j = (sum_b 256^b G_b + 128 sum_t T_{j,t}) mod p_j
+//     x_n 2^32 mod p_j = (sum_b 256^b G_b + 128 sum_t T_{j,t}) mod p_j
 //   MFMA rows are (prime, digit) pairs, columns coefficients, K-dim bytes t. The
-//   four G_b of one (j, n) land in one lane and are combined in FP64 (exact below
-//   2^53) and reduced with one FP64 quotient estimate.
+//   four G_b of one (j, n) land in one lane; they are combined into one exact
+//   64-bit integer (made positive by a multiple of p_j) and reduced by one
+//   Montgomery step, which also takes the factor 2^32 away: integer arithmetic
+//   only (FP64 appears in the CRT's quotient rounding alone).
 //
 // CRT (reference rns_kernels.cuh:265-364: x = sum_j y_j M_j + negvP_v mod q):
 //   y_{j,n} = sum_a u_{j,n,a} 256^a (4 bytes), M_j = sum_s m_{j,s} 256^s (4L + 1
@@ -63,15 +66,17 @@ __device__ __forceinline__ v4i mfma16(const v4i& a, const v4i& b, const v4i& c) 
     return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
 }
 
-// r mod p for an integer-valued double 0 <= r < 2^50: the estimate
-// floor(r / p - 1/2) (the FMA's error is below 2^-30) is floor(r / p) or one
-// less, so the exact FMA remainder lies in [0, 2p), below 2^32; it is converted
-// and (kLazy false) brought to [0, p) with one unsigned min.
+// R 2^-32 mod p for 0 <= R < 2^49, by one Montgomery step (pneg = -p^-1 mod
+// 2^32): m = R pneg mod 2^32 makes R + m p a multiple of 2^32; the low words
+// of R and m p then add to 0 if R's low word is 0 and to 2^32 otherwise, so
+// the carry into the high word is (low word != 0). (R + m p) / 2^32 < 2^17 + p
+// lies in [0, 2p), below 2^32; kLazy false brings it to [0, p) with one
+// unsigned min.
 template <bool kLazy = false>
-__device__ __forceinline__ uint32_t mod_p_f64(double r, uint32_t p, double pinv) {
-    const double qd = floor(fma(r, pinv, -0.5));
-    const uint32_t v = static_cast<uint32_t>(fma(-qd, static_cast<double>(p), r));
-    return kLazy ? v : min(v, v - p);
+__device__ __forceinline__ uint32_t mont_fold(uint64_t R, uint32_t p, uint32_t pneg) {
+    const uint32_t rl = static_cast<uint32_t>(R), rh = static_cast<uint32_t>(R >> 32);
+    const uint32_t t = rh + __umulhi(rl * pneg, p) + (rl != 0u ? 1u : 0u);
+    return kLazy ? t : min(t, t - p);
 }
 
 // Decompose, one workgroup per 64 coefficients (4 tiles of 16). Their limbs are
@@ -85,14 +90,14 @@ __device__ __forceinline__ void decompose_tile(const uint32_t* __restrict__ limb
     extern __shared__ uint32_t xs[];
     constexpr int Ls = 16 * KT + 4;
     const int L = P.L;
-    // per-prime constants for the epilogue: 1/p, corr, p
-    double* kpinv = reinterpret_cast<double*>(xs + 16 * kTiles * Ls);
-    double* kcorr = kpinv + P.K;
+    // per-prime constants for the epilogue: corr, p, -p^-1 mod 2^32
+    uint64_t* kcorr = reinterpret_cast<uint64_t*>(xs + 16 * kTiles * Ls);
     uint32_t* kp = reinterpret_cast<uint32_t*>(kcorr + P.K);
+    uint32_t* kpneg = kp + P.K;
     for (int j = threadIdx.x; j < P.K; j += blockDim.x) {
-        kpinv[j] = P.p_recip[j];
-        kp[j] = P.pc[j].p;
         kcorr[j] = P.dec_corr[j];
+        kp[j] = P.pc[j].p;
+        kpneg[j] = P.pc[j].pneg;
     }
     const uint32_t nb = k0 + blockIdx.x * (16 * kTiles);
     const uint32_t nk = min(static_cast<uint32_t>(16 * kTiles), k0 + count - nb);
@@ -190,17 +195,17 @@ __device__ __forceinline__ void decompose_tile(const uint32_t* __restrict__ limb
         // Lane (g, cl) holds rows 4g .. 4g + 3 = prime 4 jg + g, digits b = 0..3.
         const int j = 4 * jg + g;
         if (j < P.K) {
-            const uint32_t p = kp[j];
-            const double pinv = kpinv[j];
-            const double corr = kcorr[j];
+            const uint32_t p = kp[j], pneg = kpneg[j];
+            const uint64_t corr = kcorr[j];
             uint32_t* out = res + static_cast<size_t>(j) * P.N + nb;
 #pragma unroll
             for (int t = 0; t < kTiles; ++t) {
-                // |G_b| <= 2^22, so G_0 + 256 G_1 and G_2 + 256 G_3 fit in int32
+                // |G_b| <= 2^22, so G_0 + 256 G_1 and G_2 + 256 G_3 fit in int32, and
+                // R = sum_b 256^b G_b + corr lies in [0, 2^49)
                 const int lo = acc[t][0] + acc[t][1] * 256;
                 const int hi = acc[t][2] + acc[t][3] * 256;
-                const double r = fma(65536.0, static_cast<double>(hi), static_cast<double>(lo) + corr);
-                const uint32_t v = mod_p_f64<kLazyOut>(r, p, pinv);
+                const uint64_t R = static_cast<uint64_t>(static_cast<long long>(lo) + static_cast<long long>(hi) * 65536) + corr;
+                const uint32_t v = mont_fold<kLazyOut>(R, p, pneg);
                 if (static_cast<uint32_t>(16 * t + cl) < nk) out[16 * t + cl] = v;
             }
         }
@@ -677,16 +682,17 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
     if (L > 4 * kDecMaxKSteps * 4 || ((K + 3) / 4 + 1 + 3) / 4 > kCrtMaxKSteps) {
         throw std::runtime_error("point too large for the matrix-core tables");
     }
-    // Decompose tables: T_{j,t} = f_j 256^t mod p_j with f_j = 1 (the plain
-    // tables) or f_j = scale_j (the b side: b scale_j mod p_j).
+    // Decompose tables: T_{j,t} = f_j 256^t 2^32 mod p_j with f_j = 1 (the plain
+    // tables) or f_j = scale_j (the b side: b scale_j mod p_j); the factor 2^32
+    // is what the kernel's Montgomery step takes away.
     const int KT = (4 * L + 63) / 64;
     const int KG = (K + 3) / 4;
-    auto dec_tables = [&](bool scaled, const int8_t** digits, const double** corr) {
+    auto dec_tables = [&](bool scaled, const int8_t** digits, const uint64_t** corr) {
         std::vector<int8_t> dec(static_cast<size_t>(KG) * KT * 64 * 16, 0);
-        std::vector<double> dcorr(K);
+        std::vector<uint64_t> dcorr(K);
         for (int j = 0; j < K; ++j) {
             const uint64_t p = h.p[j];
-            uint64_t T = scaled ? h.scale[j] % p : 1, sum = 0;
+            uint64_t T = (scaled ? h.scale[j] % p : 1) * ((uint64_t(1) << 32) % p) % p, sum = 0;
             for (int t = 0; t < 4 * L; ++t) {
                 sum = (sum + T) % p;
                 const long long centred =
@@ -702,12 +708,12 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
                 T = T * 256 % p;
             }
             // |sum_b 256^b G_b| < 2^47: adding ceil(2^47 / p) p keeps it positive and
-            // below 2^49 (exact in FP64), without changing the residue
+            // below 2^49, without changing the residue
             const uint64_t lift = ((uint64_t(1) << 47) + p - 1) / p * p;
-            dcorr[j] = static_cast<double>(sum * 128 % p + lift);
+            dcorr[j] = sum * 128 % p + lift;
         }
         *digits = static_cast<const int8_t*>(upload(dec.data(), dec.size()));
-        *corr = static_cast<const double*>(upload(dcorr.data(), dcorr.size() * sizeof(double)));
+        *corr = static_cast<const uint64_t*>(upload(dcorr.data(), dcorr.size() * sizeof(uint64_t)));
     };
     dec_tables(false, &d.dec_digits, &d.dec_corr);
     dec_tables(true, &d.dec_digits_b, &d.dec_corr_b);
@@ -804,7 +810,7 @@ namespace {
 template <int KT>
 void dec_launch(const DevPlan& P, const uint32_t* limbs, uint32_t* res, uint32_t k0, uint32_t count, hipStream_t st,
                 bool lazy_out, const BatchArgs* ba, int nprod) {
-    const size_t lds = sizeof(uint32_t) * 16 * kTiles * (16 * KT + 4) + 20 * static_cast<size_t>(P.K);
+    const size_t lds = sizeof(uint32_t) * 16 * kTiles * (16 * KT + 4) + 16 * static_cast<size_t>(P.K);
     const dim3 block(64 * kWaves);
     if (ba) {
         const dim3 grid((P.N + 16 * kTiles - 1) / (16 * kTiles), 1, 2 * nprod);
