@@ -132,6 +132,39 @@ def test_fuzz_matches_oracle(seed):
             assert np.array_equal(plan.multiply(a, b), ref), (seed, N, W, form, basis)
 
 
+def _limbs(x, L):
+    return np.array([(x >> (32 * i)) & 0xFFFFFFFF for i in range(L)], dtype=np.uint32)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_fuzz_extreme_operands_match_oracle(seed):
+    """Random N and q width with the operands at the top of their range: q of
+    the form 2^W - 1 or 2^(W-1) + 1 with a = q - 1 and b random (even seeds),
+    or a random odd q with a = b = q - 1 (odd seeds), so that every CRT sum
+    sits near its largest value; all three forms against the oracle."""
+    rng = random.Random(92000 + seed)
+    N, W = 1 << rng.randint(1, 14), rng.randint(2, 2048)
+    q, _, _ = oracle.gen_point(92000 + seed, N, W)
+    L = q.shape[0]
+    if seed % 2 == 0:
+        qi = (1 << W) - 1 if rng.random() < 0.5 else (1 << (W - 1)) + 1
+        q = _limbs(qi, L)
+        a = np.tile(_limbs(qi - 1, L), (N, 1))
+        b = np.stack([_limbs(rng.randrange(qi), L) for _ in range(N)])
+    else:
+        qi = sum(int(w) << (32 * i) for i, w in enumerate(q))
+        a = np.tile(_limbs(qi - 1, L), (N, 1))
+        b = a.copy()
+    say("fuzz-extreme", seed, "N", N, "W", W)
+    ref = oracle.Plan(N, q).multiply(a, b)
+    for form, basis in ((polymult.FORM_PLAIN, 0), (polymult.FORM_MFMA, 0), (polymult.FORM_MFMA, 30)):
+        try:
+            with polymult.Plan(N, W, q, form=form, basis=basis) as plan:
+                assert np.array_equal(plan.multiply(a, b), ref), (seed, N, W, form, basis)
+        except polymult.PolymulError as e:   # too few primes below 2^30 for this N and width
+            assert basis == 30 and e.code == polymult.PM_EINVAL, (seed, N, W, str(e))
+
+
 @pytest.mark.parametrize("N,W", [(64, 100), (256, 1747), (4096, 256), (65536, 2048), (131072, 512)])
 def test_basis_choice(N, W):
     """The default plan takes the reference's primes (the reference's K); an
