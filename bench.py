@@ -107,11 +107,10 @@ def phase_bytes(K, L):
 def phase_kernels(plan):
     """Kernel names (as rocprofv3 reports them, spaces removed) of each phase of
     the N = 65536 one-product schedule: the register NTT kernels take the twiddle
-    source and the butterfly form (1: forward residues in [0, 2p) with the
-    reference's primes, 2: lazily reduced both ways below 2^30) as template
+    source and the butterfly form (1: forward residues in [0, 2p)) as template
     arguments, the CRT its K-steps (one group per four primes plus the quotient
     group, 16 groups per K-step)."""
-    lazy = "2" if plan.basis == 30 else "1"
+    lazy = "1"
     gen = "true" if plan.twiddles == 1 else "false"
     ks = ((plan.K + 3) // 4 + 1 + 3) // 4
     col = "k_ntt_col_reg<8,%s,5," + lazy + ">"
@@ -240,35 +239,6 @@ def batch_rates(args, plans, da, db, dc, gold, log):
     return value, sec, plain_value, mismatches
 
 
-def p30_rate(args, points, da, db, dc, gold, log):
-    """The same batch with the opt-in prime basis below 2^30 (K = 139 instead of
-    the reference's 134; lazily reduced NTT butterflies in both directions):
-    products/s and mismatches against the GMP digests. Never the headline."""
-    import polymult  # noqa: E402
-    plans = [polymult.Plan(N, W, q, form=polymult.FORM_MFMA, basis=polymult.BASIS_P30) for q, _, _ in points]
-    graph = polymult.Batch(plans, da, db, dc)
-    graph.launch()
-    polymult.synchronize()
-    L = plans[0].L
-    mismatches = sum(digest(dc[i].download((N, L))) != gold.get(SEED0 + i) for i in range(BATCH))
-    for _ in range(args.warmup):
-        graph.launch()
-    polymult.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        graph.launch()
-    polymult.synchronize()
-    sec = (time.perf_counter() - t0) / args.steps
-    out = {"value": round(BATCH / sec, 2), "unit": "polymuls/s", "K": plans[0].K, "prime_basis": "p < 2^30",
-           "mismatches": int(mismatches),
-           "note": "opt-in basis (PM_BASIS=p30): same products, more primes, cheaper butterflies"}
-    log("p30 basis: %.3f ms per step, %.1f products/s, %d mismatches" % (sec * 1e3, BATCH / sec, mismatches))
-    graph.close()
-    for p in plans:
-        p.close()
-    return out
-
-
 def uneven_rate(args, log):
     """The 64-product batch of uneven shapes (N in 4096 .. 65536, q of 128 .. 2019
     bits, seeds 4000 .. 4063 as in the golden fixtures), device-resident, replayed
@@ -371,7 +341,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--no-p30", action="store_true", help="skip the opt-in prime basis below 2^30")
     ap.add_argument("--no-host", action="store_true", help="skip the host-buffer (PCIe-inclusive) rate")
     ap.add_argument("--no-uneven", action="store_true", help="skip the 64-product uneven batch")
     ap.add_argument("--no-points", action="store_true", help="skip the reference's two measured points")
@@ -420,9 +389,6 @@ def main():
         log("one product at a time: %d mismatches vs GMP" % mismatches)
     else:
         value, sec, plain_value, mismatches = batch_rates(args, plans, da, db, dc, gold, log)
-    p30 = None
-    if not (args.phases_only or args.no_p30):
-        p30 = p30_rate(args, points, da, db, dc, gold, log)
     uneven = None
     if not (args.phases_only or args.no_uneven):
         uneven = uneven_rate(args, log)
@@ -476,14 +442,13 @@ def main():
         "config": {"workload": "batch of 16 products, N=65536, random odd 2048-bit q (L=64), device-resident", "form": "int8-MFMA decompose/CRT, batch replayed as one hipGraph", "batch": BATCH, "N": N, "W": W, "L": L, "K": K,
                    "parallelism": "single GPU", "device": name, "cus": cus,
                    "row_twiddles": "generated" if plans[0].twiddles == polymult.TWIDDLES_GEN else "table",
-                   "prime_basis": "p < 2^%d (the reference's rule)" % plans[0].basis},
+                   "prime_basis": "p < 2^31 (the reference's rule)"},
         "plain_value": None if plain_value is None else round(plain_value, 2),
         "mfma_over_plain": None if plain_value is None else round(value / plain_value, 3),
         "mismatches": int(mismatches),
         "effective_GBps_per_product_yardstick": None if value is None else round(prod_bytes * value / 1e9, 1),
         "phases_ms": {k: round(v, 5) for k, v in phases.items()},
         "host_inclusive": host,
-        "p30_basis": p30,
         "uneven_batch": uneven,
         "reference_points": ref_points,
         "roofline": roofline,

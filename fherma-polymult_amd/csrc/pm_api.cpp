@@ -143,23 +143,11 @@ int row_twiddle_policy(const pm::HostPlan& h) {
     return 16.0 * h.K * h.N > 64e6 ? 1 : 0;
 }
 
-// The prime basis of a new plan: the reference's rule (primes below 2^31)
-// unless PM_BASIS=p30 (PM_BASIS=p31 and auto name the default) or an explicit
-// PM_BASIS_P30 asks for the same walk below 2^30. `kernels`: every NTT pass of
-// the geometry has a register kernel, whose faster butterfly forms the plan
-// then uses.
-pm::HostPlan plan_in_basis(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, int basis, bool& kernels) {
-    if (basis == PM_BASIS_AUTO) {
-        basis = PM_BASIS_P31;
-        if (const char* v = std::getenv("PM_BASIS")) {
-            if (!std::strcmp(v, "p30"))
-                basis = PM_BASIS_P30;
-            else
-                need(!std::strcmp(v, "p31") || !std::strcmp(v, "auto"), "PM_BASIS must be p30, p31 or auto");
-        }
-    }
-    need(basis == PM_BASIS_P30 || basis == PM_BASIS_P31, "basis must be PM_BASIS_AUTO, PM_BASIS_P30 or PM_BASIS_P31");
-    pm::HostPlan h = pm::make_plan(N, W, q, static_cast<int>(L), basis == PM_BASIS_P30 ? 30 : 31);
+// The host plan of a point with the reference's prime rule; `kernels`: every
+// NTT pass of the geometry has a register kernel, whose faster butterfly form
+// the plan then uses.
+pm::HostPlan plan_for(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, bool& kernels) {
+    pm::HostPlan h = pm::make_plan(N, W, q, static_cast<int>(L));
     pm::DevPlan g{};
     g.logS = h.logS;
     g.logR = h.logR;
@@ -167,11 +155,11 @@ pm::HostPlan plan_in_basis(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q
     return h;
 }
 
-pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, int basis) {
+pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q) {
     need(q != nullptr, "q is null");
     std::unique_ptr<pm_plan> pl(new pm_plan());
     bool kernels = false;
-    pl->host = plan_in_basis(N, W, L, q, basis, kernels);
+    pl->host = plan_for(N, W, L, q, kernels);
     const pm::HostPlan& h = pl->host;
     const int K = h.K;
     const int row_gen = row_twiddle_policy(h);   // before any device work
@@ -198,9 +186,7 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q,
     d.L = h.L;
     d.K = K;
     d.row_gen = row_gen;
-    d.lazy = kernels ? (h.pbits == 30 ? 2 : 1) : 0;
-    d.q_drop = h.q_drop;
-    d.q_top = h.q_top;
+    d.lazy = kernels ? 1 : 0;
     d.pc = pl->upload(pc);
     d.pow2 = pl->upload(h.pow2);
     d.pow2b = pl->upload(h.pow2b);
@@ -351,18 +337,12 @@ extern "C" {
 const char* pm_last_error(void) { return g_err.c_str(); }
 
 int pm_plan_create(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, pm_plan** out) {
-    return pm_plan_create_basis(N, W, L, q, PM_BASIS_AUTO, out);
-}
-
-int pm_plan_create_basis(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, int basis, pm_plan** out) {
     return guarded([&] {
         need(out != nullptr, "out is null");
         *out = nullptr;
-        *out = make_plan_object(N, W, L, q, basis);
+        *out = make_plan_object(N, W, L, q);
     });
 }
-
-int pm_plan_basis(const pm_plan* plan) { return plan ? plan->host.pbits : -1; }
 
 void pm_plan_destroy(pm_plan* plan) { delete plan; }
 
@@ -384,15 +364,10 @@ int pm_plan_primes(const pm_plan* plan, uint32_t* out, uint32_t cap) {
 }
 
 int pm_choose_primes(uint32_t N, uint32_t qbits, uint32_t* out, uint32_t cap, uint32_t* K) {
-    return pm_choose_primes_basis(N, qbits, PM_BASIS_P31, out, cap, K);
-}
-
-int pm_choose_primes_basis(uint32_t N, uint32_t qbits, int basis, uint32_t* out, uint32_t cap, uint32_t* K) {
     return guarded([&] {
         need(out != nullptr && K != nullptr, "null argument");
         need(N >= 2 && (N & (N - 1)) == 0 && N <= (1u << pm::kMaxLogN), "N must be a power of two in [2, 2^22]");
-        need(basis == PM_BASIS_P30 || basis == PM_BASIS_P31, "basis must be PM_BASIS_P30 or PM_BASIS_P31");
-        const std::vector<uint32_t> p = pm::choose_primes(N, static_cast<int>(qbits), basis);
+        const std::vector<uint32_t> p = pm::choose_primes(N, static_cast<int>(qbits));
         need(p.size() <= cap, "cap is smaller than K");
         std::memcpy(out, p.data(), p.size() * sizeof(uint32_t));
         *K = static_cast<uint32_t>(p.size());

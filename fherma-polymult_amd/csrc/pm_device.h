@@ -22,14 +22,12 @@ struct PrimeConst {
 
 struct DevPlan {
     uint32_t N, logN, logS, logR;
-    int L, K, q_drop;
+    int L, K;
     int row_gen;              // row-pass twiddles: 1 generated (rz, rom), 0 from the full tables
     // Butterfly form of the register NTT kernels (pm_ntt.hip), used when every
-    // pass has one: 1 forward residues in [0, 2p) (any p < 2^31), 2 forward and
-    // inverse lazily reduced (every prime below 2^30); 0 canonical (the generic
+    // pass has one: 1 forward residues in [0, 2p); 0 canonical (the generic
     // kernels, or a geometry without register kernels for every pass).
     int lazy;
-    double q_top;
     const PrimeConst* pc;     // K
     const uint32_t* pow2;     // K x L: 2^(32 l) mod p_j
     // The b side of a product: tables that decompose b into b scale_j mod p_j

@@ -41,29 +41,6 @@ __device__ __forceinline__ void gs(uint32_t& x, uint32_t& y, const uint2 w, uint
     y = mul_shoup(u - y + p, w.x, w.y, p);
 }
 
-// Lazily reduced butterflies (Harvey's) for primes p < 2^30, where 4p < 2^32:
-// residues stay in [0, 4p) through the forward (CT) stages and in [0, 2p)
-// through the inverse (GS) ones, for 8 and 9 VALU instructions instead of the
-// canonical 12 and 11.
-// CT: x in [0, 4p) comes down to [0, 2p); v = y w mod p, lazily in [0, 2p) for
-// any y < 2^32, is formed negated (nv = -v mod 2^32), so that x + v in [0, 4p)
-// and x - v + 2p in (0, 4p) take one instruction each.
-__device__ __forceinline__ void ct_lazy(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    const uint32_t p2 = p + p;
-    const uint32_t nv = mulhi32(y, w.y) * p - y * w.x;
-    const uint32_t xr = csub(x, p2);
-    x = xr - nv;
-    y = xr + nv + p2;
-}
-
-// GS: x, y in [0, 2p) give x + y back in [0, 2p) and (x - y + 2p) w lazily in [0, 2p).
-__device__ __forceinline__ void gs_lazy(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    const uint32_t p2 = p + p;
-    const uint32_t s = x + y, d = x - y + p2;
-    x = csub(s, p2);
-    y = d * w.x - mulhi32(d, w.y) * p;
-}
-
 // The forward butterfly for any p < 2^31 with residues in [0, 2p): x comes
 // down to [0, p); the lazy Shoup remainder v of y w (in [0, 2p)) is formed as
 // m = -v - 1 mod 2^32, and max(m, m + p) = -v_r - 1 with v_r = v mod p in
@@ -105,15 +82,6 @@ __device__ __forceinline__ void ct_half_m(uint32_t& x, uint32_t& y, const uint2 
     y = xr + M + (p + 1u);
 }
 
-// ct_lazy's ranges (p < 2^30, x in [0, 4p)): x comes down to [0, 2p), and
-// xr + d + p, xr - d + p lie in (0, 4p).
-__device__ __forceinline__ void ct_lazy_m(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    const uint32_t nd = mulhi32(y * w.y, p) - mulhi32(y, w.x);
-    const uint32_t xp = csub(x, p + p) + p;
-    x = xp - nd;
-    y = xp + nd;
-}
-
 // Canonical GS: x, y in [0, p); x - y in (-p, p) as a signed word,
 // |(x - y) w'| < p^2.
 __device__ __forceinline__ void gs_m(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
@@ -122,40 +90,23 @@ __device__ __forceinline__ void gs_m(uint32_t& x, uint32_t& y, const uint2 w, ui
     y = canon_signed(r, p);
 }
 
-// gs_lazy's ranges (p < 2^30, x, y in [0, 2p)): x - y in (-2p, 2p),
-// |(x - y) w'| < 2 p^2 < 2^31 p; the product d + p is left in (0, 2p).
-__device__ __forceinline__ void gs_lazy_m(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    const uint32_t r = mont_tw_diff_signed(x - y, w.x, w.y, p);
-    x = csub(x + y, p + p);
-    y = r + p;
-}
-
-// Butterfly forms (kMode): 0 canonical, residues in [0, p); 1 forward for any
-// p < 2^31 with residues in [0, 2p) (ct_half), inverse canonical; 2 both
-// lazily reduced for p < 2^30 (ct_lazy, gs_lazy). kMont: the twiddle is a
-// Montgomery pair (generated row twiddles), else a Shoup pair (tables).
+// Butterfly forms (kMode): 0 canonical, residues in [0, p); 1 forward with
+// residues in [0, 2p) (ct_half), inverse canonical. kMont: the twiddle is a
+// Montgomery pair (generated row twiddles, inverse tables), else a Shoup pair.
 template <bool kFwd, int kMode, bool kMont>
 __device__ __forceinline__ void bfly(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
     if constexpr (kMont) {
-        if (kFwd && kMode == 2)
-            ct_lazy_m(x, y, w, p);
-        else if (kFwd && kMode == 1)
+        if (kFwd && kMode == 1)
             ct_half_m(x, y, w, p);
         else if (kFwd)
             ct_m(x, y, w, p);
-        else if (kMode == 2)
-            gs_lazy_m(x, y, w, p);
         else
             gs_m(x, y, w, p);
     } else {
-        if (kFwd && kMode == 2)
-            ct_lazy(x, y, w, p);
-        else if (kFwd && kMode == 1)
+        if (kFwd && kMode == 1)
             ct_half(x, y, w, p);
         else if (kFwd)
             ct(x, y, w, p);
-        else if (kMode == 2)
-            gs_lazy(x, y, w, p);
         else
             gs(x, y, w, p);
     }
@@ -325,9 +276,7 @@ struct BufTw {
 // slot padded(u) of column c is at padded(u) C + c; with t < TPS and
 // TPS NG = 16, padded(t + TPS i + 16 k) = t + TPS i + 17 k and
 // padded(16 t + k) = 17 t + k, so every slot is a lane base plus a constant.
-// kMode 1: the forward pass leaves its output in [0, 2p); kMode 2 (p < 2^30):
-// the forward pass leaves [0, 4p), the inverse pass takes input in [0, 2p) and
-// leaves it canonical.
+// kMode 1: the forward pass leaves its output in [0, 2p).
 template <int LOGM, bool kFwd, int LOGC, int kMode>
 __device__ __forceinline__ void col_pass(uint32_t* __restrict__ X, const DevPlan& P) {
     extern __shared__ uint32_t lds[];
@@ -372,10 +321,6 @@ __device__ __forceinline__ void col_pass(uint32_t* __restrict__ X, const DevPlan
 #pragma unroll
             for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = la[(TPS * i + 17 * k) * C];
         phase_a<LOGM, false, false, kMode>(r, r, tw, p);
-        if (kMode == 2) {
-#pragma unroll
-            for (uint32_t k = 0; k < 16; ++k) r[k] = csub(r[k], p);
-        }
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -398,8 +343,7 @@ __global__ void __launch_bounds__(512) k_ntt_col_batch(DevPlan P, BatchArgs ba) 
 // Row pass alone (forward or inverse): G = 2^logG rows of M = S elements.
 // Twiddles are generated per row (kGen) or read from the full table; phase
 // A's are staged once per row into LDS by the row's first threads. The output
-// is canonical; the forward pass accepts input in [0, 2p) (kMode 1) or [0, 4p)
-// (kMode 2).
+// is canonical; the forward pass accepts input in [0, 2p) (kMode 1).
 template <int LOGM, bool kFwd, bool kGen, int kMode>
 __global__ void __launch_bounds__(512)
 k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
@@ -432,10 +376,7 @@ k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) r[k] = l[padded(16 * t + k)];
         phase_b<true, false, kMode>(r, r, gen, t, pc.p);
-        if (kMode == 2) {
-#pragma unroll
-            for (uint32_t k = 0; k < 16; ++k) r[k] = csub(csub(r[k], 2 * pc.p), pc.p);
-        } else if (kMode == 1) {
+        if (kMode == 1) {
 #pragma unroll
             for (uint32_t k = 0; k < 16; ++k) r[k] = csub(r[k], pc.p);
         }
@@ -454,10 +395,6 @@ k_ntt_row_reg(uint32_t* __restrict__ X, DevPlan P, uint32_t logG) {
 #pragma unroll
             for (uint32_t k = 0; k < GA; ++k) r[i * GA + k] = l[padded(t + TPS * i + 16 * k)];
         phase_a<LOGM, false, false, kMode>(r, r, ta, pc.p);
-        if (kMode == 2) {
-#pragma unroll
-            for (uint32_t k = 0; k < 16; ++k) r[k] = csub(r[k], pc.p);
-        }
 #pragma unroll
         for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -476,9 +413,6 @@ __device__ __forceinline__ uint32_t pointwise_mul(uint32_t a, uint32_t b, const 
 // product with N^-1 (P/p_j)^-1 into A, inverse rows of A. kMode 1: input in
 // [0, 2p); a brought to [0, p) keeps a b below 2 p^2 < 2^32 p, which the
 // canonical Montgomery product needs.
-// kMode 2 (p < 2^30): input in [0, 4p); both factors are brought to [0, 2p),
-// which keeps a b below 2^32 p and the Montgomery product in [0, 2p), the
-// inverse stages' input range; the output is left in [0, 2p).
 template <int LOGM, bool kGen, int kMode>
 __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, const DevPlan& P,
                                            uint32_t logG) {
@@ -526,11 +460,7 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
 #pragma unroll
     for (uint32_t k = 0; k < 16; ++k) b[k] = la[padded(16 * t + k)];
     phase_b<true, true, kMode>(a, b, fw, t, pc.p);
-    if (kMode == 2) {
-#pragma unroll
-        for (uint32_t k = 0; k < 16; ++k)
-            a[k] = mont_mul_lazy(csub(a[k], 2 * pc.p), csub(b[k], 2 * pc.p), pc.p, pc.pneg);
-    } else if (kMode == 1) {
+    if (kMode == 1) {
 #pragma unroll
         for (uint32_t k = 0; k < 16; ++k) a[k] = pointwise_mul(csub(a[k], pc.p), b[k], pc);
     } else {
@@ -602,9 +532,7 @@ void col_launch2(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st, co
     const dim3 grid(1u << (P.logS - LOGC), P.K, ba ? (forward ? 2 * count : count) : 1);
     const dim3 block(1u << (LOGC + LOGM - 4));
     const size_t lds = sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << LOGC;
-    if (P.lazy == 2)
-        col_launch3<LOGM, LOGC, 2>(P, X, forward, st, grid, block, lds, ba);
-    else if (P.lazy == 1)
+    if (P.lazy == 1)
         col_launch3<LOGM, LOGC, 1>(P, X, forward, st, grid, block, lds, ba);
     else
         col_launch3<LOGM, LOGC, 0>(P, X, forward, st, grid, block, lds, ba);
@@ -658,9 +586,7 @@ void row_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     const dim3 block(1u << (logG + LOGM - 4));
     const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (sizeof(uint2) << (LOGM - 4 + logG));
-    if (P.lazy == 2)
-        row_launch3<LOGM, 2>(P, X, forward, st, grid, block, lds, logG);
-    else if (P.lazy == 1)
+    if (P.lazy == 1)
         row_launch3<LOGM, 1>(P, X, forward, st, grid, block, lds, logG);
     else
         row_launch3<LOGM, 0>(P, X, forward, st, grid, block, lds, logG);
@@ -674,8 +600,6 @@ void fused_launch_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStr
     const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (sizeof(uint2) << (LOGM - 4 + logG));
     switch (P.lazy * 2 + (P.row_gen ? 1 : 0)) {
-        case 5: launch((k_ntt_rows_fused_batch<LOGM, true, 2>), grid, block, lds, st, P, ba, logG); break;
-        case 4: launch((k_ntt_rows_fused_batch<LOGM, false, 2>), grid, block, lds, st, P, ba, logG); break;
         case 3: launch((k_ntt_rows_fused_batch<LOGM, true, 1>), grid, block, lds, st, P, ba, logG); break;
         case 2: launch((k_ntt_rows_fused_batch<LOGM, false, 1>), grid, block, lds, st, P, ba, logG); break;
         case 1: launch((k_ntt_rows_fused_batch<LOGM, true, 0>), grid, block, lds, st, P, ba, logG); break;
@@ -691,8 +615,6 @@ void fused_launch(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t 
     const size_t lds = (sizeof(uint32_t) * ((1u << LOGM) + (1u << (LOGM - 4))) << logG) +
                        (sizeof(uint2) << (LOGM - 4 + logG));
     switch (P.lazy * 2 + (P.row_gen ? 1 : 0)) {
-        case 5: launch((k_ntt_rows_fused_reg<LOGM, true, 2>), grid, block, lds, st, A, B, P, logG); break;
-        case 4: launch((k_ntt_rows_fused_reg<LOGM, false, 2>), grid, block, lds, st, A, B, P, logG); break;
         case 3: launch((k_ntt_rows_fused_reg<LOGM, true, 1>), grid, block, lds, st, A, B, P, logG); break;
         case 2: launch((k_ntt_rows_fused_reg<LOGM, false, 1>), grid, block, lds, st, A, B, P, logG); break;
         case 1: launch((k_ntt_rows_fused_reg<LOGM, true, 0>), grid, block, lds, st, A, B, P, logG); break;

@@ -2,12 +2,8 @@
 // kernels read. Built once per point, outside any timed region.
 //
 // The prime and K rule is the reference's (rns_plan.h:138-155): primes
-// p < 2^pbits with p = 1 (mod max(2N, 2^16)), taken downward from the largest,
-// until sum log2 p exceeds 2 * bits(q) + log2 N + 1 + 20. The reference's bound
-// is pbits = 31, which gives its K. pbits = 30 runs the same walk below 2^30:
-// about 3 % more primes, but a factor 4 of headroom in a 32-bit word, which the
-// NTT kernels use to keep residues lazily reduced. The product c is the same
-// for either prime set.
+// p < 2^31 with p = 1 (mod max(2N, 2^16)), taken downward from the largest,
+// until sum log2 p exceeds 2 * bits(q) + log2 N + 1 + 20.
 #pragma once
 
 #include <stdint.h>
@@ -20,17 +16,11 @@ namespace pm {
 struct HostPlan {
     uint32_t N = 0, logN = 0, W = 0;
     int L = 0, K = 0, qbits = 0;
-    int pbits = 31;                   // every prime is below 2^pbits
     std::vector<uint32_t> q;          // L limbs
-
-    // q_top: the top three limbs of q as a double (relative to 2^(32 q_drop)),
-    // for the quotient estimate of the final reduction mod q.
-    double q_top = 0;
-    int q_drop = 0;
 
     std::vector<uint32_t> p, pneg;    // K; pneg = -p^-1 mod 2^32
     std::vector<uint32_t> psi, ipsi;  // K: primitive 2N-th roots of unity and inverses
-    std::vector<uint32_t> pow2, pow2_s;    // K x L: 2^(32 l) mod p_j, Shoup companions
+    std::vector<uint32_t> pow2;       // K x L: 2^(32 l) mod p_j
     // K x L: scale_j 2^(32 l) mod p_j, scale_j the pointwise constant below.
     // Decomposing b with these gives b scale_j mod p_j, so in a product the
     // fused row pass's Montgomery product alone yields a b N^-1 (P/p_j)^-1.
@@ -50,14 +40,14 @@ struct HostPlan {
 
 // Throws std::runtime_error with the reference's messages for invalid points
 // (rns_plan.h:120-155).
-HostPlan make_plan(uint32_t N, uint32_t W, const uint32_t* q, int L, int pbits = 31);
+HostPlan make_plan(uint32_t N, uint32_t W, const uint32_t* q, int L);
 
 // Two-pass NTT geometry: rows of S = 2^ceil(logN / 2), columns of R = N / S
 // (one pass, R = 1, when N <= 128).
 void ntt_geometry(uint32_t logN, uint32_t& logS, uint32_t& logR);
 
-// The prime walk alone: K primes for (N, bits(q)) below 2^pbits (30 or 31), or an exception.
-std::vector<uint32_t> choose_primes(uint32_t N, int qbits, int pbits = 31);
+// The prime walk alone: the K primes for (N, bits(q)), or an exception.
+std::vector<uint32_t> choose_primes(uint32_t N, int qbits);
 
 // Twiddles w = psi^bitrev_N(i) (forward) or psi^-bitrev_N(i) (inverse), i < N,
 // for primes [j0, j1) of the plan: forward as Shoup pairs (w, floor(w 2^32 / p)),

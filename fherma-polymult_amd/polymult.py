@@ -24,7 +24,6 @@ PM_OK, PM_EINVAL, PM_EHIP, PM_ENOMEM, PM_EINTERNAL = 0, 1, 2, 3, 4
 FORM_PLAIN, FORM_MFMA = 0, 1
 HOST_STAGED, HOST_MAPPED, HOST_DIRECT = 0, 1, 2
 TWIDDLES_TABLE, TWIDDLES_GEN = 0, 1
-BASIS_AUTO, BASIS_P30, BASIS_P31 = 0, 30, 31
 NPHASES = 7
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -51,11 +50,6 @@ def _load() -> ctypes.CDLL:
         "pm_plan_primes": (ctypes.c_int, [_vp, _u32p, ctypes.c_uint32]),
         "pm_plan_set_form": (ctypes.c_int, [_vp, ctypes.c_int]),
         "pm_choose_primes": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, _u32p, ctypes.c_uint32, _u32p]),
-        "pm_choose_primes_basis": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _u32p,
-                                                  ctypes.c_uint32, _u32p]),
-        "pm_plan_create_basis": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u32p,
-                                                ctypes.c_int, ctypes.POINTER(_vp)]),
-        "pm_plan_basis": (ctypes.c_int, [_vp]),
         "pm_multiply": (ctypes.c_int, [_vp, _u32p, _u32p, _u32p]),
         "pm_plan_set_host_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
         "pm_plan_tune_host": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]),
@@ -110,15 +104,12 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_u32p)
 
 
-def choose_primes(N: int, qbits: int, basis: int = BASIS_P31) -> np.ndarray:
+def choose_primes(N: int, qbits: int) -> np.ndarray:
     """The plan's prime rule on the host alone (no device needed): the
-    reference's primes below 2^31 (BASIS_P31), or the same walk below 2^30."""
+    reference's primes below 2^31 (rns_plan.h:138-155)."""
     buf = np.zeros(256, dtype=np.uint32)
     K = ctypes.c_uint32(0)
-    if basis == BASIS_P31:
-        _check(lib.pm_choose_primes(int(N), int(qbits), _ptr(buf), 256, ctypes.byref(K)))
-    else:
-        _check(lib.pm_choose_primes_basis(int(N), int(qbits), int(basis), _ptr(buf), 256, ctypes.byref(K)))
+    _check(lib.pm_choose_primes(int(N), int(qbits), _ptr(buf), 256, ctypes.byref(K)))
     return buf[:K.value].copy()
 
 
@@ -226,13 +217,10 @@ def device_info():
 class Plan:
     """One point (N, W, q): primes, tables and workspace on the device."""
 
-    def __init__(self, N: int, W: int, q: np.ndarray, form: int | None = None, basis: int = BASIS_AUTO):
+    def __init__(self, N: int, W: int, q: np.ndarray, form: int | None = None):
         q = np.ascontiguousarray(q, dtype=np.uint32)
         h = _vp()
-        if basis == BASIS_AUTO:
-            _check(lib.pm_plan_create(int(N), int(W), int(q.shape[0]), _ptr(q), ctypes.byref(h)))
-        else:
-            _check(lib.pm_plan_create_basis(int(N), int(W), int(q.shape[0]), _ptr(q), int(basis), ctypes.byref(h)))
+        _check(lib.pm_plan_create(int(N), int(W), int(q.shape[0]), _ptr(q), ctypes.byref(h)))
         self._h = h.value
         n, l, k = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
         _check(lib.pm_plan_query(_vp(self._h), ctypes.byref(n), ctypes.byref(l), ctypes.byref(k)))
@@ -248,11 +236,6 @@ class Plan:
     def set_form(self, form: int) -> None:
         _check(lib.pm_plan_set_form(self.handle, int(form)))
         self.form = form
-
-    @property
-    def basis(self) -> int:
-        """BASIS_P30 or BASIS_P31: the bound below which the plan's primes lie."""
-        return lib.pm_plan_basis(self.handle)
 
     @property
     def primes(self) -> np.ndarray:

@@ -47,25 +47,8 @@ const char* pm_last_error(void);
  * p = 1 mod max(2N, 2^16), downward, until sum log2 p > 2 bits(q) + log2 N + 21),
  * every table, and the device workspace. Refuses N outside [2, 2^22] or not a
  * power of two, L outside [1, 64], q whose top limb is zero, and points that
- * need more primes than exist. W is recorded, not checked (as in the reference).
- *
- * Prime bases. PM_BASIS_P31 is the reference's rule above, and the default
- * (PM_BASIS_AUTO, pm_plan_create). PM_BASIS_P30 runs the same walk below 2^30:
- * about 4 % more primes, but 4p < 2^32 lets the NTT kernels keep residues lazily
- * reduced in both directions (8 and 9 instructions per butterfly against 10 and
- * 11 in the default basis, 12 and 11 canonical). The product c is identical in
- * either basis; a point with too few primes below 2^30 is refused.
- * The environment variable PM_BASIS=p30 makes pm_plan_create use PM_BASIS_P30
- * (p31 or auto: the default; any other value makes plan creation fail with
- * PM_EINVAL). */
-typedef enum {
-    PM_BASIS_AUTO = 0,
-    PM_BASIS_P30 = 30,
-    PM_BASIS_P31 = 31
-} pm_basis;
+ * need more primes than exist. W is recorded, not checked (as in the reference). */
 int pm_plan_create(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, pm_plan** out);
-int pm_plan_create_basis(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q, int basis, pm_plan** out);
-int pm_plan_basis(const pm_plan* plan);   /* PM_BASIS_P30 or PM_BASIS_P31, or -1 for a null plan */
 
 /* fherma_free (reference solve.cu:1061-1083). Null is accepted. */
 void pm_plan_destroy(pm_plan* plan);
@@ -77,10 +60,8 @@ int pm_plan_query(const pm_plan* plan, uint32_t* N, uint32_t* L, uint32_t* K);
 int pm_plan_primes(const pm_plan* plan, uint32_t* out, uint32_t cap);
 
 /* The prime rule alone, on the host (no device): the K primes for a point with
- * N coefficients and a q of `qbits` bits, in the reference's basis, or in
- * `basis` (PM_BASIS_P30 or PM_BASIS_P31). */
+ * N coefficients and a q of `qbits` bits (reference rns_plan.h:138-155). */
 int pm_choose_primes(uint32_t N, uint32_t qbits, uint32_t* out, uint32_t cap, uint32_t* K);
-int pm_choose_primes_basis(uint32_t N, uint32_t qbits, int basis, uint32_t* out, uint32_t cap, uint32_t* K);
 
 /* Selects the plain or the matrix-core form for later calls on this plan. */
 int pm_plan_set_form(pm_plan* plan, int form);

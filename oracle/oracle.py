@@ -45,9 +45,6 @@ def lib() -> ctypes.CDLL:
         L = ctypes.CDLL(LIB)
         L.pmo_plan_new.restype = ctypes.c_void_p
         L.pmo_plan_new.argtypes = [ctypes.c_uint32, _u32p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
-        L.pmo_plan_new_bits.restype = ctypes.c_void_p
-        L.pmo_plan_new_bits.argtypes = [ctypes.c_uint32, _u32p, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
-                                        ctypes.c_size_t]
         L.pmo_plan_free.argtypes = [ctypes.c_void_p]
         L.pmo_plan_K.argtypes = [ctypes.c_void_p]
         L.pmo_plan_K.restype = ctypes.c_int
@@ -56,8 +53,6 @@ def lib() -> ctypes.CDLL:
             getattr(L, name).restype = _u32p
         L.pmo_choose_primes.argtypes = [ctypes.c_uint32, ctypes.c_int, _u32p, ctypes.c_int]
         L.pmo_choose_primes.restype = ctypes.c_int
-        L.pmo_choose_primes_bits.argtypes = [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, _u32p, ctypes.c_int]
-        L.pmo_choose_primes_bits.restype = ctypes.c_int
         L.pmo_decompose.argtypes = [ctypes.c_void_p, _u32p, _u32p]
         L.pmo_ntt_forward.argtypes = [ctypes.c_void_p, _u32p]
         L.pmo_ntt_inverse.argtypes = [ctypes.c_void_p, _u32p]
@@ -125,14 +120,12 @@ def digest(arr: np.ndarray) -> str:
 class Plan:
     """The oracle's plan for one point (N, q): rns_plan.h:118-249 restated."""
 
-    def __init__(self, N: int, q_limbs: np.ndarray, pbits: int = 31):
-        """pbits: the prime bound, 31 (the reference's) or 30 (the library's
-        second basis, the same walk below 2^30)."""
+    def __init__(self, N: int, q_limbs: np.ndarray):
         q_limbs = np.ascontiguousarray(q_limbs, dtype=np.uint32)
         err = ctypes.create_string_buffer(256)
         self.N, self.L = int(N), int(q_limbs.shape[0])
         self._q = q_limbs
-        self._h = lib().pmo_plan_new_bits(self.N, _ptr(q_limbs), self.L, int(pbits), err, 256)
+        self._h = lib().pmo_plan_new(self.N, _ptr(q_limbs), self.L, err, 256)
         if not self._h:
             raise ValueError(err.value.decode())
         self.K = int(lib().pmo_plan_K(self._h))
@@ -206,9 +199,9 @@ class Plan:
         return c
 
 
-def choose_primes(N: int, qbits: int, pbits: int = 31) -> np.ndarray:
+def choose_primes(N: int, qbits: int) -> np.ndarray:
     buf = np.zeros(200, dtype=np.uint32)
-    K = lib().pmo_choose_primes_bits(int(N), int(qbits), int(pbits), _ptr(buf), 200)
+    K = lib().pmo_choose_primes(int(N), int(qbits), _ptr(buf), 200)
     if K < 0:
         raise ValueError("no prime set (code %d)" % K)
     return buf[:K].copy()

@@ -190,19 +190,16 @@ static void set_err(char* err, size_t errlen, const char* msg) {
     }
 }
 
-/* The prime set and K for (N, qbits): rns_plan.h:138-155, whose bound is
- * p < 2^31 (pbits = 31); pbits = 30 runs the same walk below 2^30 (the library's
- * second prime basis). Returns K, or -1 if there are not enough primes, -2 if K
- * exceeds `cap`, -3 for a bound other than 30 or 31. */
-int pmo_choose_primes_bits(uint32_t N, int qbits, int pbits, uint32_t* out, int cap) {
-    if (pbits != 30 && pbits != 31) return -3;
+/* The prime set and K for (N, qbits): rns_plan.h:138-155 (primes p < 2^31).
+ * Returns K, or -1 if there are not enough primes, -2 if K exceeds `cap`. */
+int pmo_choose_primes(uint32_t N, int qbits, uint32_t* out, int cap) {
     uint32_t logN = 0;
     while ((1u << logN) < N) ++logN;
     const double need_bits = 2.0 * qbits + logN + 1 + PMO_MARGIN_BITS;
     double have_bits = 0;
     const uint32_t step = 2 * N > (1u << 16) ? 2 * N : (1u << 16);
     int K = 0;
-    for (uint32_t c = ((1u << pbits) - 1) / step * step + 1; have_bits <= need_bits; c -= step) {
+    for (uint32_t c = ((1u << 31) - 1) / step * step + 1; have_bits <= need_bits; c -= step) {
         if (c < step) return -1;
         if (!is_prime_u32(c)) continue;
         if (K >= cap) return -2;
@@ -212,11 +209,7 @@ int pmo_choose_primes_bits(uint32_t N, int qbits, int pbits, uint32_t* out, int 
     return K;
 }
 
-int pmo_choose_primes(uint32_t N, int qbits, uint32_t* out, int cap) {
-    return pmo_choose_primes_bits(N, qbits, 31, out, cap);
-}
-
-pmo_plan* pmo_plan_new_bits(uint32_t N, const uint32_t* q, int L, int pbits, char* err, size_t errlen) {
+pmo_plan* pmo_plan_new(uint32_t N, const uint32_t* q, int L, char* err, size_t errlen) {
     if (N < 2 || (N & (N - 1)) || N > (1u << PMO_MAX_LOGN)) {
         set_err(err, errlen, "N must be a power of two in [2, 2^22]");
         return NULL;
@@ -236,11 +229,9 @@ pmo_plan* pmo_plan_new_bits(uint32_t N, const uint32_t* q, int L, int pbits, cha
         pmo_plan_free(pl);
         return NULL;
     }
-    int K = pmo_choose_primes_bits(N, pl->qbits, pbits, pl->p, PMO_MAX_K);
-    if (K == -1 || K == -3) {
-        set_err(err, errlen, K == -1 ? (pbits == 31 ? "not enough primes p < 2^31 with p = 1 (mod step)"
-                                                    : "not enough primes p < 2^30 with p = 1 (mod step)")
-                                     : "the prime bound must be 2^30 or 2^31");
+    int K = pmo_choose_primes(N, pl->qbits, pl->p, PMO_MAX_K);
+    if (K == -1) {
+        set_err(err, errlen, "not enough primes p < 2^31 with p = 1 (mod step)");
         pmo_plan_free(pl);
         return NULL;
     }
@@ -314,10 +305,6 @@ pmo_plan* pmo_plan_new_bits(uint32_t N, const uint32_t* q, int L, int pbits, cha
         memcpy(pl->negvP + (size_t)v * L, neg, sizeof(uint32_t) * (size_t)L);
     }
     return pl;
-}
-
-pmo_plan* pmo_plan_new(uint32_t N, const uint32_t* q, int L, char* err, size_t errlen) {
-    return pmo_plan_new_bits(N, q, L, 31, err, errlen);
 }
 
 int pmo_plan_K(const pmo_plan* pl) { return pl->K; }

@@ -46,15 +46,13 @@ def test_golden_digests_cover_the_batch():
     assert all(bench.SEED0 + i in gold for i in range(bench.BATCH))
 
 
-@pytest.mark.parametrize("basis,lazy", [(31, "1"), (30, "2")])
-def test_kernel_names_match_the_committed_profiles(basis, lazy):
-    plan = SimpleNamespace(basis=basis, twiddles=1, K=134 if basis == 31 else 139)
+def test_kernel_names_match_the_committed_profiles():
+    plan = SimpleNamespace(twiddles=1, K=134)
     names = bench.phase_kernels(plan)
-    assert names["ntt_rows_fused"] == "k_ntt_rows_fused_reg<8,true,%s>" % lazy
-    if basis == 31:   # the profiles are of the default basis
-        for phase, kernel in names.items():
-            assert bench.measured_traffic(kernel) is not None, (phase, kernel)
-            assert bench.valu_issue(kernel, 0.05, 256) is not None, (phase, kernel)
+    assert names["ntt_rows_fused"] == "k_ntt_rows_fused_reg<8,true,1>"
+    for phase, kernel in names.items():
+        assert bench.measured_traffic(kernel) is not None, (phase, kernel)
+        assert bench.valu_issue(kernel, 0.05, 256) is not None, (phase, kernel)
 
 
 def test_roofline_helpers_read_the_profiles():

@@ -5,10 +5,9 @@ Gentleman-Sande) and against each form's stated output range.
 
 The forms: Shoup twiddle pairs (w, floor(w 2^32 / p)), read from tables, and
 Montgomery pairs (w' = w 2^32 mod p, w' p^-1 mod 2^32), generated in the row
-pass; each canonical, with forward residues in [0, 2p) (primes below 2^31, the
-reference's), or lazily reduced (primes below 2^30). Inputs are drawn over each
-form's whole input range, with its edges, for primes of both bases; a wrong
-range bound, a dropped correction or a sign slip fails here."""
+pass; each canonical, or with forward residues in [0, 2p) (the reference's
+primes, below 2^31). Inputs are drawn over each form's whole input range, with
+its edges; a wrong range bound, a dropped correction or a sign slip fails here."""
 import numpy as np
 import pytest
 
@@ -65,17 +64,6 @@ def ct_half(x, y, w, ws, p):
     return u32(~M + xr), u32(xr + M + p + 1)
 
 
-def ct_lazy(x, y, w, ws, p):
-    nv = u32(mullo(mulhi(y, ws), p) - mullo(y, w))
-    xr = csub(x, 2 * p)
-    return u32(xr - nv), u32(xr + nv + 2 * p)
-
-
-def gs_lazy(x, y, w, ws, p):
-    d = u32(x - y + 2 * p)
-    return csub(u32(x + y), 2 * p), u32(mullo(d, w) - mullo(mulhi(d, ws), p))
-
-
 # ---- Montgomery pairs ------------------------------------------------------
 
 def tw_diff(y, wm, wq, p):
@@ -101,43 +89,29 @@ def ct_half_m(x, y, wm, wq, p):
     return u32(~M + xr), u32(xr + M + p + 1)
 
 
-def ct_lazy_m(x, y, wm, wq, p):
-    nd = u32(mulhi(mullo(y, wq), p) - mulhi(y, wm))
-    xp = u32(csub(x, 2 * p) + p)
-    return u32(xp - nd), u32(xp + nd)
-
-
 def gs_m(x, y, wm, wq, p):
     r = tw_diff_signed(u32(x - y), wm, wq, p)
     return csub(u32(x + y), p), np.minimum(r, u32(r + p))
 
 
-def gs_lazy_m(x, y, wm, wq, p):
-    r = tw_diff_signed(u32(x - y), wm, wq, p)
-    return csub(u32(x + y), 2 * p), u32(r + p)
-
-
-# name: (function, forward, Montgomery pair, x bound, y bound, x' bound, y' bound, prime bits)
-# bounds are multiples of p (exclusive); None for y: any 32-bit word.
+# name: (function, forward, Montgomery pair, x bound, y bound, x' bound, y' bound)
+# bounds are multiples of p (exclusive).
 FORMS = {
-    "ct": (ct, True, False, 1, 1, 1, 1, 31),
-    "gs": (gs, False, False, 1, 1, 1, 1, 31),
-    "ct_half": (ct_half, True, False, 2, 2, 2, 2, 31),
-    "ct_lazy": (ct_lazy, True, False, 4, None, 4, 4, 30),
-    "gs_lazy": (gs_lazy, False, False, 2, 2, 2, 2, 30),
-    "ct_m": (ct_m, True, True, 1, 1, 1, 1, 31),
-    "gs_m": (gs_m, False, True, 1, 1, 1, 1, 31),
-    "ct_half_m": (ct_half_m, True, True, 2, 2, 2, 2, 31),
-    "ct_lazy_m": (ct_lazy_m, True, True, 4, None, 4, 4, 30),
-    "gs_lazy_m": (gs_lazy_m, False, True, 2, 2, 2, 2, 30),
+    "ct": (ct, True, False, 1, 1, 1, 1),
+    "gs": (gs, False, False, 1, 1, 1, 1),
+    "ct_half": (ct_half, True, False, 2, 2, 2, 2),
+    "ct_m": (ct_m, True, True, 1, 1, 1, 1),
+    "gs_m": (gs_m, False, True, 1, 1, 1, 1),
+    "ct_half_m": (ct_half_m, True, True, 2, 2, 2, 2),
 }
 
 
-def primes_for(pbits):
-    ps = list(oracle.choose_primes(65536, 2048, pbits)[:3]) + list(oracle.choose_primes(4096, 256, pbits)[-2:])
-    if pbits == 31:   # forms for p < 2^31 also run with the primes below 2^30
-        ps += list(oracle.choose_primes(65536, 2048, 30)[:2])
-    return [int(p) for p in ps]
+def primes_for():
+    """The largest and smallest primes of the bench point and of the smallest
+    config, plus the smallest prime of the form below 2^30 (a prime far below
+    the top of the word)."""
+    ps = list(oracle.choose_primes(65536, 2048)[:3]) + list(oracle.choose_primes(4096, 256)[-2:])
+    return [int(p) for p in ps] + [1073479681]
 
 
 def draw(rng, bound, n):
@@ -148,14 +122,14 @@ def draw(rng, bound, n):
 
 @pytest.mark.parametrize("name", sorted(FORMS))
 def test_butterfly_form(name):
-    fn, fwd, mont, xb, yb, xob, yob, pbits = FORMS[name]
+    fn, fwd, mont, xb, yb, xob, yob = FORMS[name]
     rng = np.random.default_rng(sum(map(ord, name)))
     n = 20000
-    for p in primes_for(pbits):
-        assert p < 2**pbits
+    for p in primes_for():
+        assert p < 2**31
         P = np.uint64(p)
         x = draw(rng, xb * p, n)
-        y = draw(rng, 2**32 if yb is None else yb * p, n)
+        y = draw(rng, yb * p, n)
         rng.shuffle(y)
         ws_all = np.concatenate([[1, p - 1, 2, (p - 1) // 2], rng.integers(1, p, size=x.size - 4)])
         w = ws_all.astype(np.uint64)
@@ -182,7 +156,7 @@ def test_butterfly_form(name):
 def test_montgomery_pair_bounds():
     """hi(y w') - hi(m p) lies in (-p, p) for every 32-bit y and canonical w'."""
     rng = np.random.default_rng(7)
-    for p in primes_for(31):
+    for p in primes_for():
         y = draw(rng, 2**32, 20000)
         wm = np.concatenate([[0, p - 1, 1], rng.integers(0, p, size=y.size - 3)]).astype(np.uint64)
         wq = mullo(wm, pow(p, -1, 2**32))

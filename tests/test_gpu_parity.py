@@ -18,10 +18,6 @@ polymult = pytest.importorskip("polymult")
 pytestmark = pytest.mark.gpu
 
 FORMS = [("plain", 0), ("mfma", 1)]
-# prime bases: the default (the reference's primes below 2^31; forward
-# residues in [0, 2p) on the register-kernel geometries, N = 2^8 .. 2^16) and
-# the opt-in primes below 2^30 (lazily reduced butterflies both ways)
-BASES = [("default", 0), ("p30", 30)]
 
 
 def say(*args):
@@ -48,11 +44,11 @@ PATTERNS = ["pattern-%s-n%d-w%d" % (p, n, w) for p in ("all_q_minus_1", "zero", 
             for n, w in ((4, 64), (64, 100), (256, 1747), (4096, 256), (32768, 868), (65536, 2048))]
 
 
-def run_case(golden, name, form, basis=0):
+def run_case(golden, name, form):
     case = golden[name]
     q, a, b = pointgen.operands(case)
-    say("case", name, "N", case["N"], "L", case["L"], "form", form, "basis", basis)
-    with polymult.Plan(case["N"], case["W"], q, form=form, basis=basis) as plan:
+    say("case", name, "N", case["N"], "L", case["L"], "form", form)
+    with polymult.Plan(case["N"], case["W"], q, form=form) as plan:
         c = plan.multiply(a, b)
     got = oracle.digest(c)
     if got != case["c_sha"]:
@@ -61,18 +57,16 @@ def run_case(golden, name, form, basis=0):
         pytest.fail("%s: %d of %d coefficients differ, first %s" % (name, bad.size, case["N"], bad[:5]))
 
 
-@pytest.mark.parametrize("basis", [b for _, b in BASES], ids=[n for n, _ in BASES])
 @pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
 @pytest.mark.parametrize("name", SMALL + CONFIGS)
-def test_product_matches_gmp(golden, name, form, basis):
-    run_case(golden, name, form, basis)
+def test_product_matches_gmp(golden, name, form):
+    run_case(golden, name, form)
 
 
-@pytest.mark.parametrize("basis", [b for _, b in BASES], ids=[n for n, _ in BASES])
 @pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
 @pytest.mark.parametrize("name", PATTERNS)
-def test_patterns_match_gmp(golden, name, form, basis):
-    run_case(golden, name, form, basis)
+def test_patterns_match_gmp(golden, name, form):
+    run_case(golden, name, form)
 
 
 # The reference's test-matrix points above N = 2^17 (run_tests.sh:18-21), up to the
@@ -82,54 +76,43 @@ LARGEST = ["matrix-n%d-w%d-%s" % (n, w, kind) for n, w in ((262144, 128), (10485
            for kind in ("random", "prime", "prime-allqm1")]
 
 
-@pytest.mark.parametrize("basis", [b for _, b in BASES], ids=[n for n, _ in BASES])
 @pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
 @pytest.mark.parametrize("name", LARGEST)
-def test_largest_sizes_match_gmp(golden, name, form, basis):
-    run_case(golden, name, form, basis)
+def test_largest_sizes_match_gmp(golden, name, form):
+    run_case(golden, name, form)
 
 
-@pytest.mark.parametrize("basis", [b for _, b in BASES], ids=[n for n, _ in BASES])
 @pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
-def test_whole_test_matrix_matches_gmp(golden, form, basis):
+def test_whole_test_matrix_matches_gmp(golden, form):
     """Every point of the reference's test matrix (run_tests.sh:18-21 and the
     spec's limb edges), each with a random odd q, the spec's prime q and the
-    all-(q-1) operands: 141 products, each against its GMP digest. In the basis
-    below 2^30 a point with too few such primes must be refused cleanly."""
+    all-(q-1) operands: 141 products, each against its GMP digest."""
     names = sorted(n for n in golden if n.startswith("matrix-"))
-    bad, refused = [], []
+    bad = []
     for name in names:
         case = golden[name]
         q, a, b = pointgen.operands(case)
-        say("matrix", name, "form", form, "basis", basis)
-        try:
-            plan = polymult.Plan(case["N"], case["W"], q, form=form, basis=basis)
-        except polymult.PolymulError as e:
-            assert basis == 30 and e.code == polymult.PM_EINVAL, (name, e)
-            refused.append(name)
-            continue
-        with plan:
+        say("matrix", name, "form", form)
+        with polymult.Plan(case["N"], case["W"], q, form=form) as plan:
             if oracle.digest(plan.multiply(a, b)) != case["c_sha"]:
                 bad.append(name)
-    say("refused", refused)
     assert len(names) == 141 and not bad, bad
-    assert len(refused) <= 12, refused
 
 
 @pytest.mark.parametrize("seed", range(24))
 def test_fuzz_matches_oracle(seed):
     """Random points in the spirit of the reference's fuzz runs (fuzz.py): N a
     random power of two up to 2^14, q a random odd integer of 2 .. 2048 bits,
-    a and b uniform below q; the plain form, the matrix-core form and the
-    matrix-core form in the basis below 2^30 against the oracle."""
+    a and b uniform below q; the plain and the matrix-core form against the
+    oracle."""
     rng = random.Random(91000 + seed)
     N, W = 1 << rng.randint(1, 14), rng.randint(2, 2048)
     q, a, b = oracle.gen_point(91000 + seed, N, W)
     say("fuzz", seed, "N", N, "W", W)
     ref = oracle.Plan(N, q).multiply(a, b)
-    for form, basis in ((polymult.FORM_PLAIN, 0), (polymult.FORM_MFMA, 0), (polymult.FORM_MFMA, 30)):
-        with polymult.Plan(N, W, q, form=form, basis=basis) as plan:
-            assert np.array_equal(plan.multiply(a, b), ref), (seed, N, W, form, basis)
+    for form in (polymult.FORM_PLAIN, polymult.FORM_MFMA):
+        with polymult.Plan(N, W, q, form=form) as plan:
+            assert np.array_equal(plan.multiply(a, b), ref), (seed, N, W, form)
 
 
 def _limbs(x, L):
@@ -141,7 +124,7 @@ def test_fuzz_extreme_operands_match_oracle(seed):
     """Random N and q width with the operands at the top of their range: q of
     the form 2^W - 1 or 2^(W-1) + 1 with a = q - 1 and b random (even seeds),
     or a random odd q with a = b = q - 1 (odd seeds), so that every CRT sum
-    sits near its largest value; all three forms against the oracle."""
+    sits near its largest value; both forms against the oracle."""
     rng = random.Random(92000 + seed)
     N, W = 1 << rng.randint(1, 14), rng.randint(2, 2048)
     q, _, _ = oracle.gen_point(92000 + seed, N, W)
@@ -157,38 +140,29 @@ def test_fuzz_extreme_operands_match_oracle(seed):
         b = a.copy()
     say("fuzz-extreme", seed, "N", N, "W", W)
     ref = oracle.Plan(N, q).multiply(a, b)
-    for form, basis in ((polymult.FORM_PLAIN, 0), (polymult.FORM_MFMA, 0), (polymult.FORM_MFMA, 30)):
-        try:
-            with polymult.Plan(N, W, q, form=form, basis=basis) as plan:
-                assert np.array_equal(plan.multiply(a, b), ref), (seed, N, W, form, basis)
-        except polymult.PolymulError as e:   # too few primes below 2^30 for this N and width
-            assert basis == 30 and e.code == polymult.PM_EINVAL, (seed, N, W, str(e))
+    for form in (polymult.FORM_PLAIN, polymult.FORM_MFMA):
+        with polymult.Plan(N, W, q, form=form) as plan:
+            assert np.array_equal(plan.multiply(a, b), ref), (seed, N, W, form)
 
 
 @pytest.mark.parametrize("N,W", [(64, 100), (256, 1747), (4096, 256), (65536, 2048), (131072, 512)])
-def test_basis_choice(N, W):
-    """The default plan takes the reference's primes (the reference's K); an
-    explicit basis is kept."""
+def test_plan_primes(N, W):
+    """A plan takes the reference's primes (the reference's K)."""
     q, _, _ = oracle.gen_point(11, N, W)
     with polymult.Plan(N, W, q) as plan:
-        assert plan.basis == 31
         assert np.array_equal(plan.primes, oracle.choose_primes(N, _qbits(q)))
-    for basis in (polymult.BASIS_P30, polymult.BASIS_P31):
-        with polymult.Plan(N, W, q, basis=basis) as plan:
-            assert plan.basis == basis and plan.K == len(oracle.choose_primes(N, _qbits(q), basis))
 
 
-@pytest.mark.parametrize("pbits", [30, 31])
 @pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
 @pytest.mark.parametrize("N,W,seed", [(64, 100, 3), (4096, 256, 5), (8192, 868, 6), (32768, 868, 7),
                                       (65536, 2048, 8)])
-def test_stages_match_oracle(N, W, seed, form, pbits):
-    """Each of the five steps, device to device, against the oracle's step, in
-    both prime bases (the stage entry points leave canonical residues)."""
-    say("stages", N, W, seed, "form", form, "pbits", pbits)
+def test_stages_match_oracle(N, W, seed, form):
+    """Each of the five steps, device to device, against the oracle's step (the
+    stage entry points leave canonical residues)."""
+    say("stages", N, W, seed, "form", form)
     q, a, b = oracle.gen_point(seed, N, W)
-    ref = oracle.Plan(N, q, pbits)
-    with polymult.Plan(N, W, q, form=form, basis=pbits) as plan:
+    ref = oracle.Plan(N, q)
+    with polymult.Plan(N, W, q, form=form) as plan:
         assert plan.K == ref.K and np.array_equal(plan.primes, ref.primes)
         K, L = plan.K, plan.L
         da, db = polymult.DeviceBuffer.from_array(a), polymult.DeviceBuffer.from_array(b)
@@ -280,9 +254,8 @@ def _qbits(q):
     return int(oracle.int_of(q)).bit_length()
 
 
-@pytest.mark.parametrize("basis", [b for _, b in BASES], ids=[n for n, _ in BASES])
 @pytest.mark.parametrize("N,W", [(4096, 256), (32768, 868), (65536, 2048)])
-def test_phase_batched_graph_matches_gmp(golden, N, W, basis):
+def test_phase_batched_graph_matches_gmp(golden, N, W):
     """Products that share N, L and the primes are captured as five phase-batched
     launches; two replays, eager products in between, every answer vs GMP."""
     cases = sorted((c for c in golden.values() if c["N"] == N and c["W"] == W), key=lambda c: c["name"])
@@ -291,7 +264,7 @@ def test_phase_batched_graph_matches_gmp(golden, N, W, basis):
         q, a, b = pointgen.operands(case)
         if _qbits(q) != W:   # other q sizes have other primes: not one batch
             continue
-        plans.append(polymult.Plan(N, W, q, form=polymult.FORM_MFMA, basis=basis))
+        plans.append(polymult.Plan(N, W, q, form=polymult.FORM_MFMA))
         bufs.append((polymult.DeviceBuffer.from_array(a), polymult.DeviceBuffer.from_array(b),
                      polymult.DeviceBuffer(a.nbytes)))
         keep.append(case)
@@ -356,12 +329,12 @@ def test_row_twiddle_sources_match_gmp(golden, name):
     give the GMP product (the reference's fuzz runs tw-gen and tw-table)."""
     case = golden[name]
     q, a, b = pointgen.operands(case)
-    for form, basis in ((polymult.FORM_PLAIN, 0), (polymult.FORM_MFMA, 0), (polymult.FORM_MFMA, 30)):
-        with polymult.Plan(case["N"], case["W"], q, form=form, basis=basis) as plan:
+    for form in (polymult.FORM_PLAIN, polymult.FORM_MFMA):
+        with polymult.Plan(case["N"], case["W"], q, form=form) as plan:
             for source in (polymult.TWIDDLES_TABLE, polymult.TWIDDLES_GEN):
                 plan.set_twiddles(source)
                 assert plan.twiddles == source
-                say("twiddles", name, "form", form, "basis", plan.basis, "source", source)
+                say("twiddles", name, "form", form, "source", source)
                 assert oracle.digest(plan.multiply(a, b)) == case["c_sha"], (name, form, source)
 
 

@@ -116,13 +116,12 @@ def test_closed_forms_against_brute_force():
             assert oracle.py_negacyclic(pa, pb, q) == pc, (N, name)
 
 
-@pytest.mark.parametrize("pbits,Ks", [(31, (18, 58, 115, 134)), (30, (19, 60, 119, 139))])
-def test_prime_rule(pbits, Ks):
-    """The reference's walk (pbits 31) and the same walk below 2^30: every
-    candidate p = 1 (mod step) below the bound is tested downward, none skipped,
-    until the bits suffice."""
-    for (N, W), K in zip(((4096, 256), (32768, 868), (65536, 1747), (65536, 2048)), Ks):
-        p = oracle.choose_primes(N, W, pbits)
+def test_prime_rule():
+    """The reference's walk: every candidate p = 1 (mod step) below 2^31 is
+    tested downward, none skipped, until the bits suffice."""
+    pbits = 31
+    for (N, W), K in zip(((4096, 256), (32768, 868), (65536, 1747), (65536, 2048)), (18, 58, 115, 134)):
+        p = oracle.choose_primes(N, W)
         assert len(p) == K
         step = max(2 * N, 1 << 16)
         assert all(sympy.isprime(int(x)) and int(x) % step == 1 and x < 2 ** pbits for x in p)
