@@ -212,6 +212,7 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q)
     for (auto& e : pl->ev_stop) check(hipEventCreate(&e), "hipEventCreate");
     pl->ensure_mfma();
     pl->form = PM_FORM_MFMA;
+    (void)pm::scratch_on_current_device();   // set-up of the allocation-free entry point on this device
     return pl.release();
 }
 
@@ -272,49 +273,60 @@ const char* const kPhaseNames[PM_NPHASES] = {"decompose_a", "decompose_b", "ntt_
 
 namespace pm {
 
-// Phase i (0 .. PM_NPHASES - 1) of one product on device pointers, enqueued on st.
-void run_phase(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st, int i) {
+// Phase i (0 .. PM_NPHASES - 1) of one product on device pointers, enqueued on
+// st, with the residues of a and b in A and B (K x N words each).
+void run_phase_in(pm_plan* pl, uint32_t* A, uint32_t* B, const uint32_t* a, const uint32_t* b, uint32_t* c,
+                  hipStream_t st, int i) {
     const pm::DevPlan& P = pl->dev;
     const bool mfma = pl->form == PM_FORM_MFMA;
     switch (i) {
-        case 0: if (mfma) launch_decompose_mfma(P, a, pl->A, st, true); else launch_decompose_valu(P, a, pl->A, st); break;
+        case 0: if (mfma) launch_decompose_mfma(P, a, A, st, true); else launch_decompose_valu(P, a, A, st); break;
         case 1:   // b scaled by the pointwise constant (b_side)
-            if (mfma) launch_decompose_mfma(b_side(P), b, pl->B, st, true); else launch_decompose_valu(b_side(P), b, pl->B, st);
+            if (mfma) launch_decompose_mfma(b_side(P), b, B, st, true); else launch_decompose_valu(b_side(P), b, B, st);
             break;
-        case 2: launch_ntt_cols(P, pl->A, true, st); break;
-        case 3: launch_ntt_cols(P, pl->B, true, st); break;
-        case 4: launch_ntt_rows_fused(P, pl->A, pl->B, st); break;
-        case 5: launch_ntt_cols(P, pl->A, false, st); break;
-        case 6: if (mfma) launch_crt_mfma(P, pl->A, c, st); else launch_crt_valu(P, pl->A, c, st); break;
+        case 2: launch_ntt_cols(P, A, true, st); break;
+        case 3: launch_ntt_cols(P, B, true, st); break;
+        case 4: launch_ntt_rows_fused(P, A, B, st); break;
+        case 5: launch_ntt_cols(P, A, false, st); break;
+        case 6: if (mfma) launch_crt_mfma(P, A, c, st); else launch_crt_valu(P, A, c, st); break;
         default: throw std::logic_error("no such phase");
     }
 }
 
-// The phases of one product on device pointers, enqueued on st. Where the
-// register kernels apply, the decompositions of a and b (matrix-core form) and
-// the forward column passes of A and B go out as one launch each: one pass
-// alone can leave a partly filled last round of workgroups (8 K column blocks
-// at N = 65536 against 1024 resident workgroups), which two passes in one
-// launch share.
-void run_device(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st) {
+void run_phase(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st, int i) {
+    run_phase_in(pl, pl->A, pl->B, a, b, c, st, i);
+}
+
+// The phases of one product on device pointers, enqueued on st, working in A
+// and B. Where the register kernels apply, the decompositions of a and b
+// (matrix-core form) and the forward column passes of A and B go out as one
+// launch each: one pass alone can leave a partly filled last round of
+// workgroups (8 K column blocks at N = 65536 against 1024 resident
+// workgroups), which two passes in one launch share.
+void run_device_in(pm_plan* pl, uint32_t* A, uint32_t* B, const uint32_t* a, const uint32_t* b, uint32_t* c,
+                   hipStream_t st) {
     if (!batch_kernels_ok(pl->dev)) {
-        for (int i = 0; i < PM_NPHASES; ++i) run_phase(pl, a, b, c, st, i);
+        for (int i = 0; i < PM_NPHASES; ++i) run_phase_in(pl, A, B, a, b, c, st, i);
     } else {
         BatchArgs ba{};
         ba.a[0] = a;
         ba.b[0] = b;
-        ba.A[0] = pl->A;
-        ba.B[0] = pl->B;
+        ba.A[0] = A;
+        ba.B[0] = B;
         if (pl->form == PM_FORM_MFMA) {
             launch_decompose_mfma_batch(pl->dev, ba, 1, st);   // b with the b-side tables
         } else {
-            run_phase(pl, a, b, c, st, 0);
-            run_phase(pl, a, b, c, st, 1);
+            run_phase_in(pl, A, B, a, b, c, st, 0);
+            run_phase_in(pl, A, B, a, b, c, st, 1);
         }
         launch_ntt_cols_batch(pl->dev, ba, 1, true, st);
-        for (int i = 4; i < PM_NPHASES; ++i) run_phase(pl, a, b, c, st, i);
+        for (int i = 4; i < PM_NPHASES; ++i) run_phase_in(pl, A, B, a, b, c, st, i);
     }
     check(hipGetLastError(), "kernel launch");
+}
+
+void run_device(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st) {
+    run_device_in(pl, pl->A, pl->B, a, b, c, st);
 }
 
 }  // namespace pm
@@ -400,6 +412,21 @@ int pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint
     return guarded([&] {
         need(plan && a && b && c, "null argument");
         run_device(plan, a, b, c, stream_of(plan, stream));
+    });
+}
+
+int pm_multiply_device_noalloc(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, void* stream) {
+    return guarded([&] {
+        need(plan && a && b && c, "null argument");
+        // A, then B at the next 256-byte boundary (as separate allocations would be)
+        const size_t kn = plan->residues(), b_at = (kn + 63) / 64 * 64;
+        if (b_at + kn > pm::kScratchWords) {
+            throw std::invalid_argument("pm_multiply_device_noalloc: the product needs " + std::to_string(b_at + kn) +
+                                        " words of residue workspace (2 K N); the static workspace holds " +
+                                        std::to_string(pm::kScratchWords));
+        }
+        uint32_t* A = pm::scratch_on_current_device();
+        pm::run_device_in(plan, A, A + b_at, a, b, c, stream_of(plan, stream));
     });
 }
 

@@ -94,6 +94,16 @@ namespace pm {
 // columns, CRT. run_phase enqueues phase i alone.
 void run_device(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st);
 void run_phase(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, hipStream_t st, int i);
+// The same with the residues of a and b in caller-chosen K x N arrays A and B.
+void run_device_in(pm_plan* pl, uint32_t* A, uint32_t* B, const uint32_t* a, const uint32_t* b, uint32_t* c,
+                   hipStream_t st);
+
+// Words of the allocation-free entry point's device workspace (pm_scratch.hip):
+// 72 MiB, the two K x N residue arrays of any product up to N = 65536 with a
+// 2048-bit q (2 x 134 x 65536 words).
+constexpr size_t kScratchWords = size_t(72) << 18;
+// Its device address on the current device, looked up once per device.
+uint32_t* scratch_on_current_device();
 
 // fherma_run on host pointers through the plan's pipeline (pm_host.cpp).
 void host_multiply(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c);

@@ -54,6 +54,7 @@ def _load() -> ctypes.CDLL:
         "pm_plan_set_host_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
         "pm_plan_tune_host": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]),
         "pm_multiply_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+        "pm_multiply_device_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "pm_batch_multiply_device": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                                     ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
         "pm_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
@@ -288,6 +289,11 @@ class Plan:
 
     def multiply_device(self, a: DeviceBuffer, b: DeviceBuffer, c: DeviceBuffer, stream=None) -> None:
         _check(lib.pm_multiply_device(self.handle, *self._abc(a, b, c), _sv(stream)))
+
+    def multiply_device_noalloc(self, a: DeviceBuffer, b: DeviceBuffer, c: DeviceBuffer, stream=None) -> None:
+        """multiply_device with the residues in the library's static workspace:
+        no allocation in the call (pm_multiply_device_noalloc)."""
+        _check(lib.pm_multiply_device_noalloc(self.handle, *self._abc(a, b, c), _sv(stream)))
 
     # the five steps, device to device
     def stage_decompose(self, limbs: DeviceBuffer, res: DeviceBuffer, stream=None) -> None:

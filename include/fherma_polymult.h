@@ -86,6 +86,19 @@ int pm_plan_get_twiddles(const pm_plan* plan);   /* pm_twiddles, or -1 for a nul
 int pm_multiply(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c);
 int pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, void* stream);
 
+/* pm_multiply_device without any allocation: no counterpart in the reference.
+ * It is for callers that cannot allocate device memory at call time (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same kernels run with the two K x N residue arrays in a static device
+ * workspace of 72 MiB per GPU (held until the process ends), so the product
+ * must need 2 K N <= 18874368 words: every point up to N = 65536 with a q of
+ * up to 2048 bits; larger ones fail with PM_EINVAL before any launch. The plan
+ * (created beforehand, which allocates) supplies only its read-only tables.
+ * Every call on one GPU shares that workspace: calls that overlap on a GPU
+ * (two streams, two threads, two plans) race and must be ordered by the
+ * caller. Nothing in the workspace needs clearing between calls. */
+int pm_multiply_device_noalloc(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, void* stream);
+
 /* How pm_multiply moves host operands (reference solve.cu:14-34, 282-296):
  *   PM_HOST_STAGED  chunked copies through pinned staging (parallel host memcpy,
  *                   hipMemcpyAsync), overlapped with the decompose and CRT chunks
