@@ -17,6 +17,7 @@ import argparse
 import hashlib
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -164,11 +165,22 @@ def valu_issue(kernel, kernel_ms, cus):
     return None
 
 
+def host_cpus():
+    """(CPUs this process may run on, CPUs the host has)."""
+    try:
+        mine = len(os.sched_getaffinity(0))
+    except AttributeError:
+        mine = os.cpu_count() or 1
+    return mine, os.cpu_count() or 1
+
+
 def cpu_baseline(points, budget_s=12.0):
-    """The reference's CPU path (oracle/_ref) on the box's cores, else the oracle port."""
+    """The reference's CPU path (oracle/_ref) on the host cores this process may
+    use (OMP_NUM_THREADS where the environment sets a share), else the oracle port."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # noqa: E402  (the checker's CPU port, used only for this leg)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    mine, total = host_cpus()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or mine
     oracle.set_threads(threads)
     q, a, b = points[0]
     if oracle.ref_available():
@@ -186,7 +198,9 @@ def cpu_baseline(points, budget_s=12.0):
         if el > budget_s or done >= 64:
             break
     return {"value": done / el, "unit": "polymuls/s", "cores": threads, "kind": kind,
-            "sample": "%d products at N=65536, W=2048 (K=134), %.1f s, OMP threads=%d" % (done, el, threads)}
+            "host_cpus": total, "cpus_available": mine,
+            "sample": "%d products at N=65536, W=2048 (K=134), %.1f s, OMP threads=%d of %d CPUs available "
+                      "to this process (%d on the host)" % (done, el, threads, mine, total)}
 
 
 def batch_rates(args, plans, da, db, dc, gold, log):
@@ -285,11 +299,42 @@ def uneven_rate(args, log):
 REFERENCE_POINTS = [("point1", "matrix-n65536-w1747-prime", 2.30), ("point2", "matrix-n32768-w868-prime", 0.605)]
 
 
+def driver_times(n, w, q, a, b, c_sha, reps, log):
+    """fherma_init / fherma_run / fherma_free through the harness driver
+    (fherma-polymult_amd/solution) on a point directory of `reps` cases, as the
+    reference's main.cpp times them (main.cpp:180-186): (init seconds, the
+    per-case seconds of fherma_run alone, cases whose answer differs from GMP)."""
+    import tempfile
+    solution = os.path.join(ROOT, "fherma-polymult_amd", "solution")
+    with tempfile.TemporaryDirectory(prefix="pm_point_") as root:
+        os.makedirs(os.path.join(root, "point"))
+        q.astype("<u4").tofile(os.path.join(root, "point", "q.bin"))
+        with open(os.path.join(root, "manifest.json"), "w") as f:
+            json.dump({"N": n, "W": w, "L": q.shape[0], "cases": reps}, f)
+        for i in range(reps):
+            d = os.path.join(root, "cases", "%06d" % i)
+            os.makedirs(d)
+            a.astype("<u4").tofile(os.path.join(d, "a.bin"))
+            b.astype("<u4").tofile(os.path.join(d, "b.bin"))
+        r = subprocess.run([solution, root], capture_output=True, text=True, timeout=300)
+        if r.returncode != 0:
+            raise RuntimeError("driver failed: " + r.stderr[-2000:])
+        with open(os.path.join(root, "out", "results.json")) as f:
+            res = json.load(f)
+        bad = sum(digest(np.fromfile(os.path.join(root, "out", "%06d" % i, "c.bin"), dtype="<u4")) != c_sha
+                  for i in range(reps))
+        if any(c["status"] != "ok" for c in res["cases"]):
+            raise RuntimeError("driver: a case crashed")
+        return res["init_s"], [c["seconds"] for c in res["cases"]], bad
+
+
 def reference_points(log, reps=11):
-    """The reference's measured points with their golden q: the median end-to-end
-    time of one product from host numpy arrays (fherma_run's path, PCIe included)
-    and the mean device-resident time of one product (no batching), each product
-    checked against its GMP digest."""
+    """The reference's measured points with their golden q: the median time of
+    fherma_run per product through the harness driver (fherma_run's own path:
+    host vectors in, the answer's vector sized and filled, PCIe included; the
+    host modes from fherma_init's timed choice), the same through pm_multiply on
+    numpy arrays, and the mean device-resident time of one product (no
+    batching); every answer checked against its GMP digest."""
     import polymult  # noqa: E402
     path = os.path.join(ROOT, "tests", "golden", "golden.json")
     with open(path) as f:
@@ -301,9 +346,10 @@ def reference_points(log, reps=11):
         qi = int(c["q_hex"], 16)
         q = np.array([(qi >> (32 * l)) & 0xFFFFFFFF for l in range(L)], dtype=np.uint32)
         _, a, b = make_operands(c["seed"], n, w, q)
+        init_s, secs, mism = driver_times(n, w, q, a, b, c["c_sha"], reps, log)
         plan = polymult.Plan(n, w, q, form=polymult.FORM_MFMA)
         mode, _ = plan.tune_host()
-        mism = int(digest(plan.multiply(a, b)) != c["c_sha"])
+        mism += int(digest(plan.multiply(a, b)) != c["c_sha"])
         host = []
         for _ in range(reps):
             t0 = time.perf_counter()
@@ -320,18 +366,48 @@ def reference_points(log, reps=11):
             plan.multiply_device(da, db, dc)
         polymult.synchronize()
         dev = (time.perf_counter() - t0) / iters
+        names = {polymult.HOST_STAGED: "staged", polymult.HOST_MAPPED: "mapped", polymult.HOST_DIRECT: "direct"}
         out[key] = {"N": n, "W": w, "K": plan.K, "q": "the spec's prime", "mismatches": mism,
-                    "end_to_end_ms": round(float(np.median(host)) * 1e3, 4),
-                    "host_mode": {polymult.HOST_STAGED: "staged", polymult.HOST_MAPPED: "mapped",
-                                  polymult.HOST_DIRECT: "direct"}.get(mode, str(mode)),
+                    "fherma_run_ms": round(float(np.median(secs)) * 1e3, 4),
+                    "fherma_init_s": round(init_s, 3),
+                    "pm_multiply_ms": round(float(np.median(host)) * 1e3, 4),
+                    "host_modes": "%s in, %s out" % tuple(names.get(m, str(m)) for m in plan.host_modes),
                     "device_resident_ms": round(dev * 1e3, 4),
-                    "reference_end_to_end_ms_l40s": ref_ms}
-        log("%s (N=%d, W=%d): end to end %.3f ms (%s), device-resident %.3f ms, %d mismatches"
-            % (key, n, w, out[key]["end_to_end_ms"], out[key]["host_mode"], dev * 1e3, mism))
+                    "reference_end_to_end_ms_l40s": ref_ms,
+                    "note": "fherma_run_ms: the driver's per-case seconds around fherma_run (median of %d), "
+                            "with the answer's vector sized inside the call" % reps}
+        log("%s (N=%d, W=%d): fherma_run %.3f ms (driver, median of %d), pm_multiply %.3f ms (%s), "
+            "device-resident %.3f ms, %d mismatches"
+            % (key, n, w, out[key]["fherma_run_ms"], reps, out[key]["pm_multiply_ms"], out[key]["host_modes"],
+               dev * 1e3, mism))
         for x in (da, db, dc):
             x.close()
         plan.close()
     return out
+
+
+def issue_probe(log):
+    """Cycles per wave64 instruction per SIMD for each probe kind of
+    pm_probe_issue (fherma-polymult_amd/csrc/pm_probe.hip), at the device's
+    reported peak clock: one table line per kind."""
+    import polymult  # noqa: E402
+    name, cus = polymult.device_info()
+    simds = 4 * cus
+    rows = []
+    for kind, iters in ((0, 4096), (1, 4096), (2, 4096), (3, 2048), (4, 2048), (5, 4096), (6, 4096),
+                        (7, 2048), (8, 2048), (9, 1024)):
+        r = polymult.probe_issue(kind, iters)
+        cycles = r["ms"] * 1e-3 * r["clock_mhz"] * 1e6
+        n = r["valu_per_simd"] + r["mfma_per_simd"]
+        r.update({"cycles_per_simd": round(cycles), "simds": simds,
+                  "cycles_per_valu": round(cycles / r["valu_per_simd"], 3) if r["valu_per_simd"] and
+                  not r["mfma_per_simd"] else None,
+                  "cycles_per_mfma": round(cycles / r["mfma_per_simd"], 3) if r["mfma_per_simd"] and
+                  not r["valu_per_simd"] else None})
+        log("%-32s %8.3f ms  %12.0f VALU  %10.0f MFMA per SIMD  %10d cycles at %.0f MHz" %
+            (r["kind"], r["ms"], r["valu_per_simd"], r["mfma_per_simd"], r["cycles_per_simd"], r["clock_mhz"]))
+        rows.append(r)
+    return {"device": name, "cus": cus, "probes": rows}
 
 
 def main():
@@ -352,6 +428,8 @@ def main():
     ap.add_argument("--copy-yardstick", action="store_true",
                     help="first run three 256 MiB device copies (the known-size kernel that scales the "
                          "HBM traffic counters)")
+    ap.add_argument("--issue-probe", action="store_true",
+                    help="only the issue-rate probes (cycles per instruction kind on each SIMD), as JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the products of a batch are "
@@ -362,6 +440,10 @@ def main():
 
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
+
+    if args.issue_probe:
+        print(json.dumps(issue_probe(log)), flush=True)
+        return
 
     if args.copy_yardstick:
         src, dst = polymult.DeviceBuffer(256 << 20), polymult.DeviceBuffer(256 << 20)
@@ -400,9 +482,10 @@ def main():
     # the timed choice of the host mode on the first plan. Never the headline value.
     host = None
     if not (args.no_host or args.phases_only):
-        mode, _ = plans[0].tune_host()
+        plans[0].tune_host()
+        modes = plans[0].host_modes
         for p in plans[1:]:
-            p.set_host_mode(mode)
+            p.set_host_modes(*modes)
         for p, (_, a, b) in zip(plans, points):
             p.multiply(a, b)
         t0 = time.perf_counter()
@@ -410,9 +493,9 @@ def main():
             for p, (_, a, b) in zip(plans, points):
                 p.multiply(a, b)
         host_sec = (time.perf_counter() - t0) / (2 * BATCH)
+        names = {polymult.HOST_STAGED: "staged", polymult.HOST_MAPPED: "mapped", polymult.HOST_DIRECT: "direct"}
         host = {"value": round(1.0 / host_sec, 2), "unit": "polymuls/s",
-                "mode": {polymult.HOST_STAGED: "staged", polymult.HOST_MAPPED: "mapped",
-                         polymult.HOST_DIRECT: "direct"}.get(mode, str(mode)),
+                "mode": "%s in, %s out" % tuple(names.get(m, str(m)) for m in modes),
                 "note": "numpy arrays in and out through the plan's host pipeline, products one after another"}
         log("host buffers: %.3f ms per product (%s)" % (host_sec * 1e3, host["mode"]))
 

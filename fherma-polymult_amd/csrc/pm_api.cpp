@@ -437,11 +437,23 @@ int pm_multiply(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c
     });
 }
 
-int pm_plan_set_host_mode(pm_plan* plan, int mode) {
+int pm_plan_set_host_mode(pm_plan* plan, int mode) { return pm_plan_set_host_modes(plan, mode, mode); }
+
+int pm_plan_set_host_modes(pm_plan* plan, int in_mode, int out_mode) {
     return guarded([&] {
         need(plan != nullptr, "plan is null");
-        need(mode >= PM_HOST_STAGED && mode <= PM_HOST_DIRECT, "unknown host mode");
-        pm::host_set_mode(plan, mode);
+        need(in_mode >= PM_HOST_STAGED && in_mode <= PM_HOST_DIRECT && out_mode >= PM_HOST_STAGED &&
+                 out_mode <= PM_HOST_DIRECT,
+             "unknown host mode");
+        pm::host_set_modes(plan, in_mode, out_mode);
+    });
+}
+
+int pm_plan_get_host_modes(const pm_plan* plan, int* in_mode, int* out_mode) {
+    return guarded([&] {
+        need(plan != nullptr, "plan is null");
+        if (in_mode) *in_mode = pm::host_get_mode(plan);
+        if (out_mode) *out_mode = pm::host_get_out_mode(plan);
     });
 }
 
@@ -708,6 +720,13 @@ int pm_device_synchronize(void) {
     return guarded([&] { check(hipDeviceSynchronize(), "hipDeviceSynchronize"); });
 }
 
+int pm_probe_issue(int kind, int iters, double* ms, double* valu_per_simd, double* mfma_per_simd, double* clock_mhz) {
+    return guarded([&] {
+        need(ms && valu_per_simd && mfma_per_simd && clock_mhz && iters > 0, "bad argument");
+        pm::probe_issue(kind, iters, ms, valu_per_simd, mfma_per_simd, clock_mhz);
+    });
+}
+
 int pm_device_info(char* name, size_t len, int* compute_units) {
     return guarded([&] {
         int dev = 0;
@@ -730,20 +749,12 @@ void* fherma_init(const fherma::Point& p) {
     if (p.q.data.size() != p.L) throw std::runtime_error("q must have L limbs");
     pm_plan* plan = nullptr;
     if (pm_plan_create(p.N, p.W, p.L, p.q.data.data(), &plan) != PM_OK) throw std::runtime_error(g_err);
-    // Like the reference's init (solve.cu:817-862): time the host modes on dummy
-    // operands and keep the fastest, unless PM_HOST_MODE or PM_AUTOTUNE=0 says otherwise.
-    const char* mode = std::getenv("PM_HOST_MODE");
-    const char* tune = std::getenv("PM_AUTOTUNE");
-    int rc = PM_OK;
-    if (mode) {
-        const int m = !std::strcmp(mode, "staged") ? PM_HOST_STAGED
-                      : !std::strcmp(mode, "mapped") ? PM_HOST_MAPPED
-                      : !std::strcmp(mode, "direct") ? PM_HOST_DIRECT : -1;
-        rc = m < 0 ? (g_err = "PM_HOST_MODE must be staged, mapped or direct", PM_EINVAL)
-                   : pm_plan_set_host_mode(plan, m);
-    } else if (!(tune && !std::strcmp(tune, "0"))) {
-        rc = pm_plan_tune_host(plan, nullptr, nullptr);
-    }
+    // Like the reference's init (solve.cu:710-869): the host knobs of the
+    // environment, then, unless they name a mode or FHERMA_AUTOTUNE=0, the timed
+    // choice of the host modes on dummy operands (solve.cu:817-862).
+    const int rc = guarded([&] {
+        if (pm::host_configure_from_env(plan)) pm::host_autotune(plan);
+    });
     if (rc != PM_OK) {
         const std::string msg = g_err;
         pm_plan_destroy(plan);
@@ -761,10 +772,13 @@ fherma::Outputs fherma_run(void* state, const fherma::Inputs& in) {
     }
     fherma::Outputs out;
     out.c.shape = {static_cast<int64_t>(plan->host.N), static_cast<int64_t>(plan->host.L)};
-    out.c.data.resize(words);
-    if (pm_multiply(plan, in.a.data.data(), in.b.data.data(), out.c.data.data()) != PM_OK) {
-        throw std::runtime_error(g_err);
-    }
+    // The answer is sized (and zero-filled, as std::vector does) on the host
+    // pipeline's helper thread while the inputs stream (solve.cu:907-918).
+    const int rc = guarded([&] {
+        pm::host_multiply_sized(plan, in.a.data.data(), in.b.data.data(), [&out, words] { out.c.data.resize(words); },
+                                [&out] { return out.c.data.data(); });
+    });
+    if (rc != PM_OK) throw std::runtime_error(g_err);
     return out;
 }
 

@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -98,6 +99,10 @@ void run_phase(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c, h
 void run_device_in(pm_plan* pl, uint32_t* A, uint32_t* B, const uint32_t* a, const uint32_t* b, uint32_t* c,
                    hipStream_t st);
 
+// Issue-rate probe (pm_probe.hip): wall time of probe `kind` and the
+// instructions each SIMD issued.
+void probe_issue(int kind, int iters, double* ms, double* valu_per_simd, double* mfma_per_simd, double* clock_mhz);
+
 // Words of the allocation-free entry point's device workspace (pm_scratch.hip):
 // 72 MiB, the two K x N residue arrays of any product up to N = 65536 with a
 // 2048-bit q (2 x 134 x 65536 words).
@@ -105,12 +110,23 @@ constexpr size_t kScratchWords = size_t(72) << 18;
 // Its device address on the current device, looked up once per device.
 uint32_t* scratch_on_current_device();
 
-// fherma_run on host pointers through the plan's pipeline (pm_host.cpp).
+// A product on host pointers through the plan's pipeline (pm_host.cpp).
 void host_multiply(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c);
-// Times every host mode on dummy operands and keeps the fastest; returns its
-// median milliseconds per product (pm_host.cpp).
+// The same with the answer's memory supplied late (fherma_run): size_out runs
+// on the pipeline's helper thread while the inputs stream, and get_out, called
+// after it has finished, gives the memory just before the first answer word is
+// written.
+void host_multiply_sized(pm_plan* pl, const uint32_t* a, const uint32_t* b, std::function<void()> size_out,
+                         std::function<uint32_t*()> get_out);
+// Times the host modes on dummy operands and keeps the fastest; returns its
+// median milliseconds per product.
 double host_autotune(pm_plan* pl);
-void host_set_mode(pm_plan* pl, int mode);
-int host_get_mode(const pm_plan* pl);
+void host_set_modes(pm_plan* pl, int in_mode, int out_mode);
+int host_get_mode(const pm_plan* pl);       // the input side's mode
+int host_get_out_mode(const pm_plan* pl);
+// The host knobs of the environment (FHERMA_IN, FHERMA_OUT, FHERMA_CHUNKS, ...,
+// PM_HOST_MODE); returns whether fherma_init should run the timed choice
+// (not when a mode is given or FHERMA_AUTOTUNE / PM_AUTOTUNE is 0).
+bool host_configure_from_env(pm_plan* pl);
 
 }  // namespace pm

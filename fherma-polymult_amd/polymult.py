@@ -52,6 +52,8 @@ def _load() -> ctypes.CDLL:
         "pm_choose_primes": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, _u32p, ctypes.c_uint32, _u32p]),
         "pm_multiply": (ctypes.c_int, [_vp, _u32p, _u32p, _u32p]),
         "pm_plan_set_host_mode": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "pm_plan_set_host_modes": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+        "pm_plan_get_host_modes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "pm_plan_tune_host": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]),
         "pm_multiply_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "pm_multiply_device_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
@@ -83,6 +85,7 @@ def _load() -> ctypes.CDLL:
         "pm_stream_synchronize": (ctypes.c_int, [_vp]),
         "pm_device_synchronize": (ctypes.c_int, []),
         "pm_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)]),
+        "pm_probe_issue": (ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [ctypes.POINTER(ctypes.c_double)] * 4),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -212,6 +215,20 @@ def device_info():
     return name.value.decode(), cus.value
 
 
+PROBE_KINDS = ["v_add_u32", "v_mul_lo_u32", "v_mul_hi_u32", "v_mad_u64_u32", "v_mad_i64_i32",
+               "v_permlane32_swap_b32", "v_permlane16_swap_b32", "v_mfma_i32_16x16x64_i8",
+               "mfma_waves_beside_mul_lo_waves", "mfma_and_mul_lo_in_one_wave"]
+
+
+def probe_issue(kind: int, iters: int) -> dict:
+    """pm_probe_issue: wall time of an issue-rate probe and what each SIMD issued."""
+    ms, valu, mfma, mhz = (ctypes.c_double() for _ in range(4))
+    _check(lib.pm_probe_issue(int(kind), int(iters), ctypes.byref(ms), ctypes.byref(valu), ctypes.byref(mfma),
+                              ctypes.byref(mhz)))
+    return {"kind": PROBE_KINDS[kind], "ms": ms.value, "valu_per_simd": valu.value, "mfma_per_simd": mfma.value,
+            "clock_mhz": mhz.value}
+
+
 # --------------------------------------------------------------------------
 # Plans
 
@@ -266,6 +283,16 @@ class Plan:
 
     def set_host_mode(self, mode: int) -> None:
         _check(lib.pm_plan_set_host_mode(self.handle, int(mode)))
+
+    def set_host_modes(self, in_mode: int, out_mode: int) -> None:
+        """The input side's and the output side's host mode separately."""
+        _check(lib.pm_plan_set_host_modes(self.handle, int(in_mode), int(out_mode)))
+
+    @property
+    def host_modes(self):
+        i, o = ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib.pm_plan_get_host_modes(self.handle, ctypes.byref(i), ctypes.byref(o)))
+        return i.value, o.value
 
     def tune_host(self):
         """(chosen host mode, its median ms per product) from the timed choice."""

@@ -108,11 +108,20 @@ int pm_multiply_device_noalloc(pm_plan* plan, const uint32_t* a, const uint32_t*
  * The reference's XNACK/HMM mode (kernels on pageable memory) needs xnack+ code
  * objects and HSA_XNACK=1, which this build does not assume. */
 enum pm_host_mode { PM_HOST_STAGED = 0, PM_HOST_MAPPED = 1, PM_HOST_DIRECT = 2 };
-int pm_plan_set_host_mode(pm_plan* plan, int mode);
-/* Times each host mode on dummy operands (one warm-up, median of five) and keeps
- * the fastest; staged stays unless another is at least 5 % faster
- * (reference solve.cu:817-862). fherma_init calls it unless PM_HOST_MODE names a
- * mode or PM_AUTOTUNE=0. Either output may be null. */
+int pm_plan_set_host_mode(pm_plan* plan, int mode);   /* both directions */
+/* The input side (a, b) and the output side (c) separately, as the reference's
+ * FHERMA_IN and FHERMA_OUT do (solve.cu:13-34). */
+int pm_plan_set_host_modes(pm_plan* plan, int in_mode, int out_mode);
+int pm_plan_get_host_modes(const pm_plan* plan, int* in_mode, int* out_mode);
+/* Times the host modes on dummy operands and keeps the fastest; staged stays
+ * unless another is at least 5 % faster (reference solve.cu:817-862). Each
+ * timed call sizes a fresh answer on the helper thread as fherma_run does,
+ * after a write over 4 x the operand size to clear the host caches; one
+ * warm-up, median of seven. The candidates: staged and mapped both ways at 1, 2
+ * and 4 chunks, mapped in / staged out, staged in / mapped out, direct. The
+ * mode out is the input side's. fherma_init calls it unless FHERMA_IN,
+ * FHERMA_OUT or PM_HOST_MODE names a mode or FHERMA_AUTOTUNE (PM_AUTOTUNE) is 0.
+ * Either output may be null. */
 int pm_plan_tune_host(pm_plan* plan, int* mode, double* ms_per_product);
 
 /* A batch of independent products, each with its own plan (N and q may differ),
@@ -181,6 +190,16 @@ int pm_stream_create(void** out);
 int pm_stream_destroy(void* stream);
 int pm_stream_synchronize(void* stream);
 int pm_device_synchronize(void);
+/* Issue-rate probe of the current device, no counterpart in the reference: the
+ * wall time (ms) of a kernel issuing only one kind of instruction from
+ * independent chains at 16 waves per CU, and the instructions each SIMD issued
+ * (VALU and MFMA), with the device's peak clock, from which cycles per
+ * instruction follow. Kinds: 0 v_add_u32, 1 v_mul_lo_u32, 2 v_mul_hi_u32,
+ * 3 v_mad_u64_u32, 4 v_mad_i64_i32, 5 v_permlane32_swap, 6 v_permlane16_swap,
+ * 7 v_mfma_i32_16x16x64_i8, 8 MFMA waves beside v_mul_lo_u32 waves on each
+ * SIMD, 9 one wave stream alternating MFMA and v_mul_lo_u32. */
+int pm_probe_issue(int kind, int iters, double* ms, double* valu_per_simd, double* mfma_per_simd, double* clock_mhz);
+
 /* Device name and compute-unit count of the current device. */
 int pm_device_info(char* name, size_t len, int* compute_units);
 

@@ -103,3 +103,74 @@ def test_driver_refuses_bad_operand_size(golden, tmp_path):
     assert r.returncode == 0
     res = json.load(open(os.path.join(tmp_path, "out", "results.json")))
     assert res["cases"][0]["status"] == "ok" and res["cases"][1]["status"] == "crashed"
+
+
+@pytest.mark.parametrize("out_mode", [m for _, m in MODES], ids=[n for n, _ in MODES])
+@pytest.mark.parametrize("in_mode", [m for _, m in MODES], ids=[n for n, _ in MODES])
+def test_mixed_host_modes_match_gmp(golden, in_mode, out_mode):
+    """The input and output sides in different modes (the reference's FHERMA_IN
+    and FHERMA_OUT), every pair."""
+    case = golden["config-n32768-w868-s1"]
+    q, a, b = pointgen.operands(case)
+    say("host modes in", in_mode, "out", out_mode)
+    with polymult.Plan(case["N"], case["W"], q) as plan:
+        plan.set_host_modes(in_mode, out_mode)
+        assert plan.host_modes == (in_mode, out_mode)
+        for _ in range(2):
+            assert oracle.digest(plan.multiply(a, b)) == case["c_sha"]
+
+
+def run_driver(tmp_path, case, env):
+    write_point(str(tmp_path), case)
+    e = dict(os.environ)
+    for k in [k for k in e if k.startswith("FHERMA_") or k.startswith("PM_")]:
+        del e[k]
+    e.update(env)
+    return subprocess.run([SOLUTION, str(tmp_path)], capture_output=True, text=True, timeout=120, env=e)
+
+
+def answers_ok(tmp_path, case):
+    res = json.load(open(os.path.join(tmp_path, "out", "results.json")))
+    assert [c["status"] for c in res["cases"]] == ["ok", "ok"]
+    for i in range(2):
+        c = np.fromfile(os.path.join(tmp_path, "out", "%06d" % i, "c.bin"), dtype="<u4")
+        assert oracle.digest(c) == case["c_sha"]
+
+
+@pytest.mark.parametrize("env", [
+    {"FHERMA_IN": "staged", "FHERMA_OUT": "staged"},
+    {"FHERMA_IN": "mapped", "FHERMA_OUT": "staged"},
+    {"FHERMA_IN": "register", "FHERMA_OUT": "mapped"},
+    {"FHERMA_IN": "hmm", "FHERMA_OUT": "stream"},
+    {"FHERMA_IN": "staged", "FHERMA_CHUNKS": "3", "FHERMA_OUT_CHUNKS": "5"},
+    {"FHERMA_IN": "mapped", "FHERMA_OUT": "mapped", "FHERMA_A_CHUNKS": "1,3", "FHERMA_B_CHUNKS": "2,1,1",
+     "FHERMA_C_CHUNKS": "1,2,1"},
+    {"FHERMA_AUTOTUNE": "0", "FHERMA_THREADS": "3", "FHERMA_SPIN_MS": "5"},
+], ids=["staged", "mapped-staged", "register-mapped", "hmm-stream", "chunks", "weights", "no-autotune"])
+def test_driver_honours_the_reference_knobs(golden, tmp_path, env):
+    """fherma_init reads the reference's host knobs (solve.cu:13-49): the copy
+    modes (hmm and stream fall back to staged, register is a DMA from the
+    caller's memory), chunk counts and weights, threads; every answer vs GMP."""
+    case = golden["matrix-n32768-w868-prime"]
+    say("driver env", env)
+    r = run_driver(tmp_path, case, env)
+    assert r.returncode == 0, r.stderr
+    answers_ok(tmp_path, case)
+
+
+def test_driver_refuses_unknown_copy_mode(golden, tmp_path):
+    r = run_driver(tmp_path, golden["config-n4096-w256-s1"], {"FHERMA_IN": "pinned"})
+    assert r.returncode != 0 and "FHERMA_IN must be staged, register, mapped, hmm or stream" in r.stderr
+
+
+def test_driver_autotune_switch_and_profile(golden, tmp_path):
+    """FHERMA_AUTOTUNE_LOG lists the timed candidates unless FHERMA_AUTOTUNE=0
+    skips the choice; FHERMA_PROFILE prints the phase times of every call."""
+    case = golden["config-n4096-w256-s1"]
+    r = run_driver(tmp_path, case, {"FHERMA_AUTOTUNE_LOG": "1", "FHERMA_PROFILE": "1"})
+    assert r.returncode == 0, r.stderr
+    assert r.stderr.count("host mode in") >= 8 and r.stderr.count("gpu us: in+decompose") == 2, r.stderr
+    answers_ok(tmp_path, case)
+    r = run_driver(tmp_path, case, {"FHERMA_AUTOTUNE_LOG": "1", "FHERMA_AUTOTUNE": "0"})
+    assert r.returncode == 0 and "host mode in" not in r.stderr, r.stderr
+    answers_ok(tmp_path, case)
