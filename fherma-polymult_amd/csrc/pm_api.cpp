@@ -69,9 +69,12 @@ struct Twiddles {
     uint2* rzi = nullptr;
     uint32_t* rom = nullptr;
     uint32_t* romi = nullptr;
+    uint8_t* colm_fwd = nullptr;   // matrix-core column pass fragments (R = 256 only)
+    uint8_t* colm_inv = nullptr;
     ~Twiddles() {
         for (void* p : {static_cast<void*>(tw), static_cast<void*>(itw), static_cast<void*>(rz),
-                        static_cast<void*>(rzi), static_cast<void*>(rom), static_cast<void*>(romi)})
+                        static_cast<void*>(rzi), static_cast<void*>(rom), static_cast<void*>(romi),
+                        static_cast<void*>(colm_fwd), static_cast<void*>(colm_inv)})
             if (p) (void)hipFree(p);
     }
 };
@@ -115,6 +118,15 @@ std::shared_ptr<Twiddles> twiddles_for(const pm::HostPlan& pl) {
         check(hipMemcpy(dz, z.data(), zw * sizeof(uint32_t), hipMemcpyHostToDevice), "upload row twiddles");
         check(hipMemcpy(dom, o.data(), ow * sizeof(uint32_t), hipMemcpyHostToDevice), "upload row twiddles");
     }
+    if (pm::col_mfma_ok(pl)) {
+        std::vector<uint8_t> m;
+        for (int fwd = 1; fwd >= 0; --fwd) {
+            pm::build_col_mfma_tables(pl, fwd != 0, m);
+            uint8_t*& dst = fwd ? t->colm_fwd : t->colm_inv;
+            dst = static_cast<uint8_t*>(dev_alloc(m.size()));
+            check(hipMemcpy(dst, m.data(), m.size(), hipMemcpyHostToDevice), "upload column-pass matrices");
+        }
+    }
     // entries whose tables every plan has released are dropped here
     for (auto e = g_tw_cache.begin(); e != g_tw_cache.end();) e = e->second.expired() ? g_tw_cache.erase(e) : std::next(e);
     g_tw_cache[key] = t;
@@ -141,6 +153,20 @@ int row_twiddle_policy(const pm::HostPlan& h) {
         need(!std::strcmp(v, "auto"), "FHERMA_TWIDDLES must be table, gen or auto");
     }
     return 16.0 * h.K * h.N > 64e6 ? 1 : 0;
+}
+
+// The column pass a new plan uses: the VALU one unless PM_NTT_FORM=mfma asks
+// for the matrix-core one where its tables exist (PM_NTT_FORM=valu or auto: the
+// default, which measured faster alone and equal inside the product batch;
+// anything else is refused).
+int ntt_form_default(const pm::DevPlan& d) {
+    bool mfma = false;
+    if (const char* v = std::getenv("PM_NTT_FORM")) {
+        need(!std::strcmp(v, "valu") || !std::strcmp(v, "mfma") || !std::strcmp(v, "auto"),
+             "PM_NTT_FORM must be valu, mfma or auto");
+        mfma = std::strcmp(v, "mfma") == 0;
+    }
+    return mfma && d.colm_fwd && d.colm_inv ? 1 : 0;
 }
 
 // The host plan of a point with the reference's prime rule; `kernels`: every
@@ -196,6 +222,9 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q)
     d.rzi = pl->tw->rzi;
     d.rom = pl->tw->rom;
     d.romi = pl->tw->romi;
+    d.colm_fwd = pl->tw->colm_fwd;
+    d.colm_inv = pl->tw->colm_inv;
+    d.col_mfma = ntt_form_default(d);
     d.p_recip = pl->upload(h.p_recip);
     d.M = pl->upload(h.M);
     d.negvP = pl->upload(h.negvP);
@@ -394,6 +423,18 @@ int pm_plan_set_twiddles(pm_plan* plan, int source) {
         plan->dev.row_gen = source == PM_TWIDDLES_GEN ? 1 : 0;
     });
 }
+
+int pm_plan_set_ntt_form(pm_plan* plan, int form) {
+    return guarded([&] {
+        need(plan != nullptr, "plan is null");
+        need(form == PM_NTT_VALU || form == PM_NTT_MFMA, "form must be PM_NTT_VALU or PM_NTT_MFMA");
+        need(form == PM_NTT_VALU || (plan->dev.colm_fwd && plan->dev.colm_inv),
+             "the matrix-core column pass needs R = 256 rows (N = 65536)");
+        plan->dev.col_mfma = form == PM_NTT_MFMA ? 1 : 0;
+    });
+}
+
+int pm_plan_get_ntt_form(const pm_plan* plan) { return plan ? (plan->dev.col_mfma ? PM_NTT_MFMA : PM_NTT_VALU) : -1; }
 
 int pm_plan_get_twiddles(const pm_plan* plan) {
     return plan ? (plan->dev.row_gen ? PM_TWIDDLES_GEN : PM_TWIDDLES_TABLE) : -1;

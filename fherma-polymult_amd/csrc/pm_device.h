@@ -37,6 +37,11 @@ struct DevPlan {
     const uint32_t* pow2b;
     const int8_t* dec_digits_b;
     const uint64_t* dec_corr_b;
+    // Column pass on the matrix cores (pm_ntt_mfma.hip), R = 256: per prime and
+    // direction, 17 matrices as MFMA fragments (87040 bytes); null when absent.
+    const uint8_t* colm_fwd;
+    const uint8_t* colm_inv;
+    int col_mfma;             // 1: the column passes run on the matrix cores (tables present)
     const uint2* tw;          // K x N: (psi^bitrev(i), Shoup companion)
     const uint2* itw;         // K x N: (psi^-bitrev(i), Shoup companion)
     // Row-pass twiddles, generated: the twiddle of row rho, stage distance 2^e,
@@ -135,6 +140,10 @@ struct BatchArgs {
     const uint32_t* crt_lc[kBatchMax];
 };
 bool batch_kernels_ok(const DevPlan& P);   // register NTT kernels exist for this geometry
+// The column pass on the matrix cores, one launch (ba: a batch of `count`
+// products, as launch_ntt_cols_batch; null: X alone).
+void launch_ntt_cols_mfma(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st, const BatchArgs* ba = nullptr,
+                          int count = 0);
 void launch_decompose_mfma_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st);
 void launch_ntt_cols_batch(const DevPlan& P, const BatchArgs& ba, int count, bool forward, hipStream_t st);
 void launch_ntt_rows_fused_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st);

@@ -626,6 +626,10 @@ void fused_launch(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t 
 
 void launch_ntt_cols(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     if (P.logR == 0) return;
+    if (P.col_mfma) {
+        launch_ntt_cols_mfma(P, X, forward, st);
+        return;
+    }
     switch (P.logR) {
         case 4: col_launch<4>(P, X, forward, st); return;
         case 5: col_launch<5>(P, X, forward, st); return;
@@ -663,6 +667,10 @@ bool batch_kernels_ok(const DevPlan& P) {
 }
 
 void launch_ntt_cols_batch(const DevPlan& P, const BatchArgs& ba, int count, bool forward, hipStream_t st) {
+    if (P.col_mfma) {
+        launch_ntt_cols_mfma(P, nullptr, forward, st, &ba, count);
+        return;
+    }
     switch (col_shape(P)) {
         case 4 * 16 + 4: col_launch2<4, 4>(P, nullptr, forward, st, &ba, count); return;
         case 4 * 16 + 5: col_launch2<4, 5>(P, nullptr, forward, st, &ba, count); return;

@@ -24,6 +24,7 @@ PM_OK, PM_EINVAL, PM_EHIP, PM_ENOMEM, PM_EINTERNAL = 0, 1, 2, 3, 4
 FORM_PLAIN, FORM_MFMA = 0, 1
 HOST_STAGED, HOST_MAPPED, HOST_DIRECT = 0, 1, 2
 TWIDDLES_TABLE, TWIDDLES_GEN = 0, 1
+NTT_VALU, NTT_MFMA = 0, 1
 NPHASES = 7
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -64,6 +65,8 @@ def _load() -> ctypes.CDLL:
         "pm_batch_launch": (ctypes.c_int, [_vp, _vp]),
         "pm_batch_phased": (ctypes.c_int, [_vp]),
         "pm_plan_set_twiddles": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "pm_plan_set_ntt_form": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "pm_plan_get_ntt_form": (ctypes.c_int, [_vp]),
         "pm_plan_get_twiddles": (ctypes.c_int, [_vp]),
         "pm_batch_destroy": (None, [_vp]),
         "pm_stage_decompose": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
@@ -272,6 +275,14 @@ class Plan:
         c = np.empty((self.N, self.L), dtype=np.uint32)
         _check(lib.pm_multiply(self.handle, _ptr(a), _ptr(b), _ptr(c)))
         return c
+
+    def set_ntt_form(self, form: int) -> None:
+        """NTT_VALU or NTT_MFMA: the column passes' butterflies or matrix-core form."""
+        _check(lib.pm_plan_set_ntt_form(self.handle, int(form)))
+
+    @property
+    def ntt_form(self) -> int:
+        return lib.pm_plan_get_ntt_form(self.handle)
 
     def set_twiddles(self, source: int) -> None:
         """Row-pass twiddle source: TWIDDLES_TABLE or TWIDDLES_GEN (same results)."""

@@ -78,6 +78,21 @@ typedef enum {
 int pm_plan_set_twiddles(pm_plan* plan, int source);
 int pm_plan_get_twiddles(const pm_plan* plan);   /* pm_twiddles, or -1 for a null plan */
 
+/* The NTT's column passes (the stages of butterfly distance >= S, reference
+ * solve.cu:128-166): butterflies on the VALU, or, for R = 256 rows (N = 65536),
+ * the exact int8 matrix-core form (two radix-16 steps as v_mfma_i32_16x16x64_i8
+ * products per column, pm_ntt_mfma.hip). Both give the same NTT-domain values
+ * mod p. The default is the VALU form (PM_NTT_FORM=valu or auto);
+ * PM_NTT_FORM=mfma makes new plans use the matrix-core form where it applies
+ * (any other value makes plan creation fail with PM_EINVAL). No counterpart in
+ * the reference, which has only butterflies. */
+typedef enum {
+    PM_NTT_VALU = 0,
+    PM_NTT_MFMA = 1
+} pm_ntt_form;
+int pm_plan_set_ntt_form(pm_plan* plan, int form);
+int pm_plan_get_ntt_form(const pm_plan* plan);   /* pm_ntt_form, or -1 for a null plan */
+
 /* --- products -------------------------------------------------------------
  * pm_multiply: host pointers, synchronous; the fherma_run equivalent
  * (reference solve.cu:871-1059). Inputs cross PCIe through pinned staging.

@@ -154,15 +154,19 @@ def test_plan_primes(N, W):
 
 
 @pytest.mark.parametrize("form", [f for _, f in FORMS], ids=[n for n, _ in FORMS])
-@pytest.mark.parametrize("N,W,seed", [(64, 100, 3), (4096, 256, 5), (8192, 868, 6), (32768, 868, 7),
-                                      (65536, 2048, 8)])
-def test_stages_match_oracle(N, W, seed, form):
+@pytest.mark.parametrize("N,W,seed,ntt", [(64, 100, 3, 0), (4096, 256, 5, 0), (8192, 868, 6, 0), (32768, 868, 7, 0),
+                                          (65536, 2048, 8, 0), (65536, 2048, 8, 1), (65536, 1747, 9, 1),
+                                          (65536, 64, 10, 1)])
+def test_stages_match_oracle(N, W, seed, ntt, form):
     """Each of the five steps, device to device, against the oracle's step (the
-    stage entry points leave canonical residues)."""
-    say("stages", N, W, seed, "form", form)
+    stage entry points leave canonical residues); at N = 65536 with the column
+    passes in both forms (butterflies, int8 matrix cores)."""
+    say("stages", N, W, seed, "form", form, "ntt", ntt)
     q, a, b = oracle.gen_point(seed, N, W)
     ref = oracle.Plan(N, q)
     with polymult.Plan(N, W, q, form=form) as plan:
+        plan.set_ntt_form(ntt)
+        assert plan.ntt_form == ntt
         assert plan.K == ref.K and np.array_equal(plan.primes, ref.primes)
         K, L = plan.K, plan.L
         da, db = polymult.DeviceBuffer.from_array(a), polymult.DeviceBuffer.from_array(b)
@@ -360,3 +364,49 @@ def test_short_device_buffers_are_refused_on_the_host(golden):
         assert oracle.digest(C.download((plan.N, plan.L))) == case["c_sha"]
         for buf in (da, db, short, C, R):
             buf.close()
+
+
+@pytest.mark.parametrize("ntt", [0, 1], ids=["valu-ntt", "mfma-ntt"])
+@pytest.mark.parametrize("name", ["config-n65536-w1747-s1", "config-n65536-w2048-00", "config-n65536-w2048-07",
+                                  "pattern-all_q_minus_1-n65536-w2048", "pattern-wrap-n65536-w2048",
+                                  "pattern-alternating-n65536-w2048", "matrix-n65536-w1747-prime",
+                                  "matrix-n65536-w1747-prime-allqm1"])
+def test_column_pass_forms_match_gmp(golden, name, ntt):
+    """Products at N = 65536 with the column passes on the VALU and on the
+    matrix cores, both decompose/CRT forms, against the GMP digests."""
+    case = golden[name]
+    q, a, b = pointgen.operands(case)
+    for form in (polymult.FORM_MFMA, polymult.FORM_PLAIN):
+        say("column forms", name, "ntt", ntt, "form", form)
+        with polymult.Plan(case["N"], case["W"], q, form=form) as plan:
+            plan.set_ntt_form(ntt)
+            assert oracle.digest(plan.multiply(a, b)) == case["c_sha"], (name, ntt, form)
+
+
+def test_matrix_core_column_pass_needs_r256(golden):
+    case = golden["config-n32768-w868-s1"]
+    q, _, _ = pointgen.operands(case)
+    with polymult.Plan(case["N"], case["W"], q) as plan:
+        assert plan.ntt_form == polymult.NTT_VALU
+        with pytest.raises(polymult.PolymulError) as e:
+            plan.set_ntt_form(polymult.NTT_MFMA)
+        assert e.value.code == polymult.PM_EINVAL
+
+
+def test_ntt_form_default_and_environment(golden, monkeypatch):
+    """New plans use the VALU column pass unless PM_NTT_FORM=mfma; an unknown
+    value is refused at plan creation."""
+    case = golden["config-n65536-w1747-s1"]
+    q, _, _ = pointgen.operands(case)
+    for env, want in ((None, polymult.NTT_VALU), ("auto", polymult.NTT_VALU), ("valu", polymult.NTT_VALU),
+                      ("mfma", polymult.NTT_MFMA)):
+        if env is None:
+            monkeypatch.delenv("PM_NTT_FORM", raising=False)
+        else:
+            monkeypatch.setenv("PM_NTT_FORM", env)
+        with polymult.Plan(case["N"], case["W"], q) as plan:
+            assert plan.ntt_form == want, env
+    monkeypatch.setenv("PM_NTT_FORM", "tensor")
+    with pytest.raises(polymult.PolymulError) as e:
+        polymult.Plan(case["N"], case["W"], q)
+    assert e.value.code == polymult.PM_EINVAL
