@@ -207,7 +207,10 @@ def batch_rates(args, plans, da, db, dc, gold, log):
     """Products/s of the matrix-core batch (one hipGraph) and of the plain form."""
     import polymult  # noqa: E402
     L = plans[0].L
+    t0 = time.perf_counter()
     graph = polymult.Batch(plans, da, db, dc)
+    shape = shape_info(graph, time.perf_counter() - t0)
+    log("batch shape (timed choice): %s" % shape)
 
     def step():
         graph.launch()
@@ -250,7 +253,16 @@ def batch_rates(args, plans, da, db, dc, gold, log):
         for p in plans:
             p.set_form(polymult.FORM_MFMA)
 
-    return value, sec, plain_value, mismatches
+    return value, sec, plain_value, mismatches, shape
+
+
+def shape_info(graph, create_s):
+    """The captured shape of a timed batch, for the bench line."""
+    runs, streams, ntt, tw = graph.shape
+    names = {-1: "plans' own", 0: "valu", 1: "mfma"}
+    return {"runs": runs, "streams": streams, "ntt_form": names[ntt],
+            "twiddles": {-1: "plans' own", 0: "table", 1: "generated"}[tw],
+            "tuned_replay_ms": round(graph.tuned_ms, 4), "create_s": round(create_s, 3)}
 
 
 def uneven_rate(args, log):
@@ -268,7 +280,10 @@ def uneven_rate(args, log):
     da = [polymult.DeviceBuffer.from_array(a) for _, a, _ in points]
     db = [polymult.DeviceBuffer.from_array(b) for _, _, b in points]
     dc = [polymult.DeviceBuffer(a.nbytes) for _, a, _ in points]
+    t0 = time.perf_counter()
     graph = polymult.Batch(plans, da, db, dc)
+    shape = shape_info(graph, time.perf_counter() - t0)
+    log("uneven batch shape (timed choice): %s" % shape)
     graph.launch()
     polymult.synchronize()
     mismatches = sum(digest(x.download((c["N"], c["L"]))) != c["c_sha"] for c, x in zip(cases, dc))
@@ -281,7 +296,7 @@ def uneven_rate(args, log):
     polymult.synchronize()
     sec = (time.perf_counter() - t0) / args.steps
     out = {"value": round(len(cases) / sec, 2), "unit": "polymuls/s", "products": len(cases),
-           "ms_per_batch": round(sec * 1e3, 4), "mismatches": int(mismatches),
+           "ms_per_batch": round(sec * 1e3, 4), "mismatches": int(mismatches), "shape": shape,
            "note": "64 products of uneven N (4096-65536) and q width (128-2019 bits), one hipGraph, device-resident"}
     log("uneven batch: %.3f ms per batch of %d, %.1f products/s, %d mismatches"
         % (sec * 1e3, len(cases), len(cases) / sec, mismatches))
@@ -462,7 +477,7 @@ def main():
     dc = [polymult.DeviceBuffer(4 * N * L) for _ in points]
 
     gold = golden_digests()
-    value = sec = plain_value = None
+    value = sec = plain_value = shape = None
     if args.phases_only:
         for p, a, b, c in zip(plans, da, db, dc):
             p.multiply_device(a, b, c)
@@ -470,7 +485,7 @@ def main():
         mismatches = sum(digest(dc[i].download((N, L))) != gold.get(SEED0 + i) for i in range(BATCH))
         log("one product at a time: %d mismatches vs GMP" % mismatches)
     else:
-        value, sec, plain_value, mismatches = batch_rates(args, plans, da, db, dc, gold, log)
+        value, sec, plain_value, mismatches, shape = batch_rates(args, plans, da, db, dc, gold, log)
     uneven = None
     if not (args.phases_only or args.no_uneven):
         uneven = uneven_rate(args, log)
@@ -522,7 +537,7 @@ def main():
         "value": None if value is None else round(value, 2), "unit": "polymuls/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": None if sec is None else round(sec * 1e3, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32", "data": "synthetic",
-        "config": {"workload": "batch of 16 products, N=65536, random odd 2048-bit q (L=64), device-resident", "form": "int8-MFMA decompose/CRT, batch replayed as one hipGraph", "batch": BATCH, "N": N, "W": W, "L": L, "K": K,
+        "config": {"workload": "batch of 16 products, N=65536, random odd 2048-bit q (L=64), device-resident", "form": "int8-MFMA decompose/CRT, batch replayed as one hipGraph", "batch": BATCH, "batch_shape": shape, "N": N, "W": W, "L": L, "K": K,
                    "parallelism": "single GPU", "device": name, "cus": cus,
                    "row_twiddles": "generated" if plans[0].twiddles == polymult.TWIDDLES_GEN else "table",
                    "prime_basis": "p < 2^31 (the reference's rule)"},

@@ -61,23 +61,56 @@ int guarded(F&& f) {
 
 namespace pm {
 
-// Device twiddle tables for (N, prime list), shared between plans.
-struct Twiddles {
+// The full K x N twiddle tables, forward and inverse (psi^bitrev(i) pairs):
+// only the row pass reading its twiddles from tables and the generic NTT
+// kernels (geometries without register kernels) need them, so they exist only
+// while such a plan or batch holds them.
+struct FullTables {
     uint2* tw = nullptr;
     uint2* itw = nullptr;
+    ~FullTables() {
+        if (tw) (void)hipFree(tw);
+        if (itw) (void)hipFree(itw);
+    }
+};
+
+// Device twiddle tables for (N, prime list), shared between plans.
+struct Twiddles {
+    uint2* ctw = nullptr;    // K x R: the first R pairs of each prime's forward table (column pass)
+    uint2* citw = nullptr;   // inverse
     uint2* rz = nullptr;
     uint2* rzi = nullptr;
     uint32_t* rom = nullptr;
     uint32_t* romi = nullptr;
     uint8_t* colm_fwd = nullptr;   // matrix-core column pass fragments (R = 256 only)
     uint8_t* colm_inv = nullptr;
+    std::mutex full_mutex;
+    std::weak_ptr<FullTables> full;
     ~Twiddles() {
-        for (void* p : {static_cast<void*>(tw), static_cast<void*>(itw), static_cast<void*>(rz),
+        for (void* p : {static_cast<void*>(ctw), static_cast<void*>(citw), static_cast<void*>(rz),
                         static_cast<void*>(rzi), static_cast<void*>(rom), static_cast<void*>(romi),
                         static_cast<void*>(colm_fwd), static_cast<void*>(colm_inv)})
             if (p) (void)hipFree(p);
     }
 };
+
+// The full tables of t's primes: the existing copy while anything holds one,
+// else built and uploaded now.
+std::shared_ptr<FullTables> full_tables(Twiddles& t, const HostPlan& pl) {
+    std::lock_guard<std::mutex> lock(t.full_mutex);
+    if (auto sp = t.full.lock()) return sp;
+    auto f = std::make_shared<FullTables>();
+    const size_t words = 2 * static_cast<size_t>(pl.K) * pl.N;
+    std::vector<uint32_t> host(words);
+    f->tw = static_cast<uint2*>(dev_alloc(words * sizeof(uint32_t)));
+    f->itw = static_cast<uint2*>(dev_alloc(words * sizeof(uint32_t)));
+    build_twiddles(pl, true, 0, pl.K, host.data(), pl.N);
+    check(hipMemcpy(f->tw, host.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice), "upload twiddles");
+    build_twiddles(pl, false, 0, pl.K, host.data(), pl.N);
+    check(hipMemcpy(f->itw, host.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice), "upload twiddles");
+    t.full = f;
+    return f;
+}
 
 }  // namespace pm
 
@@ -97,14 +130,15 @@ std::shared_ptr<Twiddles> twiddles_for(const pm::HostPlan& pl) {
         if (auto sp = it->second.lock()) return sp;
     }
     auto t = std::make_shared<Twiddles>();
-    const size_t words = 2 * static_cast<size_t>(pl.K) * pl.N;
+    const uint32_t R = 1u << pl.logR;
+    const size_t words = 2 * static_cast<size_t>(pl.K) * R;
     std::vector<uint32_t> host(words);
-    t->tw = static_cast<uint2*>(dev_alloc(words * sizeof(uint32_t)));
-    t->itw = static_cast<uint2*>(dev_alloc(words * sizeof(uint32_t)));
-    pm::build_twiddles(pl, true, 0, pl.K, host.data());
-    check(hipMemcpy(t->tw, host.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice), "upload twiddles");
-    pm::build_twiddles(pl, false, 0, pl.K, host.data());
-    check(hipMemcpy(t->itw, host.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice), "upload twiddles");
+    for (int fwd = 1; fwd >= 0; --fwd) {
+        pm::build_twiddles(pl, fwd != 0, 0, pl.K, host.data(), R);
+        uint2*& dst = fwd ? t->ctw : t->citw;
+        dst = static_cast<uint2*>(dev_alloc(words * sizeof(uint32_t)));
+        check(hipMemcpy(dst, host.data(), words * sizeof(uint32_t), hipMemcpyHostToDevice), "upload twiddles");
+    }
     const size_t zw = 2 * static_cast<size_t>(pl.K) * (size_t(1) << pl.logR) * pl.logS;
     const size_t ow = static_cast<size_t>(pl.K) * ((size_t(1) << pl.logS) / 2);
     std::vector<uint32_t> z(zw), o(ow);
@@ -216,8 +250,13 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q)
     d.pc = pl->upload(pc);
     d.pow2 = pl->upload(h.pow2);
     d.pow2b = pl->upload(h.pow2b);
-    d.tw = pl->tw->tw;
-    d.itw = pl->tw->itw;
+    d.ctw = pl->tw->ctw;
+    d.citw = pl->tw->citw;
+    if (!kernels || row_gen == 0) {
+        pl->full = pm::full_tables(*pl->tw, h);
+        d.tw = pl->full->tw;
+        d.itw = pl->full->itw;
+    }
     d.rz = pl->tw->rz;
     d.rzi = pl->tw->rzi;
     d.rom = pl->tw->rom;
@@ -240,59 +279,151 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q)
     for (auto& e : pl->ev) check(hipEventCreate(&e), "hipEventCreate");
     for (auto& e : pl->ev_stop) check(hipEventCreate(&e), "hipEventCreate");
     pl->ensure_mfma();
+    pl->upload_desc();
     pl->form = PM_FORM_MFMA;
     (void)pm::scratch_on_current_device();   // set-up of the allocation-free entry point on this device
     return pl.release();
 }
 
-// A captured batch: every product's kernel chain, forked over up to kBranches
-// streams (independent products run concurrently), replayed as one hipGraph.
-constexpr int kBranches = 4;
-
-// Products that share N, L and the primes, all in the matrix-core form, each
-// with its own plan, on a geometry with register NTT kernels: they can run as
-// phase-batched launches (five per kBatchMax products).
-bool batchable(pm_plan* const* plans, int count) {
-    if (count < 1) return false;
-    const pm_plan* p0 = plans[0];
-    if (p0->form != PM_FORM_MFMA || !pm::batch_kernels_ok(p0->dev)) return false;
-    for (int i = 1; i < count; ++i) {
-        const pm_plan* p = plans[i];
-        if (p->form != PM_FORM_MFMA || p->host.N != p0->host.N || p->host.L != p0->host.L || p->host.p != p0->host.p)
-            return false;
-        for (int k = 0; k < i; ++k)
-            if (plans[k] == p) return false;
-    }
-    return true;
+// Whether the products of two plans can share batched launches: both in the
+// matrix-core form, at one N with register NTT kernels, and the shorter prime
+// list a prefix of the longer (always so for plans made by the prime rule at
+// one N: it walks one sequence of primes per N and stops at a count set by q's
+// width), so that one plan's per-prime constants and twiddles serve both.
+bool groupable(const pm_plan* x, const pm_plan* y) {
+    if (x->form != PM_FORM_MFMA || y->form != PM_FORM_MFMA) return false;
+    if (x->host.N != y->host.N || !pm::batch_kernels_ok(x->dev)) return false;
+    const bool xs = x->host.p.size() <= y->host.p.size();
+    const std::vector<uint32_t>& shorter = xs ? x->host.p : y->host.p;
+    const std::vector<uint32_t>& longer = xs ? y->host.p : x->host.p;
+    return std::equal(shorter.begin(), shorter.end(), longer.begin());
 }
 
-// Decompose every a and b, forward columns of every A and B, fused rows, inverse
-// columns, CRT: one launch each per kBatchMax products, all on st.
-void run_batched(pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b, uint32_t* const* c,
-                 int count, hipStream_t st) {
-    const pm::DevPlan& P = plans[0]->dev;
-    for (int i0 = 0; i0 < count; i0 += pm::kBatchMax) {
-        const int n = std::min(count - i0, pm::kBatchMax);
+// The column-pass form and row-pass twiddle source of a batched run, when they
+// differ from those of its longest plan (-1: the plan's own). Table twiddles
+// come from `full`, the full tables of that plan's primes.
+struct NttChoice {
+    int col_mfma = -1;
+    int row_gen = -1;
+    const pm::FullTables* full = nullptr;
+};
+
+// Entry k of ba as entry m of sub.
+void copy_entry(const pm::BatchArgs& ba, int k, pm::BatchArgs& sub, int m) {
+    sub.a[m] = ba.a[k];
+    sub.b[m] = ba.b[k];
+    sub.c[m] = ba.c[k];
+    sub.A[m] = ba.A[k];
+    sub.B[m] = ba.B[k];
+    sub.d[m] = ba.d[k];
+    sub.K[m] = ba.K[k];
+}
+
+// Launches go(sub, m, key) once per distinct key among the n entries of ba, sub
+// holding the m entries with that key.
+template <class Go>
+void by_key(const pm::BatchArgs& ba, int n, const std::vector<int>& key, Go go) {
+    std::vector<char> done(n, 0);
+    for (int k = 0; k < n; ++k) {
+        if (done[k]) continue;
+        pm::BatchArgs sub{};
+        int m = 0;
+        for (int t = k; t < n; ++t)
+            if (!done[t] && key[t] == key[k]) {
+                copy_entry(ba, t, sub, m++);
+                done[t] = 1;
+            }
+        go(sub, m, key[k]);
+    }
+}
+
+// The products idx (pairwise groupable, distinct plans) as batched launches on
+// st, kBatchMax products per launch, most primes first: decompose of every a
+// and b (one launch per decompose K-step count), forward columns of every A and
+// B, fused rows, inverse columns, CRT (one launch per CRT K-step count). The
+// plan with the most primes of each launch supplies the geometry, per-prime
+// constants and twiddles.
+void run_batched(const std::vector<int>& idx, pm_plan* const* plans, const uint32_t* const* a,
+                 const uint32_t* const* b, uint32_t* const* c, hipStream_t st, const NttChoice& ch = {}) {
+    std::vector<int> order(idx);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return plans[x]->host.K > plans[y]->host.K; });
+    for (size_t i0 = 0; i0 < order.size(); i0 += pm::kBatchMax) {
+        const int n = static_cast<int>(std::min(order.size() - i0, static_cast<size_t>(pm::kBatchMax)));
+        pm::DevPlan P = plans[order[i0]]->dev;
+        if (ch.col_mfma >= 0) P.col_mfma = ch.col_mfma;
+        if (ch.row_gen >= 0) P.row_gen = ch.row_gen;
+        if (ch.full) {
+            P.tw = ch.full->tw;
+            P.itw = ch.full->itw;
+        }
         pm::BatchArgs ba{};
+        std::vector<int> dec_key(n), crt_key(n);
         for (int k = 0; k < n; ++k) {
-            const pm_plan* p = plans[i0 + k];
-            ba.a[k] = a[i0 + k];
-            ba.b[k] = b[i0 + k];
-            ba.c[k] = c[i0 + k];
+            const int i = order[i0 + k];
+            const pm_plan* p = plans[i];
+            ba.a[k] = a[i];
+            ba.b[k] = b[i];
+            ba.c[k] = c[i];
             ba.A[k] = p->A;
             ba.B[k] = p->B;
-            ba.crt_tab[k] = p->dev.crt_tab;
-            ba.crt_fc[k] = p->dev.crt_fc;
-            ba.nq[k] = p->dev.nq;
-            ba.crt_lc[k] = p->dev.crt_lc;
+            ba.d[k] = p->desc;
+            ba.K[k] = p->dev.K;
+            dec_key[k] = p->dev.dec_ksteps;
+            crt_key[k] = p->dev.crt_ksteps;
         }
-        pm::launch_decompose_mfma_batch(P, ba, n, st);
+        by_key(ba, n, dec_key, [&](const pm::BatchArgs& sub, int m, int ks) {
+            pm::DevPlan Q = P;
+            Q.dec_ksteps = ks;
+            pm::launch_decompose_mfma_batch(Q, sub, m, st);
+        });
         pm::launch_ntt_cols_batch(P, ba, n, true, st);
         pm::launch_ntt_rows_fused_batch(P, ba, n, st);
         pm::launch_ntt_cols_batch(P, ba, n, false, st);
-        pm::launch_crt_mfma_batch(P, ba, n, st);
+        by_key(ba, n, crt_key, [&](const pm::BatchArgs& sub, int m, int ks) {
+            pm::DevPlan Q = P;
+            Q.crt_ksteps = ks;
+            pm::launch_crt_mfma_batch(Q, sub, m, st);
+        });
     }
     check(hipGetLastError(), "kernel launch");
+}
+
+// The products split into sets whose members can share batched launches
+// (first-fit over the products in order); a product that fits none (plain
+// form, or a geometry without register kernels) is a set of its own, marked
+// by `alone`.
+struct ProductSets {
+    std::vector<std::vector<int>> sets;
+    std::vector<char> alone;
+};
+
+ProductSets product_sets(pm_plan* const* plans, int count) {
+    ProductSets ps;
+    for (int i = 0; i < count; ++i) {
+        const bool ok = plans[i]->form == PM_FORM_MFMA && pm::batch_kernels_ok(plans[i]->dev);
+        bool placed = false;
+        for (size_t s = 0; ok && s < ps.sets.size() && !placed; ++s) {
+            if (ps.alone[s]) continue;
+            bool fits = true;
+            for (int k : ps.sets[s]) fits = fits && plans[k] != plans[i] && groupable(plans[k], plans[i]);
+            if (fits) {
+                ps.sets[s].push_back(i);
+                placed = true;
+            }
+        }
+        if (!placed) {
+            ps.sets.push_back({i});
+            ps.alone.push_back(ok ? 0 : 1);
+        }
+    }
+    return ps;
+}
+
+// Work of one product, for balancing runs and branches: K N (log N + 2) for
+// the NTTs and pointwise steps, plus N L K / 8 for the contractions.
+double product_work(const pm_plan* p) {
+    const pm::HostPlan& h = p->host;
+    return static_cast<double>(h.K) * h.N * (h.logN + 2) + static_cast<double>(h.N) * h.L * h.K / 8;
 }
 
 const char* const kPhaseNames[PM_NPHASES] = {"decompose_a", "decompose_b", "ntt_cols_fwd_a", "ntt_cols_fwd_b",
@@ -342,6 +473,8 @@ void run_device_in(pm_plan* pl, uint32_t* A, uint32_t* B, const uint32_t* a, con
         ba.b[0] = b;
         ba.A[0] = A;
         ba.B[0] = B;
+        ba.d[0] = pl->desc;
+        ba.K[0] = pl->dev.K;
         if (pl->form == PM_FORM_MFMA) {
             launch_decompose_mfma_batch(pl->dev, ba, 1, st);   // b with the b-side tables
         } else {
@@ -365,13 +498,294 @@ using pm::run_device;
 struct pm_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    std::vector<pm_plan*> plans;
+    pm_batch_shape shape{};
     bool phased = false;
+    double ms = 0;                                        // median replay of the timed choice, or 0
+    std::vector<std::shared_ptr<pm::FullTables>> keep;    // full tables its table-twiddle runs read
     ~pm_batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
     }
 };
+
+namespace {
+
+// One chain of a captured batch: a batched run of products, or one product's
+// own kernel chain.
+struct Unit {
+    std::vector<int> idx;
+    bool batched;
+    double work;
+};
+
+std::vector<Unit> batch_units(int count, pm_plan* const* plans, int runs) {
+    const ProductSets ps = product_sets(plans, count);
+    double total = 0;
+    for (int i = 0; i < count; ++i) total += product_work(plans[i]);
+    std::vector<Unit> units;
+    for (size_t s = 0; s < ps.sets.size(); ++s) {
+        std::vector<int> set = ps.sets[s];
+        if (ps.alone[s] || runs <= 0) {
+            for (int i : set) units.push_back({{i}, false, product_work(plans[i])});
+            continue;
+        }
+        double sw = 0;
+        for (int i : set) sw += product_work(plans[i]);
+        const int r = std::max(1, std::min(static_cast<int>(set.size()), static_cast<int>(std::lround(runs * sw / total))));
+        std::stable_sort(set.begin(), set.end(), [&](int x, int y) { return plans[x]->host.K > plans[y]->host.K; });
+        // contiguous slices of about equal work
+        Unit u{{}, true, 0.0};
+        int cut = 0;
+        double acc = 0;
+        for (size_t k = 0; k < set.size(); ++k) {
+            const double w = product_work(plans[set[k]]);
+            u.idx.push_back(set[k]);
+            u.work += w;
+            acc += w;
+            const size_t left = set.size() - k - 1;
+            if (cut < r - 1 && left > 0 && (acc >= sw * (cut + 1) / r || left < static_cast<size_t>(r - 1 - cut))) {
+                units.push_back(u);
+                u = Unit{{}, true, 0.0};
+                ++cut;
+            }
+        }
+        if (!u.idx.empty()) units.push_back(u);
+    }
+    return units;
+}
+
+// Captures the batch under `shape` into bt's graph; ntt_form and twiddles of -1
+// keep the plans' own.
+void capture_batch(pm_batch& bt, int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
+                   uint32_t* const* c, const pm_batch_shape& shape) {
+    need(shape.runs >= 0 && shape.streams >= 1, "shape: runs must be >= 0 and streams >= 1");
+    need(shape.ntt_form == -1 || shape.ntt_form == PM_NTT_VALU || shape.ntt_form == PM_NTT_MFMA,
+         "shape: ntt_form must be -1, PM_NTT_VALU or PM_NTT_MFMA");
+    need(shape.twiddles == -1 || shape.twiddles == PM_TWIDDLES_TABLE || shape.twiddles == PM_TWIDDLES_GEN,
+         "shape: twiddles must be -1, PM_TWIDDLES_TABLE or PM_TWIDDLES_GEN");
+    std::vector<Unit> units = batch_units(count, plans, shape.runs);
+    std::stable_sort(units.begin(), units.end(), [](const Unit& x, const Unit& y) { return x.work > y.work; });
+    // per batched run: its NTT choice (the matrix-core form where the run's geometry has it)
+    std::vector<NttChoice> choice(units.size());
+    std::vector<std::shared_ptr<pm::FullTables>> keep;
+    bool phased = false;
+    for (size_t u = 0; u < units.size(); ++u) {
+        if (!units[u].batched) continue;
+        phased = true;
+        pm_plan* top = plans[units[u].idx[0]];   // most primes: the run's sort order
+        for (int i : units[u].idx)
+            if (plans[i]->host.K > top->host.K) top = plans[i];
+        NttChoice& ch = choice[u];
+        if (shape.ntt_form >= 0) ch.col_mfma = shape.ntt_form == PM_NTT_MFMA && top->dev.colm_fwd && top->dev.colm_inv;
+        if (shape.twiddles == PM_TWIDDLES_GEN) ch.row_gen = 1;
+        if (shape.twiddles == PM_TWIDDLES_TABLE) {
+            ch.row_gen = 0;
+            auto f = top->full ? top->full : pm::full_tables(*top->tw, top->host);
+            ch.full = f.get();
+            keep.push_back(f);
+        }
+    }
+    const size_t nb = std::max<size_t>(1, std::min(units.size(), static_cast<size_t>(shape.streams)));
+    hipStream_t origin = nullptr;
+    check(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking), "hipStreamCreate");
+    std::vector<hipStream_t> branch(nb, nullptr);
+    std::vector<hipEvent_t> done(nb, nullptr);
+    hipEvent_t fork = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto cleanup = [&] {
+        for (auto e : done)
+            if (e) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        for (auto st : branch)
+            if (st) (void)hipStreamDestroy(st);
+        (void)hipStreamDestroy(origin);
+    };
+    try {
+        for (auto& st : branch) check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto& e : done) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        check(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "hipEventCreate");
+        check(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        try {
+            check(hipEventRecord(fork, origin), "hipEventRecord");
+            for (size_t k = 0; k < nb; ++k) check(hipStreamWaitEvent(branch[k], fork, 0), "wait");
+            // largest work first, each onto the branch with the least work so far
+            std::vector<double> load(nb, 0.0);
+            for (size_t u = 0; u < units.size(); ++u) {
+                const size_t k = static_cast<size_t>(std::min_element(load.begin(), load.end()) - load.begin());
+                if (units[u].batched) {
+                    run_batched(units[u].idx, plans, a, b, c, branch[k], choice[u]);
+                } else {
+                    const int i = units[u].idx[0];
+                    run_device(plans[i], a[i], b[i], c[i], branch[k]);
+                }
+                load[k] += units[u].work;
+            }
+            for (size_t k = 0; k < nb; ++k) {
+                check(hipEventRecord(done[k], branch[k]), "hipEventRecord");
+                check(hipStreamWaitEvent(origin, done[k], 0), "wait");
+            }
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(origin, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        check(hipStreamEndCapture(origin, &graph), "hipStreamEndCapture");
+        check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    } catch (...) {
+        if (graph) (void)hipGraphDestroy(graph);
+        cleanup();
+        throw;
+    }
+    cleanup();
+    if (bt.exec) (void)hipGraphExecDestroy(bt.exec);
+    if (bt.graph) (void)hipGraphDestroy(bt.graph);
+    bt.graph = graph;
+    bt.exec = exec;
+    bt.shape = shape;
+    bt.shape.streams = static_cast<int>(nb);
+    bt.phased = phased;
+    bt.keep = std::move(keep);
+}
+
+// Median milliseconds of one replay of bt's graph: after one warm-up, five
+// samples of two back-to-back replays each.
+double time_batch(const pm_batch& bt, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+    check(hipGraphLaunch(bt.exec, st), "hipGraphLaunch");
+    check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    std::vector<double> t;
+    for (int r = 0; r < 5; ++r) {
+        check(hipEventRecord(e0, st), "hipEventRecord");
+        check(hipGraphLaunch(bt.exec, st), "hipGraphLaunch");
+        check(hipGraphLaunch(bt.exec, st), "hipGraphLaunch");
+        check(hipEventRecord(e1, st), "hipEventRecord");
+        check(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0;
+        check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        t.push_back(ms / 2.0);
+    }
+    std::sort(t.begin(), t.end());
+    return t[t.size() / 2];
+}
+
+int env_int(const char* name, int fallback) {
+    const char* v = std::getenv(name);
+    if (!v || !*v) return fallback;
+    char* end = nullptr;
+    const long x = std::strtol(v, &end, 10);
+    need(end && *end == 0 && x >= 0 && x <= 4096, "PM_BATCH_* must be a non-negative integer");
+    return static_cast<int>(x);
+}
+
+// pm_batch_create's timed choice, dimension by dimension: (runs, streams) with
+// the plans' own forms, then the column-pass form, then the twiddle source;
+// a later candidate replaces the best so far only if at least 1 % faster.
+void tune_batch(pm_batch& bt, int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
+                uint32_t* const* c) {
+    const int runs_env = std::getenv("PM_BATCH_RUNS") ? env_int("PM_BATCH_RUNS", 0) : env_int("PM_BATCH_GROUPS", -1);
+    const int streams_env =
+        std::getenv("PM_BATCH_STREAMS") ? env_int("PM_BATCH_STREAMS", 1) : env_int("PM_BATCH_BRANCHES", -1);
+    const bool tune = env_int("PM_BATCH_TUNE", 1) != 0;
+    if (!tune) {
+        pm_batch_shape sh{runs_env >= 0 ? runs_env : 16, streams_env >= 1 ? streams_env : 16, -1, -1};
+        capture_batch(bt, count, plans, a, b, c, sh);
+        return;
+    }
+    const ProductSets ps = product_sets(plans, count);
+    int groups = 0, grouped = 0;
+    for (size_t s = 0; s < ps.sets.size(); ++s)
+        if (!ps.alone[s]) {
+            ++groups;
+            grouped += static_cast<int>(ps.sets[s].size());
+        }
+    std::vector<pm_batch_shape> cands;
+    auto add = [&](int r, int st) {
+        if (runs_env >= 0) r = runs_env;
+        if (streams_env >= 1) st = streams_env;
+        for (const auto& x : cands)
+            if (x.runs == r && x.streams == st) return;
+        cands.push_back(pm_batch_shape{r, st, -1, -1});
+    };
+    if (groups > 0) {
+        for (int m : {16, 8, 4, 2, 1}) {
+            const int r = std::min(grouped, m * groups);
+            add(r, std::min(r + count - grouped, 16));
+            add(r, std::min(r + count - grouped, 4));
+        }
+    }
+    add(0, 4);
+    add(0, 16);
+    // Timed on the null stream, which lives as long as the process: a graph's
+    // executable keeps state about the stream of its last launch, so the
+    // stream that launched the chosen graph must outlive it.
+    const hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&] {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    try {
+        check(hipEventCreate(&e0), "hipEventCreate");
+        check(hipEventCreate(&e1), "hipEventCreate");
+        pm_batch best;
+        double best_ms = 0;
+        auto consider = [&](const pm_batch_shape& sh) {
+            pm_batch trial;
+            capture_batch(trial, count, plans, a, b, c, sh);
+            const double ms = time_batch(trial, st, e0, e1);
+            if (!best.exec || ms < 0.99 * best_ms) {
+                std::swap(best.graph, trial.graph);
+                std::swap(best.exec, trial.exec);
+                best.shape = trial.shape;
+                best.phased = trial.phased;
+                best.keep.swap(trial.keep);
+                best_ms = ms;
+            }
+        };
+        for (const auto& sh : cands) consider(sh);
+        if (best.phased && !std::getenv("PM_NTT_FORM")) {
+            bool mfma_tables = false;
+            for (int i = 0; i < count; ++i) mfma_tables = mfma_tables || (plans[i]->dev.colm_fwd && plans[i]->dev.colm_inv);
+            pm_batch_shape sh = best.shape;
+            sh.ntt_form = PM_NTT_VALU;
+            consider(sh);
+            if (mfma_tables) {
+                sh.ntt_form = PM_NTT_MFMA;
+                consider(sh);
+            }
+        }
+        if (best.phased && !std::getenv("FHERMA_TWIDDLES")) {
+            pm_batch_shape sh = best.shape;
+            sh.twiddles = PM_TWIDDLES_GEN;
+            consider(sh);
+            sh.twiddles = PM_TWIDDLES_TABLE;
+            consider(sh);
+        }
+        std::swap(bt.graph, best.graph);
+        std::swap(bt.exec, best.exec);
+        bt.shape = best.shape;
+        bt.phased = best.phased;
+        bt.keep.swap(best.keep);
+        bt.ms = best_ms;
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+}
+
+void check_batch_args(int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
+                      uint32_t* const* c, pm_batch** out) {
+    need(out != nullptr && count > 0 && plans && a && b && c, "bad argument");
+    *out = nullptr;
+    for (int i = 0; i < count; ++i) {
+        need(plans[i] && a[i] && b[i] && c[i], "null entry in batch");
+        for (int k = 0; k < i; ++k) need(plans[k] != plans[i], "each product of a batch needs its own plan");
+    }
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -420,7 +834,15 @@ int pm_plan_set_twiddles(pm_plan* plan, int source) {
         need(plan != nullptr, "plan is null");
         need(source == PM_TWIDDLES_TABLE || source == PM_TWIDDLES_GEN,
              "source must be PM_TWIDDLES_TABLE or PM_TWIDDLES_GEN");
+        if (source == PM_TWIDDLES_TABLE && !plan->full) plan->full = pm::full_tables(*plan->tw, plan->host);
         plan->dev.row_gen = source == PM_TWIDDLES_GEN ? 1 : 0;
+        if (source == PM_TWIDDLES_GEN && pm::batch_kernels_ok(plan->dev)) {
+            // nothing of this plan reads the full tables now (hipFree waits for
+            // work in flight, should this release the last copy)
+            plan->full.reset();
+        }
+        plan->dev.tw = plan->full ? plan->full->tw : nullptr;
+        plan->dev.itw = plan->full ? plan->full->itw : nullptr;
     });
 }
 
@@ -512,97 +934,39 @@ int pm_batch_multiply_device(int count, pm_plan* const* plans, const uint32_t* c
     return guarded([&] {
         need(count >= 0 && (count == 0 || (plans && a && b && c)), "null argument");
         for (int i = 0; i < count; ++i) need(plans[i] && a[i] && b[i] && c[i], "null entry in batch");
-        if (stream && batchable(plans, count)) {
-            run_batched(plans, a, b, c, count, static_cast<hipStream_t>(stream));
+        if (!stream) {
+            for (int i = 0; i < count; ++i) run_device(plans[i], a[i], b[i], c[i], stream_of(plans[i], stream));
             return;
         }
-        for (int i = 0; i < count; ++i) run_device(plans[i], a[i], b[i], c[i], stream_of(plans[i], stream));
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        const ProductSets ps = product_sets(plans, count);
+        for (size_t s = 0; s < ps.sets.size(); ++s) {
+            if (ps.alone[s]) {
+                for (int i : ps.sets[s]) run_device(plans[i], a[i], b[i], c[i], st);
+            } else {
+                run_batched(ps.sets[s], plans, a, b, c, st);
+            }
+        }
     });
 }
 
 int pm_batch_create(int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
                     uint32_t* const* c, pm_batch** out) {
     return guarded([&] {
-        need(out != nullptr && count > 0 && plans && a && b && c, "bad argument");
-        *out = nullptr;
+        check_batch_args(count, plans, a, b, c, out);
         std::unique_ptr<pm_batch> bt(new pm_batch());
-        for (int i = 0; i < count; ++i) {
-            need(plans[i] && a[i] && b[i] && c[i], "null entry in batch");
-            for (int k = 0; k < i; ++k) need(plans[k] != plans[i], "each product of a batch needs its own plan");
-            bt->plans.push_back(plans[i]);
-        }
-        hipStream_t origin = nullptr;
-        check(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking), "hipStreamCreate");
-        // Phase-batched: the products split into `groups` runs of consecutive
-        // products, each run's five batched launches on a branch stream, so
-        // that different phases of different runs overlap.
-        int groups = 16, branches = kBranches;
-        if (const char* e = std::getenv("PM_BATCH_GROUPS")) groups = std::max(0, std::atoi(e));
-        if (const char* e = std::getenv("PM_BATCH_BRANCHES")) branches = std::max(1, std::atoi(e));
-        const bool phased = groups > 0 && batchable(plans, count);
-        groups = std::min(groups, count);
-        std::vector<hipStream_t> branch(phased ? groups : std::min(count, branches), nullptr);
-        std::vector<hipEvent_t> done(branch.size(), nullptr);
-        hipEvent_t fork = nullptr;
-        auto cleanup = [&] {
-            for (auto e : done)
-                if (e) (void)hipEventDestroy(e);
-            if (fork) (void)hipEventDestroy(fork);
-            for (auto s : branch)
-                if (s) (void)hipStreamDestroy(s);
-            (void)hipStreamDestroy(origin);
-        };
-        try {
-            for (auto& s : branch) check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
-            for (auto& e : done) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-            check(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "hipEventCreate");
-            check(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            try {
-                check(hipEventRecord(fork, origin), "hipEventRecord");
-                for (size_t s = 0; s < branch.size(); ++s) check(hipStreamWaitEvent(branch[s], fork, 0), "wait");
-                if (phased) {
-                    const int ng = static_cast<int>(branch.size());
-                    for (int g = 0; g < ng; ++g) {
-                        const int i0 = count * g / ng, i1 = count * (g + 1) / ng;
-                        run_batched(plans + i0, a + i0, b + i0, c + i0, i1 - i0, branch[g]);
-                    }
-                } else {
-                    // Products of different shapes, largest first, each on the branch
-                    // with the least work so far, so that the branches end together
-                    // (work: K N (log N + 2) for the NTTs and steps per prime, plus
-                    // N L K / 8 for the contractions).
-                    auto work = [&](int i) {
-                        const pm::HostPlan& h = plans[i]->host;
-                        return static_cast<double>(h.K) * h.N * (h.logN + 2) + static_cast<double>(h.N) * h.L * h.K / 8;
-                    };
-                    std::vector<int> order(count);
-                    for (int i = 0; i < count; ++i) order[i] = i;
-                    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
-                    std::vector<double> load(branch.size(), 0.0);
-                    for (int i : order) {
-                        const size_t k = static_cast<size_t>(std::min_element(load.begin(), load.end()) - load.begin());
-                        run_device(plans[i], a[i], b[i], c[i], branch[k]);
-                        load[k] += work(i);
-                    }
-                }
-                for (size_t s = 0; s < branch.size(); ++s) {
-                    check(hipEventRecord(done[s], branch[s]), "hipEventRecord");
-                    check(hipStreamWaitEvent(origin, done[s], 0), "wait");
-                }
-            } catch (...) {
-                hipGraph_t g = nullptr;
-                (void)hipStreamEndCapture(origin, &g);
-                if (g) (void)hipGraphDestroy(g);
-                throw;
-            }
-            check(hipStreamEndCapture(origin, &bt->graph), "hipStreamEndCapture");
-            check(hipGraphInstantiate(&bt->exec, bt->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-        } catch (...) {
-            cleanup();
-            throw;
-        }
-        cleanup();
-        bt->phased = phased;
+        tune_batch(*bt, count, plans, a, b, c);
+        *out = bt.release();
+    });
+}
+
+int pm_batch_create_shaped(int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
+                           uint32_t* const* c, const pm_batch_shape* shape, pm_batch** out) {
+    return guarded([&] {
+        check_batch_args(count, plans, a, b, c, out);
+        need(shape != nullptr, "shape is null");
+        std::unique_ptr<pm_batch> bt(new pm_batch());
+        capture_batch(*bt, count, plans, a, b, c, *shape);
         *out = bt.release();
     });
 }
@@ -611,6 +975,14 @@ int pm_batch_launch(pm_batch* batch, void* stream) {
     return guarded([&] {
         need(batch != nullptr, "batch is null");
         check(hipGraphLaunch(batch->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
+    });
+}
+
+int pm_batch_get_shape(const pm_batch* batch, pm_batch_shape* shape, double* ms) {
+    return guarded([&] {
+        need(batch != nullptr, "batch is null");
+        if (shape) *shape = batch->shape;
+        if (ms) *ms = batch->ms;
     });
 }
 

@@ -42,8 +42,12 @@ struct DevPlan {
     const uint8_t* colm_fwd;
     const uint8_t* colm_inv;
     int col_mfma;             // 1: the column passes run on the matrix cores (tables present)
+    // Full tables, null unless the row pass reads its twiddles from tables or
+    // the geometry needs the generic kernels.
     const uint2* tw;          // K x N: (psi^bitrev(i), Shoup companion)
-    const uint2* itw;         // K x N: (psi^-bitrev(i), Shoup companion)
+    const uint2* itw;         // K x N: (psi^-bitrev(i) 2^32 mod p, Montgomery companion)
+    const uint2* ctw;         // K x R: the first R pairs of each prime's tw (the column pass's)
+    const uint2* citw;        // K x R: the same of itw
     // Row-pass twiddles, generated: the twiddle of row rho, stage distance 2^e,
     // block b is Z[rho][e] * Om[b] (pm_ntt.hip).
     const uint2* rz;          // K x R x logS: (Z 2^32 mod p, Shoup companion), forward
@@ -121,12 +125,32 @@ void launch_ntt_rows_fused_generic(const DevPlan& P, uint32_t* A, const uint32_t
 // [0, 2p) (for the forward NTT of a product when P.lazy is set), else canonical.
 void launch_decompose_mfma(const DevPlan& P, const uint32_t* limbs, uint32_t* res, hipStream_t st,
                            bool lazy_out = false);
-// Batches of up to kBatchMax products that share N, the primes and L (so
-// every grid and every prime table): one plan P for all, and per product its
-// operands a, b, answer c, residue buffers A, B and the tables that depend on
-// q, all passed as one kernel argument. Each batched launch covers every
-// product at once (blockIdx.z picks the product) and is large enough to run in
-// steady state.
+// The tables of one product that depend on its K, L or q, as its batched
+// launches read them (one per plan, in device memory, immutable once the plan
+// exists; the kernels read it with scalar loads).
+struct ProductDesc {
+    int K, L;
+    int dec_ksteps, crt_ksteps;
+    uint32_t dec_magic, pad;
+    const int8_t* dec_digits;
+    const uint64_t* dec_corr;
+    const int8_t* dec_digits_b;
+    const uint64_t* dec_corr_b;
+    const uint32_t* crt_tab;
+    const double* crt_fc;
+    const double* nq;
+    const uint32_t* crt_lc;
+};
+
+// Batches of up to kBatchMax products that share N and whose prime lists are
+// prefixes of one list (the prime rule walks one sequence per N, so products
+// at one N differ only in how many primes they take): one plan P, that of the
+// product with the most primes, supplies the geometry, the per-prime constants
+// and the twiddles; per product, its operands a, b, answer c, residue buffers
+// A, B, its prime count K and its own tables d. Each batched launch covers
+// every product at once (blockIdx.z picks the product; workgroups of primes
+// past a product's K return at once) and is large enough to run in steady
+// state. Decompose and CRT launches take products of one K-step count.
 constexpr int kBatchMax = 32;
 struct BatchArgs {
     const uint32_t* a[kBatchMax];
@@ -134,11 +158,19 @@ struct BatchArgs {
     uint32_t* c[kBatchMax];
     uint32_t* A[kBatchMax];
     uint32_t* B[kBatchMax];
-    const uint32_t* crt_tab[kBatchMax];
-    const double* crt_fc[kBatchMax];
-    const double* nq[kBatchMax];
-    const uint32_t* crt_lc[kBatchMax];
+    const ProductDesc* d[kBatchMax];
+    int K[kBatchMax];
 };
+
+// A pointer into the constant address space: loads at uniform addresses become
+// scalar loads (the kernels never write what it points to).
+template <class T>
+using kconst_ptr = const T __attribute__((address_space(4)))*;
+template <class T>
+__device__ __forceinline__ kconst_ptr<T> as_const(const T* p) {
+    return (kconst_ptr<T>)p;
+}
+
 bool batch_kernels_ok(const DevPlan& P);   // register NTT kernels exist for this geometry
 // The column pass on the matrix cores, one launch (ba: a batch of `count`
 // products, as launch_ntt_cols_batch; null: X alone).

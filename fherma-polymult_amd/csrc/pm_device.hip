@@ -20,6 +20,7 @@
 #include "pm_bigint.h"
 
 #include <algorithm>
+#include <stdexcept>
 
 namespace pm {
 namespace {
@@ -301,6 +302,7 @@ void launch_decompose_valu_range(const DevPlan& P, const uint32_t* limbs, uint32
 
 void launch_ntt_cols_generic(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     if (P.logR == 0) return;
+    if (!P.tw || !P.itw) throw std::logic_error("the generic NTT kernels need the full twiddle tables");
     const TileGeom g = col_geom(P);
     const dim3 grid(1u << (P.logS - g.logC), P.K);
     const size_t lds = sizeof(uint32_t) << (g.logC + g.logU);
@@ -311,6 +313,7 @@ void launch_ntt_cols_generic(const DevPlan& P, uint32_t* X, bool forward, hipStr
 }
 
 void launch_ntt_rows_generic(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
+    if (!P.tw || !P.itw) throw std::logic_error("the generic NTT kernels need the full twiddle tables");
     const TileGeom g = row_geom(P, 13);
     const dim3 grid(1u << (P.logR - g.logU), P.K);
     const size_t lds = sizeof(uint32_t) << (g.logC + g.logU);
@@ -321,6 +324,7 @@ void launch_ntt_rows_generic(const DevPlan& P, uint32_t* X, bool forward, hipStr
 }
 
 void launch_ntt_rows_fused_generic(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st) {
+    if (!P.tw || !P.itw) throw std::logic_error("the generic NTT kernels need the full twiddle tables");
     const TileGeom g = row_geom(P, 12);
     const dim3 grid(1u << (P.logR - g.logU), P.K);
     const size_t lds = 2 * (sizeof(uint32_t) << (g.logC + g.logU));

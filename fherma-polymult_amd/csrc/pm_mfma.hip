@@ -219,14 +219,19 @@ k_decompose_mfma(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
     decompose_tile<KT, kLazyOut>(limbs, res, P, k0, count);
 }
 
-// A batch: blockIdx.z = 2 (product) + (0: a into A, 1: b into B).
+// A batch: blockIdx.z = 2 (product) + (0: a into A, 1: b into B); every
+// product's L needs KT K-steps.
 template <int KT, bool kLazyOut>
 __global__ void __launch_bounds__(64 * kWaves) k_decompose_mfma_batch(DevPlan P, BatchArgs ba) {
     const int i = blockIdx.z >> 1;
     const bool second = (blockIdx.z & 1) != 0;   // b: scaled by the pointwise constant
+    const auto d = as_const(ba.d[i]);
     DevPlan Q = P;
-    Q.dec_digits = second ? P.dec_digits_b : P.dec_digits;
-    Q.dec_corr = second ? P.dec_corr_b : P.dec_corr;
+    Q.K = d->K;
+    Q.L = d->L;
+    Q.dec_magic = d->dec_magic;
+    Q.dec_digits = second ? d->dec_digits_b : d->dec_digits;
+    Q.dec_corr = second ? d->dec_corr_b : d->dec_corr;
     decompose_tile<KT, kLazyOut>(second ? ba.b[i] : ba.a[i], second ? ba.B[i] : ba.A[i], Q, 0, P.N);
 }
 
@@ -653,15 +658,19 @@ k_crt_mfma(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, DevPlan P, 
     crt_tiles<KS>(Y, c, P, k0, count, blockIdx.x, gridDim.x);
 }
 
-// A batch: blockIdx.z = product, with that product's q-dependent tables.
+// A batch: blockIdx.z = product, with that product's tables; every product's
+// K needs KS K-steps.
 template <int KS>
 __global__ void __launch_bounds__(64 * kWaves) k_crt_mfma_batch(DevPlan P, BatchArgs ba) {
     const int i = blockIdx.z;
+    const auto d = as_const(ba.d[i]);
     DevPlan Q = P;
-    Q.crt_tab = ba.crt_tab[i];
-    Q.crt_fc = ba.crt_fc[i];
-    Q.nq = ba.nq[i];
-    Q.crt_lc = ba.crt_lc[i];
+    Q.K = d->K;
+    Q.L = d->L;
+    Q.crt_tab = d->crt_tab;
+    Q.crt_fc = d->crt_fc;
+    Q.nq = d->nq;
+    Q.crt_lc = d->crt_lc;
     crt_tiles<KS>(ba.A[i], ba.c[i], Q, 0, P.N, blockIdx.x, gridDim.x);
 }
 

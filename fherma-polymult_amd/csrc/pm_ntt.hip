@@ -20,6 +20,8 @@
 
 #include "pm_arith.h"
 
+#include <stdexcept>
+
 namespace pm {
 
 void launch_ntt_cols_generic(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st);
@@ -251,7 +253,8 @@ struct Buf {
     }
 };
 
-// The column pass's twiddles from the table through a buffer: phase A's are
+// The column pass's twiddles (the first R pairs of the prime's table, all
+// that its stages use) through a buffer: phase A's are
 // wave-uniform (scalar loads), phase B's index (B >> (e + 1)) + (t << (3 - e)) + blk
 // splits into a lane part (t << (3 - e), four values fixed per thread) and a
 // uniform part.
@@ -261,8 +264,8 @@ struct BufTw {
     const uint2* __restrict__ tab;
     Buf buf;
     uint32_t B, vt[4];
-    __device__ __forceinline__ BufTw(const uint2* tab_, uint32_t N, uint32_t B_, uint32_t t)
-        : tab(tab_), buf(tab_, 8u * N), B(B_) {
+    __device__ __forceinline__ BufTw(const uint2* tab_, uint32_t entries, uint32_t B_, uint32_t t)
+        : tab(tab_), buf(tab_, 8u * entries), B(B_) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) vt[e] = 8u * (t << (3 - e));
     }
@@ -288,7 +291,7 @@ __device__ __forceinline__ void col_pass(uint32_t* __restrict__ X, const DevPlan
     const Buf x(X + static_cast<size_t>(j) * P.N, 4u * P.N);
     const uint32_t col = (blockIdx.x << LOGC) + c;
     const uint32_t va = 4u * (t * S + col), vb = 4u * (16u * t * S + col);   // phase A / phase B rows
-    const BufTw<!kFwd> tw((kFwd ? P.tw : P.itw) + static_cast<size_t>(j) * P.N, P.N, M, t);
+    const BufTw<!kFwd> tw((kFwd ? P.ctw : P.citw) + static_cast<size_t>(j) * M, M, M, t);
     uint32_t* la = lds + t * C + c;          // phase A slots: + (TPS i + 17 k) C
     uint32_t* lb = lds + 17u * t * C + c;    // phase B slots: + k C
     const uint32_t p = P.pc[j].p;
@@ -337,6 +340,7 @@ __global__ void __launch_bounds__(512) k_ntt_col_reg(uint32_t* __restrict__ X, D
 template <int LOGM, bool kFwd, int LOGC, int kMode>
 __global__ void __launch_bounds__(512) k_ntt_col_batch(DevPlan P, BatchArgs ba) {
     const int i = kFwd ? blockIdx.z >> 1 : blockIdx.z;
+    if (static_cast<int>(blockIdx.y) >= ba.K[i]) return;   // primes past this product's K
     col_pass<LOGM, kFwd, LOGC, kMode>(kFwd && (blockIdx.z & 1) ? ba.B[i] : ba.A[i], P);
 }
 
@@ -494,6 +498,7 @@ k_ntt_rows_fused_reg(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, 
 // A batch: blockIdx.z = product.
 template <int LOGM, bool kGen, int kMode>
 __global__ void __launch_bounds__(512) k_ntt_rows_fused_batch(DevPlan P, BatchArgs ba, uint32_t logG) {
+    if (static_cast<int>(blockIdx.y) >= ba.K[blockIdx.z]) return;   // primes past this product's K
     rows_fused<LOGM, kGen, kMode>(ba.A[blockIdx.z], ba.B[blockIdx.z], P, logG);
 }
 
@@ -566,6 +571,11 @@ void col_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
     }
 }
 
+// The row pass reading its twiddles from the full tables needs them present.
+void need_full_tables(const DevPlan& P) {
+    if (!P.row_gen && (!P.tw || !P.itw)) throw std::logic_error("row pass from tables without the full tables");
+}
+
 template <int LOGM, int kMode>
 void row_launch3(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st, dim3 grid, dim3 block, size_t lds,
                  uint32_t logG) {
@@ -581,6 +591,7 @@ void row_launch3(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st, di
 
 template <int LOGM>
 void row_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
+    need_full_tables(P);
     const uint32_t logG = row_logG(P);
     const dim3 grid(1u << (P.logR - logG), P.K);
     const dim3 block(1u << (logG + LOGM - 4));
@@ -594,6 +605,7 @@ void row_launch(const DevPlan& P, uint32_t* X, bool forward, hipStream_t st) {
 
 template <int LOGM>
 void fused_launch_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st) {
+    need_full_tables(P);
     const uint32_t logG = row_logG(P);
     const dim3 grid(1u << (P.logR - logG), P.K, count);
     const dim3 block(1u << (logG + LOGM - 4));
@@ -609,6 +621,7 @@ void fused_launch_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStr
 
 template <int LOGM>
 void fused_launch(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st) {
+    need_full_tables(P);
     const uint32_t logG = row_logG(P);
     const dim3 grid(1u << (P.logR - logG), P.K);
     const dim3 block(1u << (logG + LOGM - 4));

@@ -224,8 +224,8 @@ __device__ __forceinline__ void col_mfma(uint32_t* __restrict__ X, const DevPlan
         }
 }
 
-// At most 168 VGPRs (three waves per SIMD): with two, the 2 K * 8 waves of a pass at
-// N = 65536 (2144 at K = 134) would not all be resident at once.
+// The forward kernel needs 172 VGPRs (two waves per SIMD), the inverse 168
+// (three); capping the forward one at 168 made the compiler spill, slower.
 template <bool kFwd>
 __global__ void __launch_bounds__(64 * kWaves)
 k_ntt_col_mfma(uint32_t* __restrict__ X, DevPlan P) {
@@ -237,6 +237,7 @@ template <bool kFwd>
 __global__ void __launch_bounds__(64 * kWaves)
 k_ntt_col_mfma_batch(DevPlan P, BatchArgs ba) {
     const int i = kFwd ? blockIdx.z >> 1 : blockIdx.z;
+    if (static_cast<int>(blockIdx.y) >= ba.K[i]) return;   // primes past this product's K
     col_mfma<kFwd>(kFwd && (blockIdx.z & 1) ? ba.B[i] : ba.A[i], P, blockIdx.y, blockIdx.x);
 }
 

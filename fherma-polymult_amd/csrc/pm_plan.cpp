@@ -330,21 +330,25 @@ HostPlan make_plan(uint32_t N, uint32_t W, const uint32_t* q, int L) {
     return pl;
 }
 
-void build_twiddles(const HostPlan& pl, bool forward, int j0, int j1, uint32_t* out) {
+void build_twiddles(const HostPlan& pl, bool forward, int j0, int j1, uint32_t* out, uint32_t count) {
     const uint32_t N = pl.N;
-    std::vector<uint32_t> powers(N);
+    if (count == 0 || count > N || (count & (count - 1)) != 0) throw std::invalid_argument("twiddle count");
+    std::vector<uint32_t> powers(count < N ? 0 : N);
     for (int j = j0; j < j1; ++j) {
         const uint32_t p = pl.p[j];
         const uint32_t root = forward ? pl.psi[j] : pl.ipsi[j];
-        uint32_t x = 1;
-        for (uint32_t e = 0; e < N; ++e) {
-            powers[e] = x;
-            x = mul_mod(x, root, p);
+        if (count == N) {
+            uint32_t x = 1;
+            for (uint32_t e = 0; e < N; ++e) {
+                powers[e] = x;
+                x = mul_mod(x, root, p);
+            }
         }
-        uint32_t* o = out + 2 * static_cast<size_t>(j - j0) * N;
+        uint32_t* o = out + 2 * static_cast<size_t>(j - j0) * count;
         const uint32_t pinv = 0u - pl.pneg[j];
-        for (uint32_t i = 0; i < N; ++i) {
-            const uint32_t w = powers[pl.logN ? bitrev32(i, pl.logN) : 0];
+        for (uint32_t i = 0; i < count; ++i) {
+            const uint32_t e = pl.logN ? bitrev32(i, pl.logN) : 0;
+            const uint32_t w = count == N ? powers[e] : exp_mod(root, e, p);
             if (forward) {
                 o[2 * i] = w;
                 o[2 * i + 1] = shoup_companion(w, p);

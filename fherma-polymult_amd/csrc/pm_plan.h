@@ -49,11 +49,12 @@ void ntt_geometry(uint32_t logN, uint32_t& logS, uint32_t& logR);
 // The prime walk alone: the K primes for (N, bits(q)), or an exception.
 std::vector<uint32_t> choose_primes(uint32_t N, int qbits);
 
-// Twiddles w = psi^bitrev_N(i) (forward) or psi^-bitrev_N(i) (inverse), i < N,
-// for primes [j0, j1) of the plan: forward as Shoup pairs (w, floor(w 2^32 / p)),
-// inverse as Montgomery pairs (w 2^32 mod p, that times p^-1 mod 2^32), which
-// the inverse butterflies take for one VALU op less.
-void build_twiddles(const HostPlan& pl, bool forward, int j0, int j1, uint32_t* out_pairs);
+// Twiddles w = psi^bitrev_N(i) (forward) or psi^-bitrev_N(i) (inverse), i < count
+// (count a power of two <= N), for primes [j0, j1) of the plan, `count` pairs per
+// prime: forward as Shoup pairs (w, floor(w 2^32 / p)), inverse as Montgomery
+// pairs (w 2^32 mod p, that times p^-1 mod 2^32), which the inverse butterflies
+// take for one VALU op less. The first R entries are all the column pass reads.
+void build_twiddles(const HostPlan& pl, bool forward, int j0, int j1, uint32_t* out_pairs, uint32_t count);
 
 // Row-pass twiddle factors for primes [j0, j1): the twiddle psi^bitrev_N(idx)
 // of row rho (S contiguous elements), stage distance 2^e, block b is

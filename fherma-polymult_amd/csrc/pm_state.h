@@ -33,6 +33,7 @@ inline void* dev_alloc(size_t bytes) {
 }
 
 struct Twiddles;        // device twiddle tables shared between plans (pm_api.cpp)
+struct FullTables;      // the full K x N tables, built when a plan needs them (pm_api.cpp)
 struct HostPipeline;    // host transfer pipeline of a plan (pm_host.cpp)
 void destroy_pipeline(HostPipeline* hp);
 
@@ -44,9 +45,11 @@ struct pm_plan {
     pm::DevPlan dev{};
     int form = PM_FORM_PLAIN;
     std::shared_ptr<pm::Twiddles> tw;
+    std::shared_ptr<pm::FullTables> full;   // null unless table-mode rows or the generic kernels need them
     std::vector<void*> owned;            // device allocations
     uint32_t* A = nullptr;               // K x N residues of a, then y
     uint32_t* B = nullptr;               // K x N residues of b
+    const pm::ProductDesc* desc = nullptr;   // its tables as batched launches read them (device)
     pm::HostPipeline* pipe = nullptr;    // host-pointer calls (created on first use)
     hipStream_t stream = nullptr;
     hipEvent_t ev[PM_NPHASES + 1] = {};    // kernel start events of the phase profile, and spares
@@ -84,6 +87,25 @@ struct pm_plan {
             return d;
         });
         mfma_ready = true;
+    }
+
+    // The device copy of this plan's per-product tables (after ensure_mfma).
+    void upload_desc() {
+        pm::ProductDesc d{};
+        d.K = dev.K;
+        d.L = dev.L;
+        d.dec_ksteps = dev.dec_ksteps;
+        d.crt_ksteps = dev.crt_ksteps;
+        d.dec_magic = dev.dec_magic;
+        d.dec_digits = dev.dec_digits;
+        d.dec_corr = dev.dec_corr;
+        d.dec_digits_b = dev.dec_digits_b;
+        d.dec_corr_b = dev.dec_corr_b;
+        d.crt_tab = dev.crt_tab;
+        d.crt_fc = dev.crt_fc;
+        d.nq = dev.nq;
+        d.crt_lc = dev.crt_lc;
+        desc = upload(std::vector<pm::ProductDesc>{d});
     }
 };
 

@@ -31,6 +31,12 @@ _u32p = ctypes.POINTER(ctypes.c_uint32)
 _vp = ctypes.c_void_p
 
 
+class BatchShape(ctypes.Structure):
+    """pm_batch_shape: runs, streams, ntt_form, twiddles."""
+    _fields_ = [("runs", ctypes.c_int), ("streams", ctypes.c_int), ("ntt_form", ctypes.c_int),
+                ("twiddles", ctypes.c_int)]
+
+
 class PolymulError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__("%s (status %d)" % (msg, code))
@@ -62,6 +68,10 @@ def _load() -> ctypes.CDLL:
                                                     ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
         "pm_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                            ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+        "pm_batch_create_shaped": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                  ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                  ctypes.POINTER(BatchShape), ctypes.POINTER(_vp)]),
+        "pm_batch_get_shape": (ctypes.c_int, [_vp, ctypes.POINTER(BatchShape), ctypes.POINTER(ctypes.c_double)]),
         "pm_batch_launch": (ctypes.c_int, [_vp, _vp]),
         "pm_batch_phased": (ctypes.c_int, [_vp]),
         "pm_plan_set_twiddles": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -391,13 +401,36 @@ def batch_multiply_device(plans, a_bufs, b_bufs, c_bufs, stream=None) -> None:
 
 
 class Batch:
-    """A batch of products captured once as a hipGraph; launch() replays it."""
+    """A batch of products captured once as a hipGraph; launch() replays it.
 
-    def __init__(self, plans, a_bufs, b_bufs, c_bufs):
+    Without `shape`, creation times candidate shapes and keeps the fastest
+    (pm_batch_create); with shape = (runs, streams, ntt_form, twiddles) it
+    captures that one (pm_batch_create_shaped)."""
+
+    def __init__(self, plans, a_bufs, b_bufs, c_bufs, shape=None):
         h = _vp()
-        _check(lib.pm_batch_create(*_batch_args(plans, a_bufs, b_bufs, c_bufs), ctypes.byref(h)))
+        args = _batch_args(plans, a_bufs, b_bufs, c_bufs)
+        if shape is None:
+            _check(lib.pm_batch_create(*args, ctypes.byref(h)))
+        else:
+            sh = BatchShape(*[int(x) for x in shape])
+            _check(lib.pm_batch_create_shaped(*args, ctypes.byref(sh), ctypes.byref(h)))
         self._h = h.value
         self._keep = (list(plans), list(a_bufs), list(b_bufs), list(c_bufs))
+
+    @property
+    def shape(self):
+        """(runs, streams, ntt_form, twiddles) as captured."""
+        sh = BatchShape()
+        _check(lib.pm_batch_get_shape(_vp(self._h), ctypes.byref(sh), None))
+        return (sh.runs, sh.streams, sh.ntt_form, sh.twiddles)
+
+    @property
+    def tuned_ms(self) -> float:
+        """Median replay (ms) of the shape the timed creation chose, else 0."""
+        ms = ctypes.c_double()
+        _check(lib.pm_batch_get_shape(_vp(self._h), None, ctypes.byref(ms)))
+        return ms.value
 
     def launch(self, stream=None) -> None:
         _check(lib.pm_batch_launch(_vp(self._h), _sv(stream)))

@@ -142,28 +142,62 @@ int pm_plan_tune_host(pm_plan* plan, int* mode, double* ms_per_product);
 /* A batch of independent products, each with its own plan (N and q may differ),
  * all device pointers, enqueued on `stream`. No counterpart in the reference:
  * it is for callers with many products per call (uneven batches). With an
- * explicit stream, distinct plans that share N, L and the primes in the
- * matrix-core form run as phase-batched launches (each kernel covers every
- * product); otherwise product by product (on each plan's stream if `stream`
- * is null). */
+ * explicit stream, products of distinct plans in the matrix-core form at one N
+ * (any q width: their prime lists are prefixes of one list) run as batched
+ * launches, one kernel per phase covering all of them (decompose and CRT: one
+ * per K-step count); the rest product by product. Without a stream, product by
+ * product on each plan's stream. */
 int pm_batch_multiply_device(int count, pm_plan* const* plans, const uint32_t* const* a,
                              const uint32_t* const* b, uint32_t* const* c, void* stream);
 
+/* The layout of a captured batch.
+ *   runs      products that can share batched launches (matrix-core form, one
+ *             N) are cut into about this many runs, each set of one N into a
+ *             share by its work, as contiguous slices of equal work after
+ *             sorting by K; each run is one chain of batched launches. 0: every
+ *             product as its own chain of launches (no batched launches).
+ *   streams   graph branches the runs (or chains) are spread over, largest
+ *             work first onto the least-loaded branch.
+ *   ntt_form  pm_ntt_form of the batched runs' column passes, or -1 for the
+ *             plans' own.
+ *   twiddles  pm_twiddles of the batched runs' row passes, or -1 for the plans'
+ *             own. */
+typedef struct {
+    int runs;
+    int streams;
+    int ntt_form;
+    int twiddles;
+} pm_batch_shape;
+
 /* The same batch captured once as a hipGraph and replayed by pm_batch_launch:
- * one launch per batch, no per-kernel host overhead. Products that share N, L
- * and the primes in the matrix-core form are split into up to 16 groups of
- * consecutive products (PM_BATCH_GROUPS), each group captured as five
- * phase-batched launches on a stream of its own; other batches as every
- * product's kernel chain, largest first, on the least-loaded of four streams.
- * The device pointers and the plans' forms and twiddle sources are fixed at
+ * one launch per batch, no per-kernel host overhead. pm_batch_create picks the
+ * shape by timing: it captures the candidate shapes (runs, streams; then the
+ * column-pass form and the row-pass twiddle source of the batched runs), replays
+ * each (one warm-up, the median of five) and keeps the fastest, so creation
+ * runs the batch a few dozen times: a and b must hold readable device data and
+ * c receives the products. No counterpart in the reference, which times only
+ * its copy modes (solve.cu:817-862). The environment overrides a dimension:
+ * PM_BATCH_RUNS, PM_BATCH_STREAMS, PM_NTT_FORM, FHERMA_TWIDDLES (the last two
+ * as the plans' own); PM_BATCH_TUNE=0 skips the timing (runs 16, streams 16,
+ * the plans' own forms). pm_batch_create_shaped captures one given shape and
+ * times nothing. The device pointers and the plans' forms are fixed at
  * creation; each product needs its own plan, and the plans must outlive the
- * batch. A batch works in its plans' residue buffers: its launches must not
- * overlap each other or other calls on those plans. */
+ * batch, as must every stream a batch was launched on (pm_batch_create times
+ * on the null stream). A batch works in its plans' residue buffers: its
+ * launches must not overlap each other or other calls on those plans. */
 typedef struct pm_batch pm_batch;
 int pm_batch_create(int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
                     uint32_t* const* c, pm_batch** out);
+int pm_batch_create_shaped(int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
+                           uint32_t* const* c, const pm_batch_shape* shape, pm_batch** out);
 int pm_batch_launch(pm_batch* batch, void* stream);
-/* 1 if the batch was captured as phase-batched launches, 0 if as per-product chains. */
+/* The batch's shape as captured (streams: the branches used; ntt_form and
+ * twiddles -1 where the plans' own were kept) and, when pm_batch_create timed
+ * it, the median replay in ms of the chosen shape (else 0). Either output may
+ * be null. */
+int pm_batch_get_shape(const pm_batch* batch, pm_batch_shape* shape, double* ms);
+/* 1 if any products of the batch were captured as batched launches, 0 if every
+ * product as its own chain. */
 int pm_batch_phased(const pm_batch* batch);
 void pm_batch_destroy(pm_batch* batch);
 

@@ -260,8 +260,8 @@ def _qbits(q):
 
 @pytest.mark.parametrize("N,W", [(4096, 256), (32768, 868), (65536, 2048)])
 def test_phase_batched_graph_matches_gmp(golden, N, W):
-    """Products that share N, L and the primes are captured as five phase-batched
-    launches; two replays, eager products in between, every answer vs GMP."""
+    """Products that share N, L and the primes captured as batched launches, one
+    run per product; two replays, eager products in between, every answer vs GMP."""
     cases = sorted((c for c in golden.values() if c["N"] == N and c["W"] == W), key=lambda c: c["name"])
     plans, bufs, keep = [], [], []
     for case in cases:
@@ -274,7 +274,9 @@ def test_phase_batched_graph_matches_gmp(golden, N, W):
         keep.append(case)
     assert len(plans) >= 3
     say("phase-batched graph", len(plans), "products, N", N, "W", W)
-    batch = polymult.Batch(plans, [x[0] for x in bufs], [x[1] for x in bufs], [x[2] for x in bufs])
+    # one run per product on 16 branches (the timed choice may prefer chains)
+    batch = polymult.Batch(plans, [x[0] for x in bufs], [x[1] for x in bufs], [x[2] for x in bufs],
+                           shape=(len(plans), 16, -1, -1))
     assert batch.phased
     for rep in range(2):
         for _, _, dc in bufs:
@@ -292,8 +294,9 @@ def test_phase_batched_graph_matches_gmp(golden, N, W):
 
 def test_phase_batched_chunks_match_single_products():
     """40 products (two launches of at most 32) at N = 256, W = 128, random q:
-    the phase-batched results equal the product-by-product ones, through the
-    graph and through pm_batch_multiply_device on an explicit stream."""
+    the batched results (one run in the graph) equal the product-by-product
+    ones, through the graph and through pm_batch_multiply_device on an
+    explicit stream."""
     N, W, n = 256, 128, 40
     pts = [oracle.gen_point(7000 + i, N, W) for i in range(n)]
     pts = [pt for pt in pts if _qbits(pt[0]) == W]
@@ -308,7 +311,7 @@ def test_phase_batched_chunks_match_single_products():
     polymult.synchronize()
     want = [c.download((N, L)) for c in ref]
     say("phase-batched chunks", len(plans))
-    batch = polymult.Batch(plans, da, db, out)
+    batch = polymult.Batch(plans, da, db, out, shape=(1, 1, -1, -1))
     assert batch.phased
     batch.launch()
     polymult.synchronize()
