@@ -674,7 +674,7 @@ int env_int(const char* name, int fallback) {
     if (!v || !*v) return fallback;
     char* end = nullptr;
     const long x = std::strtol(v, &end, 10);
-    need(end && *end == 0 && x >= 0 && x <= 4096, "PM_BATCH_* must be a non-negative integer");
+    need(end && *end == 0 && x >= 0 && x <= 4096, (std::string(name) + " must be a non-negative integer").c_str());
     return static_cast<int>(x);
 }
 
@@ -897,6 +897,17 @@ int pm_multiply(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c
     return guarded([&] {
         need(plan && a && b && c, "null argument");
         pm::host_multiply(plan, a, b, c);
+    });
+}
+
+int pm_batch_multiply_host(int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
+                           uint32_t* const* c, int depth) {
+    return guarded([&] {
+        need(count >= 0 && (count == 0 || (plans && a && b && c)), "null argument");
+        for (int i = 0; i < count; ++i) need(plans[i] && a[i] && b[i] && c[i], "null entry in batch");
+        if (depth <= 0) depth = env_int("PM_HOST_DEPTH", 3);
+        need(depth >= 1 && depth <= 16, "depth must be 1 .. 16");
+        pm::host_multiply_batch(count, plans, a, b, c, depth);
     });
 }
 

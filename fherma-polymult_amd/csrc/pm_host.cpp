@@ -491,6 +491,52 @@ void host_multiply_sized(pm_plan* pl, const uint32_t* a, const uint32_t* b, std:
     run(pl, hp, a, b, out);
 }
 
+// Products on host memory, up to `depth` at once, each through its own plan's
+// pipeline (its own pinned buffers, streams and copy threads), so that one
+// product's input copies overlap another's kernels and a third's output. The
+// products of one plan run one after another.
+void host_multiply_batch(int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
+                         uint32_t* const* c, int depth) {
+    if (count <= 0) return;
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    for (int i = 0; i < count; ++i) pipeline(plans[i]);   // created here, on the caller's thread
+    std::vector<std::unique_ptr<std::mutex>> locks;
+    std::vector<int> lock_of(count);
+    for (int i = 0; i < count; ++i) {
+        lock_of[i] = i;
+        for (int k = 0; k < i; ++k)
+            if (plans[k] == plans[i]) {
+                lock_of[i] = lock_of[k];
+                break;
+            }
+        locks.emplace_back(new std::mutex());
+    }
+    std::atomic<int> next{0};
+    std::atomic<bool> failed{false};
+    std::exception_ptr first;
+    std::mutex first_mutex;
+    auto work = [&](bool own_thread) {
+        try {
+            if (own_thread) check(hipSetDevice(dev), "hipSetDevice");
+            for (int i; !failed.load() && (i = next.fetch_add(1)) < count;) {
+                std::lock_guard<std::mutex> lock(*locks[lock_of[i]]);
+                host_multiply(plans[i], a[i], b[i], c[i]);
+            }
+        } catch (...) {
+            std::lock_guard<std::mutex> lock(first_mutex);
+            if (!first) first = std::current_exception();
+            failed = true;
+        }
+    };
+    std::vector<std::thread> threads;
+    const int n = std::max(1, std::min(depth, count));
+    for (int t = 1; t < n; ++t) threads.emplace_back(work, true);
+    work(false);
+    for (auto& t : threads) t.join();
+    if (first) std::rethrow_exception(first);
+}
+
 void host_set_modes(pm_plan* pl, int in_mode, int out_mode) {
     HostPipeline& hp = pipeline(pl);
     hp.in_mode = in_mode;

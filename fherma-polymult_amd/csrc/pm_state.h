@@ -134,6 +134,9 @@ uint32_t* scratch_on_current_device();
 
 // A product on host pointers through the plan's pipeline (pm_host.cpp).
 void host_multiply(pm_plan* pl, const uint32_t* a, const uint32_t* b, uint32_t* c);
+// Products on host pointers, up to `depth` at once through their plans' pipelines.
+void host_multiply_batch(int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
+                         uint32_t* const* c, int depth);
 // The same with the answer's memory supplied late (fherma_run): size_out runs
 // on the pipeline's helper thread while the inputs stream, and get_out, called
 // after it has finished, gives the memory just before the first answer word is

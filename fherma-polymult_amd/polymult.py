@@ -66,6 +66,8 @@ def _load() -> ctypes.CDLL:
         "pm_multiply_device_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "pm_batch_multiply_device": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                                     ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp]),
+        "pm_batch_multiply_host": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                                  ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.c_int]),
         "pm_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                            ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
         "pm_batch_create_shaped": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
@@ -398,6 +400,27 @@ def _batch_args(plans, a_bufs, b_bufs, c_bufs):
 
 def batch_multiply_device(plans, a_bufs, b_bufs, c_bufs, stream=None) -> None:
     _check(lib.pm_batch_multiply_device(*_batch_args(plans, a_bufs, b_bufs, c_bufs), _sv(stream)))
+
+
+def batch_multiply_host(plans, a_list, b_list, depth=0, out=None):
+    """Products of numpy operands through the plans' host pipelines, up to
+    `depth` at once (pm_batch_multiply_host); returns the answers (or fills
+    the arrays in `out`)."""
+    n = len(plans)
+    if not (len(a_list) == len(b_list) == n):
+        raise ValueError("plans and operands differ in number")
+    a_list = [p._operand(a) for p, a in zip(plans, a_list)]
+    b_list = [p._operand(b) for p, b in zip(plans, b_list)]
+    if out is None:
+        out = [np.empty((p.N, p.L), dtype=np.uint32) for p in plans]
+    elif len(out) != n or any(not (isinstance(c, np.ndarray) and c.dtype == np.uint32 and c.shape == (p.N, p.L)
+                                   and c.flags["C_CONTIGUOUS"]) for p, c in zip(plans, out)):
+        raise ValueError("out must hold one C-contiguous uint32 (N, L) array per product")
+    arr = lambda xs: (_vp * n)(*xs)  # noqa: E731
+    addr = lambda xs: arr([x.ctypes.data for x in xs])  # noqa: E731
+    _check(lib.pm_batch_multiply_host(n, arr([p.handle for p in plans]), addr(a_list), addr(b_list), addr(out),
+                                      int(depth)))
+    return out
 
 
 class Batch:

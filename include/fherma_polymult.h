@@ -99,6 +99,13 @@ int pm_plan_get_ntt_form(const pm_plan* plan);   /* pm_ntt_form, or -1 for a nul
  * pm_multiply_device: device pointers, enqueued on `stream` (a hipStream_t;
  * null means the plan's own stream), returns without waiting. */
 int pm_multiply(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c);
+/* Products on host pointers (pm_multiply's path), up to `depth` at once, each
+ * through its own plan's pipeline: one product's inputs cross PCIe while
+ * another's kernels run and a third's answer comes back. Products of the same
+ * plan run one after another. depth <= 0: PM_HOST_DEPTH, else 3. Synchronous.
+ * No counterpart in the reference, which runs one product per fherma_run. */
+int pm_batch_multiply_host(int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
+                           uint32_t* const* c, int depth);
 int pm_multiply_device(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint32_t* c, void* stream);
 
 /* pm_multiply_device without any allocation: no counterpart in the reference.

@@ -174,3 +174,30 @@ def test_driver_autotune_switch_and_profile(golden, tmp_path):
     r = run_driver(tmp_path, case, {"FHERMA_AUTOTUNE_LOG": "1", "FHERMA_AUTOTUNE": "0"})
     assert r.returncode == 0 and "host mode in" not in r.stderr, r.stderr
     answers_ok(tmp_path, case)
+
+
+@pytest.mark.parametrize("depth", [1, 2, 3])
+def test_host_batch_matches_gmp(golden, depth):
+    """pm_batch_multiply_host: products of numpy operands, `depth` in flight,
+    mixed shapes and host modes, one plan used twice (its products serialize);
+    every answer against its GMP digest."""
+    names = ["config-n65536-w2048-00", "config-n65536-w2048-01", "config-n32768-w868-s1",
+             "config-n4096-w256-s1", "config-n65536-w1747-s1"]
+    cases = [golden[n] for n in names]
+    ops = [pointgen.operands(c) for c in cases]
+    plans = [polymult.Plan(c["N"], c["W"], q) for c, (q, _, _) in zip(cases, ops)]
+    plans[1].set_host_modes(polymult.HOST_MAPPED, polymult.HOST_STAGED)
+    plans[2].set_host_modes(polymult.HOST_DIRECT, polymult.HOST_MAPPED)
+    # the first plan twice: its second product squares case 0's a
+    order = [0, 1, 2, 3, 4, 0]
+    use_plans = [plans[i] for i in order]
+    A = [ops[i][1] for i in order[:5]] + [ops[0][1]]
+    B = [ops[i][2] for i in order[:5]] + [ops[0][1]]
+    say("host batch depth", depth)
+    out = polymult.batch_multiply_host(use_plans, A, B, depth=depth)
+    for k, i in enumerate(order[:5]):
+        assert oracle.digest(out[k]) == cases[i]["c_sha"], (depth, cases[i]["name"])
+    want = oracle.Plan(cases[0]["N"], ops[0][0]).multiply(ops[0][1], ops[0][1])
+    assert np.array_equal(out[5], want)
+    for p in plans:
+        p.close()
