@@ -265,6 +265,67 @@ def shape_info(graph, create_s):
             "tuned_replay_ms": round(graph.tuned_ms, 4), "create_s": round(create_s, 3)}
 
 
+def short_kernel(name):
+    """A rocprofv3 kernel name without namespaces, parameters and spaces."""
+    return name[name.find("k_"):].split("(")[0].replace(" ", "")
+
+
+def trace_stats(path, log):
+    """Steady-state durations per kernel from a rocprofv3 kernel trace (CSV):
+    the median and the 10 %-trimmed mean drop the first launches' warm-up."""
+    import csv
+    import statistics
+    runs = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if "k_" in r["Kernel_Name"]:
+                runs.setdefault(short_kernel(r["Kernel_Name"]), []).append(
+                    int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    out = {}
+    for k, v in runs.items():
+        v.sort()
+        cut = len(v) // 10
+        trim = v[cut:len(v) - cut]
+        out[k] = {"launches": len(v), "median_us": round(statistics.median(v) / 1e3, 2),
+                  "trimmed_mean_us": round(sum(trim) / len(trim) / 1e3, 2), "min_us": round(v[0] / 1e3, 2)}
+        log("%-40s %5d launches  median %8.2f us  trimmed mean %8.2f us  min %8.2f us"
+            % (k, len(v), out[k]["median_us"], out[k]["trimmed_mean_us"], out[k]["min_us"]))
+    return out
+
+
+def trace_shares(path, replays, log):
+    """The last of `replays` graph replays in a rocprofv3 kernel trace (CSV,
+    columns Kernel_Name, Start_Timestamp, End_Timestamp in ns): every instant
+    of its span is split evenly among the kernels running then. Returns per
+    kernel (template arguments dropped) its microseconds of the replay, its
+    share, and per product (16 per replay)."""
+    import csv
+    with open(path) as f:   # the library's kernels are named k_*; template arguments dropped
+        rows = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]), short_kernel(r["Kernel_Name"]).split("<")[0])
+                for r in csv.DictReader(f) if "k_" in r["Kernel_Name"]]   # runtime copy kernels left out
+    rows.sort()
+    per = len(rows) // replays
+    if per == 0 or len(rows) % replays:
+        raise SystemExit("%d dispatches do not split into %d equal replays" % (len(rows), replays))
+    last = rows[-per:]
+    edges = sorted({t for s, e, _ in last for t in (s, e)})
+    share = {}
+    idle = 0
+    for t0, t1 in zip(edges, edges[1:]):
+        run = [k for s, e, k in last if s <= t0 and e >= t1]
+        if not run:
+            idle += t1 - t0
+        for k in run:
+            share[k] = share.get(k, 0.0) + (t1 - t0) / len(run)
+    span = edges[-1] - edges[0]
+    out = {"dispatches_per_replay": per, "replay_us": round(span / 1e3, 2), "idle_us": round(idle / 1e3, 2),
+           "kernels": {k: {"us": round(v / 1e3, 2), "share": round(v / span, 4), "per_product_us": round(v / 1e3 / BATCH, 2)}
+                       for k, v in sorted(share.items(), key=lambda kv: -kv[1])}}
+    for k, v in out["kernels"].items():
+        log("%-34s %9.1f us  %5.1f %%  %6.2f us per product" % (k, v["us"], 100 * v["share"], v["per_product_us"]))
+    return out
+
+
 def uneven_rate(args, log):
     """The 64-product batch of uneven shapes (N in 4096 .. 65536, q of 128 .. 2019
     bits, seeds 4000 .. 4063 as in the golden fixtures), device-resident, replayed
@@ -441,8 +502,17 @@ def main():
                     help="only the per-kernel HIP-event profile (one product at a time on one stream), so that "
                          "a kernel trace of this run holds no concurrently running kernels")
     ap.add_argument("--copy-yardstick", action="store_true",
-                    help="first run three 256 MiB device copies (the known-size kernel that scales the "
-                         "HBM traffic counters)")
+                    help="first run three 256 MiB device copies and three 256 MiB read-only passes (the "
+                         "known-size kernels that scale the HBM traffic counters)")
+    ap.add_argument("--graph-only", action="store_true",
+                    help="only the 16-product batch graph: create it, replay it warmup + steps times, print "
+                         "its rate (for profiling the graph itself)")
+    ap.add_argument("--trace-stats", metavar="CSV",
+                    help="per-kernel duration statistics (count, median, 10 %%-trimmed mean, min) of a "
+                         "rocprofv3 kernel trace (CSV)")
+    ap.add_argument("--trace-shares", metavar="CSV",
+                    help="read a rocprofv3 kernel trace (CSV) of a --graph-only run and print how the wall time "
+                         "of its last replay divides among the kernels running at each instant")
     ap.add_argument("--issue-probe", action="store_true",
                     help="only the issue-rate probes (cycles per instruction kind on each SIMD), as JSON")
     args = ap.parse_args()
@@ -459,12 +529,22 @@ def main():
     if args.issue_probe:
         print(json.dumps(issue_probe(log)), flush=True)
         return
+    if args.trace_stats:
+        print(json.dumps(trace_stats(args.trace_stats, log)), flush=True)
+        return
+    if args.trace_shares:
+        print(json.dumps(trace_shares(args.trace_shares, args.steps + args.warmup + 1, log)), flush=True)
+        return
 
     if args.copy_yardstick:
+        # known-size yardsticks for the traffic counters: a 256 MiB copy (reads and
+        # writes 256 MiB) three times, then a read-only pass over 256 MiB three times
         src, dst = polymult.DeviceBuffer(256 << 20), polymult.DeviceBuffer(256 << 20)
         for _ in range(3):
             polymult.device_copy(dst, src)
         polymult.synchronize()
+        for _ in range(3):
+            polymult.device_sum(src)
         src.close()
         dst.close()
 
@@ -478,6 +558,13 @@ def main():
 
     gold = golden_digests()
     value = sec = plain_value = shape = None
+    if args.graph_only:
+        args.no_plain = True
+        value, sec, _, mismatches, shape = batch_rates(args, plans, da, db, dc, gold, log)
+        print(json.dumps({"graph_only": True, "value": round(value, 2), "ms_per_step": round(sec * 1e3, 4),
+                          "replays": args.warmup + args.steps + 1, "mismatches": int(mismatches),
+                          "shape": shape}), flush=True)
+        return
     if args.phases_only:
         for p, a, b, c in zip(plans, da, db, dc):
             p.multiply_device(a, b, c)
@@ -563,6 +650,12 @@ def main():
         "uneven_batch": uneven,
         "reference_points": ref_points,
         "roofline": roofline,
+        # every phase against the same roofline (the CRT and the decompositions take a larger
+        # share of the batch graph's wall time than the fused row pass, which is slowest alone)
+        "roofline_by_phase": {k: {"algorithmic_bytes": nbytes[k], "ms": round(v, 5),
+                                  "achieved_GBps": round(nbytes[k] / (v * 1e-3) / 1e9, 1),
+                                  "frac": round(nbytes[k] / (v * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+                              for k, v in phases.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -1121,6 +1121,24 @@ int pm_device_copy(void* dst, const void* src, size_t bytes, void* stream) {
     });
 }
 
+int pm_device_sum(const void* src, size_t bytes, uint64_t* sum) {
+    return guarded([&] {
+        need(src && sum, "null pointer");
+        need(bytes % 16 == 0 && reinterpret_cast<uintptr_t>(src) % 16 == 0,
+             "pm_device_sum needs a 16-byte aligned pointer and size");
+        uint64_t* d = static_cast<uint64_t*>(dev_alloc((pm::kSumBlocks + 1) * sizeof(uint64_t)));
+        try {
+            pm::launch_sum16(d, d + pm::kSumBlocks, src, bytes, nullptr);
+            check(hipGetLastError(), "kernel launch");
+            check(hipMemcpy(sum, d + pm::kSumBlocks, sizeof(uint64_t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        } catch (...) {
+            (void)hipFree(d);
+            throw;
+        }
+        check(hipFree(d), "hipFree");
+    });
+}
+
 int pm_stream_create(void** out) {
     return guarded([&] {
         need(out != nullptr, "out is null");

@@ -183,6 +183,10 @@ void launch_crt_mfma_batch(const DevPlan& P, const BatchArgs& ba, int count, hip
 
 // Plain grid-stride copy of `bytes` (a multiple of 16) with 16-byte accesses.
 void launch_copy16(void* dst, const void* src, size_t bytes, hipStream_t st);
+// Read-only pass over `bytes` (a multiple of 16): out[0] = the sum mod 2^64 of
+// its 32-bit words; partials holds kSumBlocks words of workspace.
+constexpr unsigned kSumBlocks = 8192;
+void launch_sum16(uint64_t* partials, uint64_t* out, const void* src, size_t bytes, hipStream_t st);
 
 void launch_crt_mfma(const DevPlan& P, const uint32_t* Y, uint32_t* c, hipStream_t st);
 // The same over coefficients [k0, k0 + count) only (chunked host pipeline).

@@ -280,6 +280,47 @@ __global__ void __launch_bounds__(256) k_copy16(uint4* __restrict__ dst, const u
     for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// Read-only pass: per workgroup, the sum mod 2^64 of the 32-bit words of its
+// grid-stride share of 16-byte units; a second one-workgroup launch adds the
+// partials. One 8-byte write per workgroup: a yardstick for the read counters.
+__global__ void __launch_bounds__(256) k_sum16(uint64_t* __restrict__ part, const uint4* __restrict__ src, size_t n) {
+    __shared__ uint64_t red[256];
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    uint64_t acc = 0;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint4 v = src[i];
+        acc += static_cast<uint64_t>(v.x) + v.y + v.z + v.w;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (unsigned h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_sum64(uint64_t* __restrict__ out, const uint64_t* __restrict__ part,
+                                               unsigned count) {
+    __shared__ uint64_t red[256];
+    uint64_t acc = 0;
+    for (unsigned i = threadIdx.x; i < count; i += blockDim.x) acc += part[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (unsigned h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+void launch_sum16(uint64_t* partials, uint64_t* out, const void* src, size_t bytes, hipStream_t st) {
+    const size_t n = bytes / 16;
+    const unsigned blocks = static_cast<unsigned>(std::max<size_t>(1, std::min<size_t>((n + 255) / 256, kSumBlocks)));
+    launch(k_sum16, dim3(blocks), dim3(256), 0, st, partials, static_cast<const uint4*>(src), n);
+    launch(k_sum64, dim3(1), dim3(256), 0, st, out, static_cast<const uint64_t*>(partials), blocks);
+}
+
 void launch_copy16(void* dst, const void* src, size_t bytes, hipStream_t st) {
     const size_t n = bytes / 16;
     if (n == 0) return;

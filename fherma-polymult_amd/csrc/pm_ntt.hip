@@ -417,28 +417,11 @@ __device__ __forceinline__ uint32_t pointwise_mul(uint32_t a, uint32_t b, const 
 // product with N^-1 (P/p_j)^-1 into A, inverse rows of A. kMode 1: input in
 // [0, 2p); a brought to [0, p) keeps a b below 2 p^2 < 2^32 p, which the
 // canonical Montgomery product needs.
-template <int LOGM, bool kGen, int kMode>
-__device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, const DevPlan& P,
-                                           uint32_t logG) {
-    extern __shared__ uint32_t lds[];
-    constexpr uint32_t M = 1u << LOGM, TPS = M / 16, GA = TPS, NG = 16 / GA, ROW = M + M / 16;
-    const uint32_t t = threadIdx.x & (TPS - 1), rl = threadIdx.x / TPS;
-    const uint32_t G = 1u << logG;
-    const uint32_t rho = (blockIdx.x << logG) + rl;
-    const int j = blockIdx.y;
-    const size_t row = static_cast<size_t>(j) * P.N + static_cast<size_t>(rho) * M;
-    const PrimeConst pc = P.pc[j];
-    const auto fw = row_tw<LOGM, kGen, true>(P, j, rho, pc);
-    const auto iw = row_tw<LOGM, kGen, false>(P, j, rho, pc);
-    // one exchange buffer per row, used by a and then b; one phase-A twiddle
-    // slot set per row, forward first, refilled with the inverse ones before the
-    // inverse phase A (19 KB of LDS per 16 rows: 8 workgroups per CU)
-    uint32_t* la = lds + rl * ROW;
-    uint2* twa = reinterpret_cast<uint2*>(lds + G * ROW) + rl * GA;
-    fill_row_phase_a<LOGM>(twa, fw, t);
-    const LdsTwA<LOGM, decltype(fw)::kMont> fa{twa};
-    const LdsTwA<LOGM, decltype(iw)::kMont> ia{twa};
-    uint32_t a[16], b[16];
+// The 16 values per thread of a row of A and of B (phase A layout) at `row`.
+template <int LOGM>
+__device__ __forceinline__ void load_row_pair(const uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm,
+                                              size_t row, uint32_t t, uint32_t (&a)[16], uint32_t (&b)[16]) {
+    constexpr uint32_t M = 1u << LOGM, TPS = M / 16, GA = TPS, NG = 16 / GA;
 #pragma unroll
     for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
@@ -446,6 +429,21 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
             a[i * GA + k] = A[row + t + TPS * i + 16 * k];
             b[i * GA + k] = Bm[row + t + TPS * i + 16 * k];
         }
+}
+
+// One row of a product once its values are in registers: forward rows of A and
+// B (sharing twiddle loads), the pointwise product into A, inverse row of A,
+// stored at `row`. la: the row's exchange buffer, twa: its phase-A twiddle slots.
+template <int LOGM, bool kGen, int kMode>
+__device__ __forceinline__ void row_item(uint32_t* __restrict__ A, const DevPlan& P, int j, uint32_t rho, size_t row,
+                                         uint32_t t, uint32_t* la, uint2* twa, uint32_t (&a)[16], uint32_t (&b)[16]) {
+    constexpr uint32_t M = 1u << LOGM, TPS = M / 16, GA = TPS, NG = 16 / GA;
+    const PrimeConst pc = P.pc[j];
+    const auto fw = row_tw<LOGM, kGen, true>(P, j, rho, pc);
+    const auto iw = row_tw<LOGM, kGen, false>(P, j, rho, pc);
+    fill_row_phase_a<LOGM>(twa, fw, t);
+    const LdsTwA<LOGM, decltype(fw)::kMont> fa{twa};
+    const LdsTwA<LOGM, decltype(iw)::kMont> ia{twa};
     wave_sync();
     phase_a<LOGM, true, true, kMode>(a, b, fa, pc.p);
 #pragma unroll
@@ -487,6 +485,30 @@ __device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint3
     for (uint32_t i = 0; i < NG; ++i)
 #pragma unroll
         for (uint32_t k = 0; k < GA; ++k) A[row + t + TPS * i + 16 * k] = a[i * GA + k];
+}
+
+// Row pass fused: forward rows of A and B, the pointwise product with
+// N^-1 (P/p_j)^-1 into A, inverse rows of A; G = 2^logG rows per workgroup.
+// kMode 1: input in [0, 2p); a brought to [0, p) keeps a b below 2 p^2 < 2^32 p,
+// which the canonical Montgomery product needs. One exchange buffer per row,
+// used by a and then b; one phase-A twiddle slot set per row, forward first,
+// refilled with the inverse ones before the inverse phase A (19 KB of LDS per
+// 16 rows).
+template <int LOGM, bool kGen, int kMode>
+__device__ __forceinline__ void rows_fused(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm, const DevPlan& P,
+                                           uint32_t logG) {
+    extern __shared__ uint32_t lds[];
+    constexpr uint32_t M = 1u << LOGM, TPS = M / 16, GA = TPS, ROW = M + M / 16;
+    const uint32_t t = threadIdx.x & (TPS - 1), rl = threadIdx.x / TPS;
+    const uint32_t G = 1u << logG;
+    const uint32_t rho = (blockIdx.x << logG) + rl;
+    const int j = blockIdx.y;
+    const size_t row = static_cast<size_t>(j) * P.N + static_cast<size_t>(rho) * M;
+    uint32_t* la = lds + rl * ROW;
+    uint2* twa = reinterpret_cast<uint2*>(lds + G * ROW) + rl * GA;
+    uint32_t a[16], b[16];
+    load_row_pair<LOGM>(A, Bm, row, t, a, b);
+    row_item<LOGM, kGen, kMode>(A, P, j, rho, row, t, la, twa, a, b);
 }
 
 template <int LOGM, bool kGen, int kMode>

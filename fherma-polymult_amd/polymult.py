@@ -95,6 +95,7 @@ def _load() -> ctypes.CDLL:
         "pm_copy_to_device": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
         "pm_copy_to_host": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
         "pm_device_copy": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t, _vp]),
+        "pm_device_sum": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64)]),
         "pm_stream_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
         "pm_stream_destroy": (ctypes.c_int, [_vp]),
         "pm_stream_synchronize": (ctypes.c_int, [_vp]),
@@ -221,6 +222,16 @@ def device_copy(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int = None, stream
     if n > dst.nbytes or n > src.nbytes:
         raise ValueError("copy larger than its buffers")
     _check(lib.pm_device_copy(_vp(dst.ptr), _vp(src.ptr), n, _sv(stream)))
+
+
+def device_sum(src: DeviceBuffer, nbytes: int = None) -> int:
+    """Sum mod 2^64 of the buffer's 32-bit words, by a read-only kernel (synchronous)."""
+    n = src.nbytes if nbytes is None else int(nbytes)
+    if n > src.nbytes:
+        raise ValueError("sum larger than its buffer")
+    out = ctypes.c_uint64()
+    _check(lib.pm_device_sum(_vp(src.ptr), n, ctypes.byref(out)))
+    return out.value
 
 
 def device_info():

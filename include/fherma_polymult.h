@@ -242,6 +242,11 @@ int pm_copy_to_host(void* dst, const void* src, size_t bytes);
  * the HBM traffic counters (FETCH_SIZE, WRITE_SIZE) of the product's kernels
  * are scaled. */
 int pm_device_copy(void* dst, const void* src, size_t bytes, void* stream);
+/* The sum mod 2^64 of the 32-bit words of a device buffer (16-byte aligned,
+ * bytes a multiple of 16), by a read-only streaming kernel; synchronous, on
+ * the null stream. No counterpart in the reference: the read-only yardstick
+ * for FETCH_SIZE, and a checksum of device results. */
+int pm_device_sum(const void* src, size_t bytes, uint64_t* sum);
 int pm_stream_create(void** out);
 int pm_stream_destroy(void* stream);
 int pm_stream_synchronize(void* stream);
