@@ -137,19 +137,30 @@ def measured_traffic(kernel):
     return None
 
 
-# Issue rate of wave64 VALU instructions per SIMD on gfx950 (integer add, xor,
-# v_mul_lo_u32 and v_mul_hi_u32 alike, measured with a probe kernel), and the
+# Cycles per wave64 instruction per SIMD on gfx950, from the issue-rate probe
+# (pm_probe_issue; profiles/issue_rates_gfx950.md): plain integer ops, 32-bit
+# multiplies (low and high half), 32 x 32 -> 64-bit multiply-adds; and the
 # MI355X peak engine clock.
-VALU_CYCLES_PER_INST = 4.4
+ISSUE_CYCLES = {"other": 2.92, "mul_lo": 5.04, "mul_hi": 4.64, "mad64": 5.9}
 CLOCK_GHZ = 2.4
+# VALU instructions per wave of the NTT kernels by class, counted in the gfx950
+# assembly of this build (straight-line kernels: the static count is the
+# per-wave count; it matches SQ_INSTS_VALU per wave).
+VALU_MIX = {
+    "k_ntt_rows_fused_reg<8,true,1>": {"other": 1584, "mul_hi": 448, "mul_lo": 321, "mad64": 1},
+    "k_ntt_col_reg<8,true,5,1>": {"other": 464, "mul_hi": 64, "mul_lo": 129},
+    "k_ntt_col_reg<8,false,5,1>": {"other": 464, "mul_hi": 128, "mul_lo": 65},
+}
 
 
 def valu_issue(kernel, kernel_ms, cus):
-    """The VALU issue time of one launch of a kernel from the committed SQ
-    counter table (profiles/sq_counters_n65536_w2048.txt: waves and VALU
-    instructions per wave), against its measured duration; None when absent."""
+    """The VALU issue time of one launch of a kernel: its instructions per wave
+    by class (VALU_MIX) at the probe's rates, times its waves per launch from the
+    committed SQ counter table (profiles/sq_counters_n65536_w2048.txt), over
+    the SIMDs; None when either is absent."""
     path = os.path.join(ROOT, "profiles", "sq_counters_n65536_w2048.txt")
-    if not os.path.exists(path):
+    mix = VALU_MIX.get(kernel)
+    if mix is None or not os.path.exists(path):
         return None
     with open(path) as f:
         for line in f:
@@ -157,11 +168,15 @@ def valu_issue(kernel, kernel_ms, cus):
             if len(parts) >= 3 and parts[0] == kernel:
                 waves, per_wave = int(parts[1]), float(parts[2])
                 simds = 4 * cus
-                ms = per_wave * waves * VALU_CYCLES_PER_INST / simds / (CLOCK_GHZ * 1e9) * 1e3
-                return {"instructions_per_wave": per_wave, "waves": waves, "cycles_per_instruction": VALU_CYCLES_PER_INST,
+                cycles = sum(n * ISSUE_CYCLES[k] for k, n in mix.items())
+                ms = cycles * waves / simds / (CLOCK_GHZ * 1e9) * 1e3
+                return {"instructions_per_wave": per_wave, "mix": mix, "cycles_per_wave": round(cycles, 1),
+                        "cycles_per_instruction": round(cycles / sum(mix.values()), 3), "waves": waves,
                         "clock_ghz": CLOCK_GHZ, "simds": simds, "issue_ms": round(ms, 5),
                         "frac": round(ms / kernel_ms, 4),
-                        "note": "integer VALU issue is this kernel's bound; frac = issue time / measured time"}
+                        "note": "issue time of the kernel's VALU instructions at the probe's rates over its "
+                                "measured time; the rest is dependency and LDS latency that eight waves per "
+                                "SIMD do not hide (fewer, persistent waves that prefetch were slower)"}
     return None
 
 

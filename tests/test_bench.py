@@ -52,7 +52,8 @@ def test_kernel_names_match_the_committed_profiles():
     assert names["ntt_rows_fused"] == "k_ntt_rows_fused_reg<8,true,1>"
     for phase, kernel in names.items():
         assert bench.measured_traffic(kernel) is not None, (phase, kernel)
-        assert bench.valu_issue(kernel, 0.05, 256) is not None, (phase, kernel)
+        if phase.startswith("ntt"):
+            assert bench.valu_issue(kernel, 0.05, 256) is not None, (phase, kernel)
 
 
 def test_roofline_helpers_read_the_profiles():
@@ -62,7 +63,13 @@ def test_roofline_helpers_read_the_profiles():
     assert alg <= traffic <= 1.2 * alg
     v = bench.valu_issue(rows, 0.044, 256)
     assert v["waves"] == 8576 and 2000 < v["instructions_per_wave"] < 3000
-    expect = v["instructions_per_wave"] * v["waves"] * 4.4 / 1024 / 2.4e9 * 1e3
+    # the assembly's class counts add up to the counter's VALU instructions per wave
+    for kernel, mix in bench.VALU_MIX.items():
+        w = bench.valu_issue(kernel, 0.02, 256)
+        assert sum(mix.values()) == round(w["instructions_per_wave"]), kernel
+    # 1584 plain ops at 2.92 cycles, 448 high and 321 low multiplies at 4.64 and 5.04, one 64-bit multiply-add
+    cycles = 1584 * 2.92 + 448 * 4.64 + 321 * 5.04 + 5.9
+    expect = cycles * 8576 / 1024 / 2.4e9 * 1e3
     assert abs(v["issue_ms"] - expect) < 1e-4 and abs(v["frac"] - expect / 0.044) < 1e-3
 
 
