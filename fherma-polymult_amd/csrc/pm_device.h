@@ -33,10 +33,10 @@ struct DevPlan {
     // The b side of a product: tables that decompose b into b scale_j mod p_j
     // (scale_j the pointwise constant of PrimeConst), so that the fused row
     // pass needs only a Montgomery product. pow2b: K x L; the MFMA tables
-    // dec_digits_b / dec_corr_b are laid out as dec_digits / dec_corr.
+    // dec_digits_b / dec_init_b are laid out as dec_digits / dec_init.
     const uint32_t* pow2b;
     const int8_t* dec_digits_b;
-    const uint64_t* dec_corr_b;
+    const int2* dec_init_b;
     // Column pass on the matrix cores (pm_ntt_mfma.hip), R = 256: per prime and
     // direction, 17 matrices as MFMA fragments (87040 bytes); null when absent.
     const uint8_t* colm_fwd;
@@ -65,10 +65,11 @@ struct DevPlan {
     const double* nq;         // K + 1: negvP_v / q
     // int8 matrix-core forms (pm_mfma.hip); null until prepared.
     const int8_t* dec_digits; // [ceil(K/4)][dec_ksteps][64 lanes][16]: MFMA A fragments
-    // K: 128 sum_t T_{j,t} mod p_j (T the table entries, 256^t 2^32 mod p_j
-    // times the side's factor) plus the least multiple of p_j that is >= 2^47,
-    // so that every decompose sum is positive before its reduction
-    const uint64_t* dec_corr;
+    // K: the initial values of prime j's digit-0 and digit-2 accumulators
+    // (x, y): with them the folded sum R = G_0 + 256 G_1 + 2^16 (G_2 + 256 G_3)
+    // is positive, below 2^48, and congruent to the exact sum x T (T the table
+    // entries 256^t 2^32 mod p_j times the side's factor; pm_mfma.hip)
+    const int2* dec_init;
     // CRT digit table, column-major: word J kTabRows + R (pm_mfma.hip) holds
     // byte R - 3 of the balanced digits of M_j for primes j = 4J .. 4J + 3
     // (zero for j >= K and outside digit rows 0 .. 4L).
@@ -90,7 +91,7 @@ inline DevPlan b_side(const DevPlan& P) {
     DevPlan Q = P;
     Q.pow2 = P.pow2b;
     Q.dec_digits = P.dec_digits_b;
-    Q.dec_corr = P.dec_corr_b;
+    Q.dec_init = P.dec_init_b;
     return Q;
 }
 
@@ -133,9 +134,9 @@ struct ProductDesc {
     int dec_ksteps, crt_ksteps;
     uint32_t dec_magic, pad;
     const int8_t* dec_digits;
-    const uint64_t* dec_corr;
+    const int2* dec_init;
     const int8_t* dec_digits_b;
-    const uint64_t* dec_corr_b;
+    const int2* dec_init_b;
     const uint32_t* crt_tab;
     const double* crt_fc;
     const double* nq;

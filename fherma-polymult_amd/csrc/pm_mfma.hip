@@ -66,19 +66,6 @@ __device__ __forceinline__ v4i mfma16(const v4i& a, const v4i& b, const v4i& c) 
     return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
 }
 
-// R 2^-32 mod p for 0 <= R < 2^49, by one Montgomery step (pneg = -p^-1 mod
-// 2^32): m = R pneg mod 2^32 makes R + m p a multiple of 2^32; the low words
-// of R and m p then add to 0 if R's low word is 0 and to 2^32 otherwise, so
-// the carry into the high word is (low word != 0). (R + m p) / 2^32 < 2^17 + p
-// lies in [0, 2p), below 2^32; kLazy false brings it to [0, p) with one
-// unsigned min.
-template <bool kLazy = false>
-__device__ __forceinline__ uint32_t mont_fold(uint64_t R, uint32_t p, uint32_t pneg) {
-    const uint32_t rl = static_cast<uint32_t>(R), rh = static_cast<uint32_t>(R >> 32);
-    const uint32_t t = rh + __umulhi(rl * pneg, p) + (rl != 0u ? 1u : 0u);
-    return kLazy ? t : min(t, t - p);
-}
-
 // Decompose, one workgroup per 64 coefficients (4 tiles of 16). Their limbs are
 // copied to LDS with coalesced loads (rows of 16 KT + 4 words, zero beyond L,
 // stored as int8 digits u ^ 0x80), every wave takes its B fragments from there,
@@ -90,12 +77,12 @@ __device__ __forceinline__ void decompose_tile(const uint32_t* __restrict__ limb
     extern __shared__ uint32_t xs[];
     constexpr int Ls = 16 * KT + 4;
     const int L = P.L;
-    // per-prime constants for the epilogue: corr, p, -p^-1 mod 2^32
-    uint64_t* kcorr = reinterpret_cast<uint64_t*>(xs + 16 * kTiles * Ls);
-    uint32_t* kp = reinterpret_cast<uint32_t*>(kcorr + P.K);
+    // per-prime constants: the accumulators' initial values, p, -p^-1 mod 2^32
+    int2* kinit = reinterpret_cast<int2*>(xs + 16 * kTiles * Ls);
+    uint32_t* kp = reinterpret_cast<uint32_t*>(kinit + P.K);
     uint32_t* kpneg = kp + P.K;
     for (int j = threadIdx.x; j < P.K; j += blockDim.x) {
-        kcorr[j] = P.dec_corr[j];
+        kinit[j] = P.dec_init[j];
         kp[j] = P.pc[j].p;
         kpneg[j] = P.pc[j].pneg;
     }
@@ -178,6 +165,10 @@ __device__ __forceinline__ void decompose_tile(const uint32_t* __restrict__ limb
     v4i an[KT];
 #pragma unroll
     for (int ks = 0; ks < KT; ++ks) an[ks] = dig[(min(wave, groups - 1) * KT + ks) * 64 + lane];
+    // 2^16, opaque to the compiler: R below is one v_mad_u64_u32 instead of
+    // sign extensions and 64-bit shifts
+    uint32_t k16 = 1u << 16;
+    asm volatile("" : "+s"(k16));
     for (int jg = wave; jg < groups; jg += kWaves) {
         v4i a[KT];
 #pragma unroll
@@ -185,28 +176,33 @@ __device__ __forceinline__ void decompose_tile(const uint32_t* __restrict__ limb
         const int jn = min(jg + kWaves, groups - 1);
 #pragma unroll
         for (int ks = 0; ks < KT; ++ks) an[ks] = dig[(jn * KT + ks) * 64 + lane];
+        // Lane (g, cl) holds rows 4g .. 4g + 3 = prime 4 jg + g, digits b = 0..3;
+        // digits 0 and 2 start from the prime's offsets (lanes past K: any
+        // prime's, their sums are dropped)
+        const int j = 4 * jg + g;
+        const int2 init = kinit[min(j, P.K - 1)];
         v4i acc[kTiles];
 #pragma unroll
-        for (int t = 0; t < kTiles; ++t) acc[t] = v4i{0, 0, 0, 0};
+        for (int t = 0; t < kTiles; ++t) acc[t] = v4i{init.x, 0, init.y, 0};
 #pragma unroll
         for (int ks = 0; ks < KT; ++ks)
 #pragma unroll
             for (int t = 0; t < kTiles; ++t) acc[t] = mfma16(a[ks], bf[t][ks], acc[t]);
-        // Lane (g, cl) holds rows 4g .. 4g + 3 = prime 4 jg + g, digits b = 0..3.
-        const int j = 4 * jg + g;
         if (j < P.K) {
             const uint32_t p = kp[j], pneg = kpneg[j];
-            const uint64_t corr = kcorr[j];
             uint32_t* out = res + static_cast<size_t>(j) * P.N + nb;
 #pragma unroll
             for (int t = 0; t < kTiles; ++t) {
-                // |G_b| <= 2^22, so G_0 + 256 G_1 and G_2 + 256 G_3 fit in int32, and
-                // R = sum_b 256^b G_b + corr lies in [0, 2^49)
+                // lo = G_0 + 256 G_1 fits int32 (|lo| < 2^30 + 2^23); hi, with the
+                // offset, is in [2^23, 2^31 + 2^25) as an unsigned word; R = 2^16 hi + lo
+                // lies in (0, 2^48) and is congruent to the exact sum. One Montgomery
+                // step: t = (R + m p) / 2^32 < p + 2^16, below 2p.
                 const int lo = acc[t][0] + acc[t][1] * 256;
-                const int hi = acc[t][2] + acc[t][3] * 256;
-                const uint64_t R = static_cast<uint64_t>(static_cast<long long>(lo) + static_cast<long long>(hi) * 65536) + corr;
-                const uint32_t v = mont_fold<kLazyOut>(R, p, pneg);
-                if (static_cast<uint32_t>(16 * t + cl) < nk) out[16 * t + cl] = v;
+                const uint32_t hi = static_cast<uint32_t>(acc[t][2]) + static_cast<uint32_t>(acc[t][3]) * 256u;
+                const uint64_t R = static_cast<uint64_t>(hi) * k16 + static_cast<uint64_t>(static_cast<int64_t>(lo));
+                const uint32_t m = static_cast<uint32_t>(R) * pneg;
+                const uint32_t v = static_cast<uint32_t>((static_cast<uint64_t>(m) * p + R) >> 32);
+                if (static_cast<uint32_t>(16 * t + cl) < nk) out[16 * t + cl] = kLazyOut ? v : min(v, v - p);
             }
         }
     }
@@ -231,7 +227,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_decompose_mfma_batch(DevPlan P,
     Q.L = d->L;
     Q.dec_magic = d->dec_magic;
     Q.dec_digits = second ? d->dec_digits_b : d->dec_digits;
-    Q.dec_corr = second ? d->dec_corr_b : d->dec_corr;
+    Q.dec_init = second ? d->dec_init_b : d->dec_init;
     decompose_tile<KT, kLazyOut>(second ? ba.b[i] : ba.a[i], second ? ba.B[i] : ba.A[i], Q, 0, P.N);
 }
 
@@ -696,9 +692,9 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
     // is what the kernel's Montgomery step takes away.
     const int KT = (4 * L + 63) / 64;
     const int KG = (K + 3) / 4;
-    auto dec_tables = [&](bool scaled, const int8_t** digits, const uint64_t** corr) {
+    auto dec_tables = [&](bool scaled, const int8_t** digits, const int2** init) {
         std::vector<int8_t> dec(static_cast<size_t>(KG) * KT * 64 * 16, 0);
-        std::vector<uint64_t> dcorr(K);
+        std::vector<int2> dinit(K);
         for (int j = 0; j < K; ++j) {
             const uint64_t p = h.p[j];
             uint64_t T = (scaled ? h.scale[j] % p : 1) * ((uint64_t(1) << 32) % p) % p, sum = 0;
@@ -716,16 +712,24 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
                 }
                 T = T * 256 % p;
             }
-            // |sum_b 256^b G_b| < 2^47: adding ceil(2^47 / p) p keeps it positive and
-            // below 2^49, without changing the residue
-            const uint64_t lift = ((uint64_t(1) << 47) + p - 1) / p * p;
-            dcorr[j] = sum * 128 % p + lift;
+            // The bytes enter as u - 128, so the sums miss 128 sum_t T; with the
+            // offset 2^16 c (c = 2^30 + 2^24) that keeps the folded sum positive,
+            // the accumulators must add d = 128 sum_t T - 2^16 c (mod p), taken
+            // centred (|d| < p/2 < 2^30) and split as d = d_lo + 2^16 d_hi with
+            // d_lo in [0, 2^16): digit 0 starts at d_lo, digit 2 at d_hi + c.
+            const long long c = (1ll << 30) + (1ll << 24);
+            long long dd = static_cast<long long>((sum * 128 % p + p - (static_cast<uint64_t>(c) << 16) % p) % p);
+            if (dd > static_cast<long long>(p / 2)) dd -= static_cast<long long>(p);
+            const long long d_hi = dd >= 0 ? dd >> 16 : -((-dd + 65535) >> 16);
+            const long long d_lo = dd - (d_hi << 16);
+            if (d_lo < 0 || d_lo >= 65536) throw std::logic_error("decompose offsets");
+            dinit[j] = make_int2(static_cast<int>(d_lo), static_cast<int>(d_hi + c));
         }
         *digits = static_cast<const int8_t*>(upload(dec.data(), dec.size()));
-        *corr = static_cast<const uint64_t*>(upload(dcorr.data(), dcorr.size() * sizeof(uint64_t)));
+        *init = static_cast<const int2*>(upload(dinit.data(), dinit.size() * sizeof(int2)));
     };
-    dec_tables(false, &d.dec_digits, &d.dec_corr);
-    dec_tables(true, &d.dec_digits_b, &d.dec_corr_b);
+    dec_tables(false, &d.dec_digits, &d.dec_init);
+    dec_tables(true, &d.dec_digits_b, &d.dec_init_b);
     d.dec_ksteps = KT;
     // used only for L >= 8, so L / 4 >= 2 and the magic fits 32 bits
     d.dec_magic = static_cast<uint32_t>((uint64_t(1) << 32) / static_cast<uint64_t>(std::max(L / 4, 2)) + 1);

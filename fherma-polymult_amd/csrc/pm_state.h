@@ -98,9 +98,9 @@ struct pm_plan {
         d.crt_ksteps = dev.crt_ksteps;
         d.dec_magic = dev.dec_magic;
         d.dec_digits = dev.dec_digits;
-        d.dec_corr = dev.dec_corr;
+        d.dec_init = dev.dec_init;
         d.dec_digits_b = dev.dec_digits_b;
-        d.dec_corr_b = dev.dec_corr_b;
+        d.dec_init_b = dev.dec_init_b;
         d.crt_tab = dev.crt_tab;
         d.crt_fc = dev.crt_fc;
         d.nq = dev.nq;
