@@ -83,7 +83,7 @@ struct DevPlan {
     int dec_ksteps;           // ceil(4L / 64)
     int crt_ksteps;           // ceil(4K / 64)
     int crt_stiles;           // ceil((4L + 4) / 16)
-    uint32_t dec_magic;       // floor(2^32 / (L / 4)) + 1: i / (L / 4) = mulhi(i, dec_magic) for i < 2^12
+    uint32_t dec_magic;       // floor(2^32 / D) + 1, D = L / 4 if 4 | L and L >= 8, else L: i / D = mulhi(i, dec_magic) for i < 2^12
 };
 
 // P with the b-side decompose tables in place of the plain ones.

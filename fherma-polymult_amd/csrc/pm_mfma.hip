@@ -66,6 +66,15 @@ __device__ __forceinline__ v4i mfma16(const v4i& a, const v4i& b, const v4i& c) 
     return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
 }
 
+// The digits of the words past L in each of a tile's 64 LDS rows: byte 0,
+// stored as 0x80 (the bias of u - 128).
+template <int KT>
+__device__ __forceinline__ void pad_rows(uint32_t* xs, int L) {
+    constexpr int Ls = 16 * KT + 4;
+    for (uint32_t r = threadIdx.x; r < 16u * kTiles; r += blockDim.x)
+        for (int w = L; w < Ls; ++w) xs[r * Ls + w] = 0x80808080u;
+}
+
 // Decompose, one workgroup per 64 coefficients (4 tiles of 16). Their limbs are
 // copied to LDS with coalesced loads (rows of 16 KT + 4 words, zero beyond L,
 // stored as int8 digits u ^ 0x80), every wave takes its B fragments from there,
@@ -108,12 +117,7 @@ __device__ __forceinline__ void decompose_tile(const uint32_t* __restrict__ limb
                     make_uint4(v[r].x ^ 0x80808080u, v[r].y ^ 0x80808080u, v[r].z ^ 0x80808080u, v[r].w ^ 0x80808080u);
             }
         }
-        // digits of the padding words past L: byte 0
-        const uint32_t pad = Ls - L;
-        for (uint32_t i = threadIdx.x; i < 16u * kTiles * pad; i += blockDim.x) {
-            const uint32_t row = i / pad;
-            xs[row * Ls + L + (i - row * pad)] = 0x80808080u;
-        }
+        pad_rows<KT>(xs, L);
     } else if (nk == 16u * kTiles) {
         // A full tile, L not a multiple of 4: at most KT 16-byte loads per thread.
         const uint32_t words = 16u * kTiles * static_cast<uint32_t>(L);
@@ -131,17 +135,13 @@ __device__ __forceinline__ void decompose_tile(const uint32_t* __restrict__ limb
                 const uint32_t w4[4] = {v[r].x, v[r].y, v[r].z, v[r].w};
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const uint32_t i = 4 * i4 + e, row = i / L;
+                    // i / L by the plan's magic multiplier (exact for i < 2^12)
+                    const uint32_t i = 4 * i4 + e, row = L == 1 ? i : mulhi32(i, P.dec_magic);
                     xs[row * Ls + (i - row * L)] = w4[e] ^ 0x80808080u;
                 }
             }
         }
-        // digits of the padding words past L: byte 0
-        const uint32_t pad = Ls - L;
-        for (uint32_t i = threadIdx.x; i < 16u * kTiles * pad; i += blockDim.x) {
-            const uint32_t row = i / pad;
-            xs[row * Ls + L + (i - row * pad)] = 0x80808080u;
-        }
+        pad_rows<KT>(xs, L);
     } else {
         for (uint32_t i = threadIdx.x; i < kSlots; i += blockDim.x) {
             const uint32_t r = i / Ls, w = i - r * Ls;
@@ -731,8 +731,12 @@ void prepare_mfma(const HostPlan& h, DevPlan& d, const std::function<void*(const
     dec_tables(false, &d.dec_digits, &d.dec_init);
     dec_tables(true, &d.dec_digits_b, &d.dec_init_b);
     d.dec_ksteps = KT;
-    // used only for L >= 8, so L / 4 >= 2 and the magic fits 32 bits
-    d.dec_magic = static_cast<uint32_t>((uint64_t(1) << 32) / static_cast<uint64_t>(std::max(L / 4, 2)) + 1);
+    // i / D = mulhi(i, floor(2^32 / D) + 1) for i < 2^12 and 2 <= D <= 64
+    // (every such i and D checked exhaustively): D = L / 4 where the kernel divides
+    // 16-byte unit indices (L a multiple of 4, at least 8), else D = L (word
+    // indices; L = 1 divides by nothing)
+    const uint64_t D = (L % 4 == 0 && L >= 8) ? static_cast<uint64_t>(L / 4) : static_cast<uint64_t>(std::max(L, 2));
+    d.dec_magic = static_cast<uint32_t>((uint64_t(1) << 32) / D + 1);
 
     // CRT tables: the prime groups, then the quotient group (column KJ).
     const int KS = ((K + 3) / 4 + 1 + 3) / 4;

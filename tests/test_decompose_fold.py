@@ -127,3 +127,14 @@ def test_decompose_epilogue(p, L, side):
         assert t[k] % p == x * f % p, (p, L, side, k)
         assert t[k] < 2 * p
         assert min(t[k], (t[k] - p) % (1 << 32)) == x * f % p   # the canonical output
+
+
+def test_division_magic():
+    """The decompose kernel's LDS fill divides an index i < 2^12 by D (L / 4 or L,
+    2 <= D <= 64) as mulhi(i, floor(2^32 / D) + 1); exact for every such i and D
+    (and a magic one too small would fail)."""
+    i = np.arange(1 << 12, dtype=np.uint64)
+    for D in range(2, 65):
+        magic = np.uint64((1 << 32) // D + 1)
+        assert np.array_equal((i * magic) >> np.uint64(32), i // np.uint64(D)), D
+    assert not np.array_equal((i * np.uint64((1 << 32) // 3)) >> np.uint64(32), i // np.uint64(3))
