@@ -77,3 +77,39 @@ def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def _trace(path, rows):
+    with open(path, "w") as f:
+        f.write("Kernel_Name,Start_Timestamp,End_Timestamp\n")
+        for name, s, e in rows:
+            f.write('"%s",%d,%d\n' % (name, s, e))
+
+
+def test_trace_shares_splits_overlap_evenly(tmp_path):
+    """Three replays of two overlapping kernels and a runtime copy: only the last
+    replay counts, the copy is left out, and time run by both is halved."""
+    rows = []
+    for r in range(3):
+        t = 10000 * r
+        rows += [("void pm::(anonymous namespace)::k_a<8, true>(pm::DevPlan)", t, t + 1000),
+                 ("void pm::(anonymous namespace)::k_b<9>(pm::DevPlan)", t + 500, t + 2000),
+                 ("__amd_rocclr_copyBuffer", t + 3000, t + 4000)]
+    path = tmp_path / "trace.csv"
+    _trace(path, rows)
+    out = bench.trace_shares(str(path), 3, lambda *a: None)
+    assert out["dispatches_per_replay"] == 2 and out["idle_us"] == 0
+    assert out["replay_us"] == 2.0
+    assert out["kernels"]["k_a"]["us"] == 0.75 and out["kernels"]["k_b"]["us"] == 1.25
+    assert abs(sum(v["share"] for v in out["kernels"].values()) - 1) < 1e-9
+    with pytest.raises(SystemExit):
+        bench.trace_shares(str(path), 4, lambda *a: None)
+
+
+def test_trace_stats_median_and_trim(tmp_path):
+    rows = [("void pm::k_c<1>(int)", 0, d) for d in [1000] * 18 + [9000, 100]]
+    path = tmp_path / "trace.csv"
+    _trace(path, rows)
+    out = bench.trace_stats(str(path), lambda *a: None)["k_c<1>"]
+    assert out["launches"] == 20 and out["median_us"] == 1.0 and out["trimmed_mean_us"] == 1.0
+    assert out["min_us"] == 0.1

@@ -413,3 +413,20 @@ def test_ntt_form_default_and_environment(golden, monkeypatch):
     with pytest.raises(polymult.PolymulError) as e:
         polymult.Plan(case["N"], case["W"], q)
     assert e.value.code == polymult.PM_EINVAL
+
+
+def test_device_sum_and_copy_yardsticks():
+    """The traffic yardsticks compute what they claim: pm_device_sum is the sum
+    mod 2^64 of the 32-bit words (sizes from one 16-byte unit to past one
+    workgroup's grid-stride share), pm_device_copy a plain copy."""
+    rng = np.random.default_rng(11)
+    for words in (4, 1024, 4 * 256 * 8192 + 4, 3 << 20):
+        x = rng.integers(0, 1 << 32, words, dtype=np.uint64).astype(np.uint32)
+        d = polymult.DeviceBuffer.from_array(x)
+        assert polymult.device_sum(d) == int(x.astype(np.uint64).sum()) % (1 << 64), words
+        c = polymult.DeviceBuffer(x.nbytes)
+        polymult.device_copy(c, d)
+        polymult.synchronize()
+        assert np.array_equal(c.download((words,)), x), words
+        d.close()
+        c.close()
