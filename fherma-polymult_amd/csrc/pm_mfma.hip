@@ -18,10 +18,11 @@
 //     G_b[j][n] = sum_t e_{j,t,b} (u_{n,t} - 128)        |G_b| <= 4L 2^14 <= 2^22
 //     x_n 2^32 mod p_j = (sum_b 256^b G_b + 128 sum_t T_{j,t}) mod p_j
 //   MFMA rows are (prime, digit) pairs, columns coefficients, K-dim bytes t. The
-//   four G_b of one (j, n) land in one lane; they are combined into one exact
-//   64-bit integer (made positive by a multiple of p_j) and reduced by one
-//   Montgomery step, which also takes the factor 2^32 away: integer arithmetic
-//   only (FP64 appears in the CRT's quotient rounding alone).
+//   four G_b of one (j, n) land in one lane; the bias term and a positive offset
+//   enter as the accumulators' initial values, the four sums are combined into
+//   one exact positive 64-bit integer and reduced by one Montgomery step, which
+//   also takes the factor 2^32 away: integer arithmetic only (FP64 appears in
+//   the CRT's quotient rounding alone).
 //
 // CRT (reference rns_kernels.cuh:265-364: x = sum_j y_j M_j + negvP_v mod q):
 //   y_{j,n} = sum_a u_{j,n,a} 256^a (4 bytes), M_j = sum_s m_{j,s} 256^s (4L + 1
