@@ -135,3 +135,26 @@ def test_invalid_twiddle_policy_is_refused_before_device_work():
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
     assert out.stdout.startswith("%d " % polymult.PM_EINVAL), out.stdout + out.stderr
     assert "FHERMA_TWIDDLES" in out.stdout
+
+
+def test_new_entry_points_refuse_bad_arguments_on_the_host():
+    """The batch, host-batch, shape, sum and form entry points check their
+    arguments before any device work (no GPU is needed for these)."""
+    import ctypes
+    lib = polymult.lib
+    vp = ctypes.c_void_p
+    null_arr = ctypes.POINTER(vp)()
+    assert lib.pm_batch_multiply_host(0, null_arr, null_arr, null_arr, null_arr, 0) == polymult.PM_OK
+    assert lib.pm_batch_multiply_host(2, null_arr, null_arr, null_arr, null_arr, 0) == polymult.PM_EINVAL
+    two = (vp * 2)(None, None)
+    assert lib.pm_batch_multiply_host(2, two, two, two, two, 1) == polymult.PM_EINVAL
+    assert "null" in lib.pm_last_error().decode()
+    out = vp()
+    shape = polymult.BatchShape(1, 1, -1, -1)
+    assert lib.pm_batch_create_shaped(0, two, two, two, two, ctypes.byref(shape), ctypes.byref(out)) == polymult.PM_EINVAL
+    assert lib.pm_batch_create(1, two, two, two, two, None) == polymult.PM_EINVAL
+    assert lib.pm_batch_get_shape(None, ctypes.byref(shape), None) == polymult.PM_EINVAL
+    s = ctypes.c_uint64()
+    assert lib.pm_device_sum(None, 16, ctypes.byref(s)) == polymult.PM_EINVAL
+    assert lib.pm_plan_set_ntt_form(None, 0) == polymult.PM_EINVAL
+    assert lib.pm_plan_get_ntt_form(None) == -1
