@@ -179,3 +179,45 @@ def test_full_twiddle_tables_on_demand(golden):
             plan.set_twiddles(source)
             say("twiddles", source)
             assert oracle.digest(plan.multiply(a, b)) == case["c_sha"], source
+
+
+def test_more_than_32_uneven_products_of_one_n_match_single_products():
+    """40 products at N = 4096 with q widths from 100 to 2048 bits (K from 9 to
+    127, L from 4 to 64): one run is cut into launches of 32 and 8, each with
+    its own plan of most primes and its own K-step buckets; the results equal
+    each product computed alone, through the graph and on a stream."""
+    rng = np.random.default_rng(4096)
+    N = 4096
+    widths = [int(w) for w in rng.integers(100, 2049, 40)]
+    pts = [oracle.gen_point(9000 + i, N, w) for i, w in enumerate(widths)]
+    plans = [polymult.Plan(N, w, q) for (q, _, _), w in zip(pts, widths)]
+    assert len({p.K for p in plans}) > 10 and len({p.L for p in plans}) > 10
+    da = [polymult.DeviceBuffer.from_array(a) for _, a, _ in pts]
+    db = [polymult.DeviceBuffer.from_array(b) for _, _, b in pts]
+    ref = [polymult.DeviceBuffer(a.nbytes) for _, a, _ in pts]
+    out = [polymult.DeviceBuffer(a.nbytes) for _, a, _ in pts]
+    for p, a, b, c in zip(plans, da, db, ref):
+        p.multiply_device(a, b, c)
+    polymult.synchronize()
+    want = [c.download((N, p.L)) for c, p in zip(ref, plans)]
+    # the single products themselves against the oracle, for three of them
+    for i in (0, 17, 39):
+        q, a, b = pts[i]
+        assert np.array_equal(want[i], oracle.Plan(N, q).multiply(a, b)), i
+    say("40 uneven products at one N: one run")
+    batch = polymult.Batch(plans, da, db, out, shape=(1, 1, -1, -1))
+    assert batch.phased
+    batch.launch()
+    polymult.synchronize()
+    for i, (c, p) in enumerate(zip(out, plans)):
+        assert np.array_equal(c.download((N, p.L)), want[i]), ("graph", i)
+    batch.close()
+    stream = polymult.Stream()
+    for c in out:
+        c.upload(np.zeros(c.nbytes // 4, dtype=np.uint32))
+    polymult.batch_multiply_device(plans, da, db, out, stream=stream)
+    stream.synchronize()
+    for i, (c, p) in enumerate(zip(out, plans)):
+        assert np.array_equal(c.download((N, p.L)), want[i]), ("stream", i)
+    for x in da + db + ref + out + plans:
+        x.close()
