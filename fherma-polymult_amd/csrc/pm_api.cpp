@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -775,6 +776,67 @@ void tune_batch(pm_batch& bt, int count, pm_plan* const* plans, const uint32_t* 
     cleanup();
 }
 
+// The timed choice of a plan's row-pass twiddle source and column-pass form:
+// device-resident products on dummy operands (zeros), two warm-up runs and the
+// median of five per candidate; the plan's own setting (the reference's size
+// rule for the twiddles, the butterflies for the columns) is kept unless
+// another candidate is at least 5 % faster, the reference's rule for its host
+// modes (solve.cu:817-862). FHERMA_TWIDDLES / PM_NTT_FORM in the environment
+// pin their dimension. Returns the chosen setting's median ms per product.
+double tune_kernels(pm_plan* pl) {
+    const size_t words = pl->words();
+    uint32_t* buf = static_cast<uint32_t*>(dev_alloc(3 * words * sizeof(uint32_t)));
+    double best = 0;
+    try {
+        check(hipMemset(buf, 0, 3 * words * sizeof(uint32_t)), "hipMemset");
+        const uint32_t *a = buf, *b = buf + words;
+        uint32_t* c = buf + 2 * words;
+        const bool log = std::getenv("FHERMA_AUTOTUNE_LOG") || std::getenv("PM_HOST_TUNE_LOG");
+        auto time_it = [&] {
+            std::vector<double> t;
+            for (int r = 0; r < 7; ++r) {
+                check(hipEventRecord(pl->ev[0], pl->stream), "hipEventRecord");
+                run_device(pl, a, b, c, pl->stream);
+                check(hipEventRecord(pl->ev[1], pl->stream), "hipEventRecord");
+                check(hipEventSynchronize(pl->ev[1]), "hipEventSynchronize");
+                float ms = 0;
+                check(hipEventElapsedTime(&ms, pl->ev[0], pl->ev[1]), "hipEventElapsedTime");
+                if (r >= 2) t.push_back(ms);
+            }
+            std::sort(t.begin(), t.end());
+            if (log)
+                std::fprintf(stderr, "device twiddles %s, column pass %s: %.3f ms (median of 5)\n",
+                             pl->dev.row_gen ? "gen" : "table", pl->dev.col_mfma ? "mfma" : "valu", t[2]);
+            return t[2];
+        };
+        best = time_it();
+        if (!std::getenv("FHERMA_TWIDDLES") && pm::batch_kernels_ok(pl->dev)) {
+            const int own = pl->dev.row_gen ? PM_TWIDDLES_GEN : PM_TWIDDLES_TABLE;
+            const int other = own == PM_TWIDDLES_GEN ? PM_TWIDDLES_TABLE : PM_TWIDDLES_GEN;
+            if (pm_plan_set_twiddles(pl, other) != PM_OK) throw std::runtime_error(g_err);
+            const double ms = time_it();
+            if (ms < 0.95 * best)
+                best = ms;
+            else if (pm_plan_set_twiddles(pl, own) != PM_OK)
+                throw std::runtime_error(g_err);
+        }
+        if (!std::getenv("PM_NTT_FORM") && pl->dev.colm_fwd && pl->dev.colm_inv) {
+            const int own = pl->dev.col_mfma;
+            pl->dev.col_mfma = own ? 0 : 1;
+            const double ms = time_it();
+            if (ms < 0.95 * best)
+                best = ms;
+            else
+                pl->dev.col_mfma = own;
+        }
+    } catch (...) {
+        (void)hipFree(buf);
+        throw;
+    }
+    check(hipFree(buf), "hipFree");
+    return best;
+}
+
 void check_batch_args(int count, pm_plan* const* plans, const uint32_t* const* a, const uint32_t* const* b,
                       uint32_t* const* c, pm_batch** out) {
     need(out != nullptr && count > 0 && plans && a && b && c, "bad argument");
@@ -1187,6 +1249,14 @@ int pm_device_info(char* name, size_t len, int* compute_units) {
 
 // --- FHERMA C++ boundary ------------------------------------------------------
 
+int pm_plan_tune_kernels(pm_plan* plan, double* ms_per_product) {
+    return guarded([&] {
+        need(plan != nullptr, "plan is null");
+        const double ms = tune_kernels(plan);
+        if (ms_per_product) *ms_per_product = ms;
+    });
+}
+
 void* fherma_init(const fherma::Point& p) {
     if (p.q.data.size() != p.L) throw std::runtime_error("q must have L limbs");
     pm_plan* plan = nullptr;
@@ -1195,7 +1265,10 @@ void* fherma_init(const fherma::Point& p) {
     // environment, then, unless they name a mode or FHERMA_AUTOTUNE=0, the timed
     // choice of the host modes on dummy operands (solve.cu:817-862).
     const int rc = guarded([&] {
-        if (pm::host_configure_from_env(plan)) pm::host_autotune(plan);
+        if (pm::host_configure_from_env(plan)) {
+            pm::host_autotune(plan);
+            (void)tune_kernels(plan);
+        }
     });
     if (rc != PM_OK) {
         const std::string msg = g_err;

@@ -62,6 +62,7 @@ def _load() -> ctypes.CDLL:
         "pm_plan_set_host_modes": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
         "pm_plan_get_host_modes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "pm_plan_tune_host": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]),
+        "pm_plan_tune_kernels": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_double)]),
         "pm_multiply_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "pm_multiply_device_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "pm_batch_multiply_device": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
@@ -333,6 +334,13 @@ class Plan:
         m, ms = ctypes.c_int(0), ctypes.c_double(0)
         _check(lib.pm_plan_tune_host(self.handle, ctypes.byref(m), ctypes.byref(ms)))
         return m.value, ms.value
+
+    def tune_kernels(self) -> float:
+        """The timed choice of the row-pass twiddle source and the column-pass
+        form; returns the chosen setting's median ms per device product."""
+        ms = ctypes.c_double(0)
+        _check(lib.pm_plan_tune_kernels(self.handle, ctypes.byref(ms)))
+        return ms.value
 
     @property
     def limb_bytes(self) -> int:

@@ -145,6 +145,16 @@ int pm_plan_get_host_modes(const pm_plan* plan, int* in_mode, int* out_mode);
  * FHERMA_OUT or PM_HOST_MODE names a mode or FHERMA_AUTOTUNE (PM_AUTOTUNE) is 0.
  * Either output may be null. */
 int pm_plan_tune_host(pm_plan* plan, int* mode, double* ms_per_product);
+/* The timed choice of the plan's row-pass twiddle source (pm_plan_set_twiddles)
+ * and, where it exists, its column-pass form (pm_plan_set_ntt_form): device-
+ * resident products on dummy operands, two warm-up runs and the median of five
+ * per candidate; the current setting (the reference's size rule for the
+ * twiddles, rns_plan.h:269-280) is kept unless another is at least 5 % faster.
+ * FHERMA_TWIDDLES / PM_NTT_FORM in the environment pin their dimension.
+ * fherma_init calls it after the host choice, under the same switch
+ * (FHERMA_AUTOTUNE). ms_per_product, if not null, receives the chosen setting's
+ * median. */
+int pm_plan_tune_kernels(pm_plan* plan, double* ms_per_product);
 
 /* A batch of independent products, each with its own plan (N and q may differ),
  * all device pointers, enqueued on `stream`. No counterpart in the reference:

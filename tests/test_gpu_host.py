@@ -66,6 +66,31 @@ def test_timed_mode_choice(golden):
         assert oracle.digest(plan.multiply(a, b)) == case["c_sha"]
 
 
+def test_timed_kernel_choice(golden, monkeypatch):
+    """pm_plan_tune_kernels times the other twiddle source and, at N = 65536,
+    the other column-pass form; the settings it keeps give the GMP product, and
+    FHERMA_TWIDDLES / PM_NTT_FORM pin their dimension."""
+    for name in ("config-n65536-w2048-01", "config-n32768-w868-s1"):
+        case = golden[name]
+        q, a, b = pointgen.operands(case)
+        with polymult.Plan(case["N"], case["W"], q) as plan:
+            ms = plan.tune_kernels()
+            say(name, "kept twiddles", plan.twiddles, "ntt form", plan.ntt_form, "%.3f ms" % ms)
+            assert ms > 0
+            assert plan.twiddles in (polymult.TWIDDLES_TABLE, polymult.TWIDDLES_GEN)
+            if case["N"] != 65536:
+                assert plan.ntt_form == polymult.NTT_VALU
+            assert oracle.digest(plan.multiply(a, b)) == case["c_sha"]
+    case = golden["config-n65536-w2048-01"]
+    q, a, b = pointgen.operands(case)
+    monkeypatch.setenv("FHERMA_TWIDDLES", "table")
+    monkeypatch.setenv("PM_NTT_FORM", "mfma")
+    with polymult.Plan(case["N"], case["W"], q) as plan:
+        assert plan.tune_kernels() > 0
+        assert plan.twiddles == polymult.TWIDDLES_TABLE and plan.ntt_form == polymult.NTT_MFMA
+        assert oracle.digest(plan.multiply(a, b)) == case["c_sha"]
+
+
 def write_point(root, case, patterns=("random",)):
     q, a, b = pointgen.operands(case)
     os.makedirs(os.path.join(root, "point"), exist_ok=True)
@@ -164,15 +189,16 @@ def test_driver_refuses_unknown_copy_mode(golden, tmp_path):
 
 
 def test_driver_autotune_switch_and_profile(golden, tmp_path):
-    """FHERMA_AUTOTUNE_LOG lists the timed candidates unless FHERMA_AUTOTUNE=0
-    skips the choice; FHERMA_PROFILE prints the phase times of every call."""
+    """FHERMA_AUTOTUNE_LOG lists the timed candidates (host modes, then the
+    twiddle sources) unless FHERMA_AUTOTUNE=0 skips the choice; FHERMA_PROFILE prints the phase times of every call."""
     case = golden["config-n4096-w256-s1"]
     r = run_driver(tmp_path, case, {"FHERMA_AUTOTUNE_LOG": "1", "FHERMA_PROFILE": "1"})
     assert r.returncode == 0, r.stderr
     assert r.stderr.count("host mode in") >= 8 and r.stderr.count("gpu us: in+decompose") == 2, r.stderr
+    assert r.stderr.count("device twiddles") >= 2, r.stderr
     answers_ok(tmp_path, case)
     r = run_driver(tmp_path, case, {"FHERMA_AUTOTUNE_LOG": "1", "FHERMA_AUTOTUNE": "0"})
-    assert r.returncode == 0 and "host mode in" not in r.stderr, r.stderr
+    assert r.returncode == 0 and "host mode in" not in r.stderr and "device twiddles" not in r.stderr, r.stderr
     answers_ok(tmp_path, case)
 
 

@@ -158,3 +158,5 @@ def test_new_entry_points_refuse_bad_arguments_on_the_host():
     assert lib.pm_device_sum(None, 16, ctypes.byref(s)) == polymult.PM_EINVAL
     assert lib.pm_plan_set_ntt_form(None, 0) == polymult.PM_EINVAL
     assert lib.pm_plan_get_ntt_form(None) == -1
+    ms = ctypes.c_double()
+    assert lib.pm_plan_tune_kernels(None, ctypes.byref(ms)) == polymult.PM_EINVAL
