@@ -115,6 +115,11 @@ def phase_kernels(plan):
     gen = "true" if plan.twiddles == 1 else "false"
     ks = ((plan.K + 3) // 4 + 1 + 3) // 4
     col = "k_ntt_col_reg<8,%s,5," + lazy + ">"
+    if plan.fused_cols == 1:
+        # the first four column stages inside decompose, the next four in the row pass
+        return {"decompose_a": "k_decompose_cols_batch<4,1>", "decompose_b": "k_decompose_cols_batch<4,1>",
+                "ntt_rows_fused": "k_ntt_rows_fused_mid_batch<%s,%s>" % (gen, lazy),
+                "ntt_cols_inv": col % "false", "crt": "k_crt_mfma<%d>" % ks}
     return {"decompose_a": "k_decompose_mfma<4,true>", "decompose_b": "k_decompose_mfma<4,true>",
             "ntt_cols_fwd_a": col % "true", "ntt_cols_fwd_b": col % "true",
             "ntt_rows_fused": "k_ntt_rows_fused_reg<8,%s,%s>" % (gen, lazy), "ntt_cols_inv": col % "false",
@@ -634,6 +639,9 @@ def main():
     for p, a, b, c in zip(plans, da, db, dc):
         for k, v in p.profile_phases(a, b, c, args.phase_iters).items():
             phases[k] = phases.get(k, 0.0) + v / BATCH
+    # phases a plan's schedule leaves empty (reported as 0: the forward column
+    # passes, when their stages run inside decompose and the row pass)
+    phases = {k: v for k, v in phases.items() if v > 0}
     log("phases (ms): " + ", ".join("%s %.4f" % kv for kv in phases.items()))
     nbytes = phase_bytes(K, L)
     dom = max(phases, key=phases.get)

@@ -204,6 +204,20 @@ int ntt_form_default(const pm::DevPlan& d) {
     return mfma && d.colm_fwd && d.colm_inv ? 1 : 0;
 }
 
+// Whether a new plan's product path runs the first forward column stages
+// inside the decompose and the next ones in the row pass (DevPlan::fused_cols)
+// where the geometry allows: PM_FUSED_COLS=1 or auto (the default) yes, 0 no;
+// anything else is refused.
+int fused_cols_default(const pm::DevPlan& d) {
+    bool on = true;
+    if (const char* v = std::getenv("PM_FUSED_COLS")) {
+        need(!std::strcmp(v, "0") || !std::strcmp(v, "1") || !std::strcmp(v, "auto"),
+             "PM_FUSED_COLS must be 0, 1 or auto");
+        on = std::strcmp(v, "0") != 0;
+    }
+    return on && pm::fused_cols_ok(d) ? 1 : 0;
+}
+
 // The host plan of a point with the reference's prime rule; `kernels`: every
 // NTT pass of the geometry has a register kernel, whose faster butterfly form
 // the plan then uses.
@@ -282,6 +296,7 @@ pm_plan* make_plan_object(uint32_t N, uint32_t W, uint32_t L, const uint32_t* q)
     pl->ensure_mfma();
     pl->upload_desc();
     pl->form = PM_FORM_MFMA;
+    d.fused_cols = fused_cols_default(d);
     (void)pm::scratch_on_current_device();   // set-up of the allocation-free entry point on this device
     return pl.release();
 }
@@ -305,6 +320,7 @@ bool groupable(const pm_plan* x, const pm_plan* y) {
 // come from `full`, the full tables of that plan's primes.
 struct NttChoice {
     int col_mfma = -1;
+    int fused_cols = -1;
     int row_gen = -1;
     const pm::FullTables* full = nullptr;
 };
@@ -353,6 +369,7 @@ void run_batched(const std::vector<int>& idx, pm_plan* const* plans, const uint3
         pm::DevPlan P = plans[order[i0]]->dev;
         if (ch.col_mfma >= 0) P.col_mfma = ch.col_mfma;
         if (ch.row_gen >= 0) P.row_gen = ch.row_gen;
+        if (ch.fused_cols >= 0) P.fused_cols = ch.fused_cols && pm::fused_cols_ok(P) ? 1 : 0;
         if (ch.full) {
             P.tw = ch.full->tw;
             P.itw = ch.full->itw;
@@ -375,10 +392,17 @@ void run_batched(const std::vector<int>& idx, pm_plan* const* plans, const uint3
         by_key(ba, n, dec_key, [&](const pm::BatchArgs& sub, int m, int ks) {
             pm::DevPlan Q = P;
             Q.dec_ksteps = ks;
-            pm::launch_decompose_mfma_batch(Q, sub, m, st);
+            if (P.fused_cols)
+                pm::launch_decompose_cols_batch(Q, sub, m, st);
+            else
+                pm::launch_decompose_mfma_batch(Q, sub, m, st);
         });
-        pm::launch_ntt_cols_batch(P, ba, n, true, st);
-        pm::launch_ntt_rows_fused_batch(P, ba, n, st);
+        if (P.fused_cols) {
+            pm::launch_ntt_rows_fused_mid_batch(P, ba, n, st);
+        } else {
+            pm::launch_ntt_cols_batch(P, ba, n, true, st);
+            pm::launch_ntt_rows_fused_batch(P, ba, n, st);
+        }
         pm::launch_ntt_cols_batch(P, ba, n, false, st);
         by_key(ba, n, crt_key, [&](const pm::BatchArgs& sub, int m, int ks) {
             pm::DevPlan Q = P;
@@ -436,10 +460,37 @@ namespace pm {
 
 // Phase i (0 .. PM_NPHASES - 1) of one product on device pointers, enqueued on
 // st, with the residues of a and b in A and B (K x N words each).
+// The product as one batched launch of its own (the kernels' per-product
+// arguments for a batch of one).
+BatchArgs one_product(pm_plan* pl, uint32_t* A, uint32_t* B, const uint32_t* a, const uint32_t* b, uint32_t* c) {
+    BatchArgs ba{};
+    ba.a[0] = a;
+    ba.b[0] = b;
+    ba.c[0] = c;
+    ba.A[0] = A;
+    ba.B[0] = B;
+    ba.d[0] = pl->desc;
+    ba.K[0] = pl->dev.K;
+    return ba;
+}
+
+// A plan whose product runs the fused column stages (matrix-core form only).
+bool runs_fused_cols(const pm_plan* pl) { return pl->form == PM_FORM_MFMA && pl->dev.fused_cols; }
+
+bool phase_empty(const pm_plan* pl, int i) { return runs_fused_cols(pl) && (i == 2 || i == 3); }
+
 void run_phase_in(pm_plan* pl, uint32_t* A, uint32_t* B, const uint32_t* a, const uint32_t* b, uint32_t* c,
                   hipStream_t st, int i) {
     const pm::DevPlan& P = pl->dev;
     const bool mfma = pl->form == PM_FORM_MFMA;
+    if (runs_fused_cols(pl) && i <= 4) {
+        // decompose_a / _b include the first four column stages, the row pass
+        // the next four; the forward column phases are empty
+        const BatchArgs ba = one_product(pl, A, B, a, b, c);
+        if (i <= 1) launch_decompose_cols_batch(P, ba, 1, st, i);
+        if (i == 4) launch_ntt_rows_fused_mid_batch(P, ba, 1, st);
+        return;
+    }
     switch (i) {
         case 0: if (mfma) launch_decompose_mfma(P, a, A, st, true); else launch_decompose_valu(P, a, A, st); break;
         case 1:   // b scaled by the pointwise constant (b_side)
@@ -468,14 +519,13 @@ void run_device_in(pm_plan* pl, uint32_t* A, uint32_t* B, const uint32_t* a, con
                    hipStream_t st) {
     if (!batch_kernels_ok(pl->dev)) {
         for (int i = 0; i < PM_NPHASES; ++i) run_phase_in(pl, A, B, a, b, c, st, i);
+    } else if (runs_fused_cols(pl)) {
+        const BatchArgs ba = one_product(pl, A, B, a, b, c);
+        launch_decompose_cols_batch(pl->dev, ba, 1, st);
+        launch_ntt_rows_fused_mid_batch(pl->dev, ba, 1, st);
+        for (int i = 5; i < PM_NPHASES; ++i) run_phase_in(pl, A, B, a, b, c, st, i);
     } else {
-        BatchArgs ba{};
-        ba.a[0] = a;
-        ba.b[0] = b;
-        ba.A[0] = A;
-        ba.B[0] = B;
-        ba.d[0] = pl->desc;
-        ba.K[0] = pl->dev.K;
+        const BatchArgs ba = one_product(pl, A, B, a, b, c);
         if (pl->form == PM_FORM_MFMA) {
             launch_decompose_mfma_batch(pl->dev, ba, 1, st);   // b with the b-side tables
         } else {
@@ -918,6 +968,17 @@ int pm_plan_set_ntt_form(pm_plan* plan, int form) {
     });
 }
 
+int pm_plan_set_fused_cols(pm_plan* plan, int on) {
+    return guarded([&] {
+        need(plan != nullptr, "plan is null");
+        need(on == 0 || on == 1, "fused_cols must be 0 or 1");
+        need(on == 0 || pm::fused_cols_ok(plan->dev), "fused column stages need N = 65536 and the matrix-core tables");
+        plan->dev.fused_cols = on;
+    });
+}
+
+int pm_plan_get_fused_cols(const pm_plan* plan) { return plan ? plan->dev.fused_cols : -1; }
+
 int pm_plan_get_ntt_form(const pm_plan* plan) { return plan ? (plan->dev.col_mfma ? PM_NTT_MFMA : PM_NTT_VALU) : -1; }
 
 int pm_plan_get_twiddles(const pm_plan* plan) {
@@ -1144,7 +1205,8 @@ int pm_profile_phases(pm_plan* plan, const uint32_t* a, const uint32_t* b, uint3
         for (int i = 0; i < PM_NPHASES; ++i) {
             float ms = 0;
             check(hipEventElapsedTime(&ms, plan->ev[i], plan->ev_stop[i]), "hipEventElapsedTime");
-            ms_per_phase[i] = ms / iters;
+            // the forward column phases of a plan with fused column stages launch nothing
+            ms_per_phase[i] = pm::phase_empty(plan, i) ? 0.0 : ms / iters;
         }
     });
 }

@@ -42,6 +42,13 @@ struct DevPlan {
     const uint8_t* colm_fwd;
     const uint8_t* colm_inv;
     int col_mfma;             // 1: the column passes run on the matrix cores (tables present)
+    // 1: a product's forward NTT runs its first four column stages (16-point
+    // columns of stride N/16) inside the matrix-core decompose and the next
+    // four (those within a block of 16 rows) at the start of the fused row
+    // pass (launch_decompose_cols_batch, launch_ntt_rows_fused_mid_batch):
+    // N = 65536 (R = S = 256), matrix-core form. Read only by the product
+    // paths that launch those kernels; every other launcher ignores it.
+    int fused_cols;
     // Full tables, null unless the row pass reads its twiddles from tables or
     // the geometry needs the generic kernels.
     const uint2* tw;          // K x N: (psi^bitrev(i), Shoup companion)
@@ -181,6 +188,12 @@ void launch_decompose_mfma_batch(const DevPlan& P, const BatchArgs& ba, int coun
 void launch_ntt_cols_batch(const DevPlan& P, const BatchArgs& ba, int count, bool forward, hipStream_t st);
 void launch_ntt_rows_fused_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st);
 void launch_crt_mfma_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st);
+// fused_cols plans (DevPlan::fused_cols): the decompose of a and b (side -1),
+// of a only (0) or of b only (1), each followed by the first four forward
+// column stages, and the fused row pass that starts with the next four.
+bool fused_cols_ok(const DevPlan& P);   // the geometry and tables these kernels need
+void launch_decompose_cols_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st, int side = -1);
+void launch_ntt_rows_fused_mid_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st);
 
 // Plain grid-stride copy of `bytes` (a multiple of 16) with 16-byte accesses.
 void launch_copy16(void* dst, const void* src, size_t bytes, hipStream_t st);

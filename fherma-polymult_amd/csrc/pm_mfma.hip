@@ -45,6 +45,7 @@
 // inside a lane only has to agree between A and B, which it does by
 // construction.
 #include "pm_arith.h"
+#include "pm_bfly.h"
 #include "pm_bigint.h"
 #include "pm_device.h"
 #include "pm_mfma_host.h"
@@ -230,6 +231,172 @@ __global__ void __launch_bounds__(64 * kWaves) k_decompose_mfma_batch(DevPlan P,
     Q.dec_digits = second ? d->dec_digits_b : d->dec_digits;
     Q.dec_init = second ? d->dec_init_b : d->dec_init;
     decompose_tile<KT, kLazyOut>(second ? ba.b[i] : ba.a[i], second ? ba.B[i] : ba.A[i], Q, 0, P.N);
+}
+
+// Decompose fused with the first four forward column stages of N = 65536
+// (DevPlan::fused_cols): coefficient n = s + N1 r with N1 = N / 16, r < 16;
+// the stages of distance N/2 .. N/16 are 16-point transforms over r for each
+// inner position s, with twiddle index (16 + r) >> (e + 1) for the element r
+// (phase B's form for t = 0, B = 16: the first 16 pairs of the prime's
+// forward table). A workgroup of eight waves takes 32 consecutive s and all
+// 16 r (512 coefficients): wave w holds the B fragments of the four tiles
+// r = 4 (w & 3) .. + 3 at s = s0 + 16 (w >> 2) + [0, 16) in registers (loaded
+// straight from the limbs, no LDS) and loops over the prime groups
+// [g_begin, g_end). After each group the residues go to an LDS exchange buffer
+// [prime of the batch][r][s]; after every four groups (16 primes) one barrier,
+// and each thread runs one (prime, s) column through the four stages in
+// registers and stores its 16 residues (two primes x 32 s per wave: 128-byte
+// segments). Two exchange buffers alternate, so one barrier per batch keeps
+// every read of a buffer ahead of its next write. The decompose's [0, 2p)
+// residues feed ct_half as they are (kMode 1).
+constexpr int kFcWaves = 8;
+constexpr int kFcCols = 32;                    // inner positions per workgroup
+constexpr int kFcStride = 16 * kFcCols + 16;   // exchange words per prime; 16 mod 64 banks apart
+
+// LDS: two exchange buffers, two sets of the batch's 16 x 16 twiddle pairs,
+// the per-prime constants
+__host__ __device__ constexpr uint32_t fc_lds_bytes(int K) {
+    return 4u * 2u * 16u * kFcStride + 2u * 16u * 16u * 8u + 16u * static_cast<uint32_t>(K);
+}
+
+struct OuterTw {   // the first 16 pairs of the prime's forward table (Shoup pairs), staged in LDS
+    static constexpr bool kMont = false;
+    const uint2* tab;
+    __device__ __forceinline__ uint2 b(int e, uint32_t, int blk) const { return tab[(16u >> (e + 1)) + blk]; }
+};
+
+template <int KT, int kMode>
+__device__ __forceinline__ void decompose_cols_block(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
+                                                     const DevPlan& P, int g_begin, int g_end) {
+    extern __shared__ uint32_t xs[];
+    const int K = P.K, L = P.L;
+    uint2* twl = reinterpret_cast<uint2*>(xs + 2 * 16 * kFcStride);   // [2][16 primes][16 pairs]
+    int2* kinit = reinterpret_cast<int2*>(twl + 2 * 16 * 16);
+    uint32_t* kp = reinterpret_cast<uint32_t*>(kinit + K);
+    uint32_t* kpneg = kp + K;
+    for (int j = threadIdx.x; j < K; j += blockDim.x) {
+        kinit[j] = P.dec_init[j];
+        kp[j] = P.pc[j].p;
+        kpneg[j] = P.pc[j].pneg;
+    }
+    const uint32_t N1 = P.N >> 4;
+    const uint32_t s0 = blockIdx.x * kFcCols;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, cl = lane & 15;
+    const int sh = wave >> 2, rq = wave & 3;
+    // B fragments: tile t is r = 4 rq + t; lane (g, cl) holds words 16 ks + 4 g
+    // .. + 3 of coefficient s0 + 16 sh + cl + N1 r as int8 digits u ^ 0x80,
+    // words past L as 0 (digit 0x80)
+    v4i bf[kTiles][KT];
+#pragma unroll
+    for (int t = 0; t < kTiles; ++t) {
+        const uint32_t n = s0 + 16 * sh + cl + N1 * (4 * rq + t);
+        const uint32_t* src = limbs + static_cast<size_t>(n) * L;
+#pragma unroll
+        for (int ks = 0; ks < KT; ++ks) {
+            const int w0 = 16 * ks + 4 * g;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if ((L & 3) == 0) {
+                // 16-byte aligned: n L and w0 are multiples of 4, and w0 < L
+                // puts all four words below L
+                if (w0 < L) v = *reinterpret_cast<const uint4*>(src + w0);
+            } else {
+                v.x = w0 < L ? src[w0] : 0u;
+                v.y = w0 + 1 < L ? src[w0 + 1] : 0u;
+                v.z = w0 + 2 < L ? src[w0 + 2] : 0u;
+                v.w = w0 + 3 < L ? src[w0 + 3] : 0u;
+            }
+            bf[t][ks] = v4i{static_cast<int>(v.x ^ 0x80808080u), static_cast<int>(v.y ^ 0x80808080u),
+                            static_cast<int>(v.z ^ 0x80808080u), static_cast<int>(v.w ^ 0x80808080u)};
+        }
+    }
+    const v4i* dig = reinterpret_cast<const v4i*>(P.dec_digits);
+    const int groups = (K + 3) >> 2;
+    v4i an[KT];
+#pragma unroll
+    for (int ks = 0; ks < KT; ++ks) an[ks] = dig[(min(g_begin, groups - 1) * KT + ks) * 64 + lane];
+    uint32_t k16 = 1u << 16;
+    asm volatile("" : "+s"(k16));
+    // the transform's thread: primes of a batch in pairs 2 apart per wave (the
+    // pair's exchange rows then sit 32 banks apart), 32 s per prime
+    const int tw_ = threadIdx.x >> 6, th = (threadIdx.x >> 5) & 1;
+    const int tpl = 4 * (tw_ >> 1) + (tw_ & 1) + 2 * th;
+    const uint32_t tsl = threadIdx.x & 31;
+    __syncthreads();   // the per-prime constants
+    int buf = 0;
+    for (int jb = g_begin; jb < g_end; jb += 4, buf ^= 1) {
+        uint32_t* xe = xs + buf * 16 * kFcStride;
+        uint2* twb = twl + buf * 256;
+        // the batch's twiddles: thread i < 256 copies pair i % 16 of prime 4 jb + i / 16
+        // (a prime past K copies the last prime's, never read)
+        if (threadIdx.x < 256)
+            twb[threadIdx.x] = P.ctw[(static_cast<size_t>(min(4 * jb + static_cast<int>(threadIdx.x >> 4), K - 1)) << P.logR) +
+                                     (threadIdx.x & 15)];
+#pragma unroll 1
+        for (int q = 0; q < 4; ++q) {
+            const int jg = jb + q;
+            if (jg >= g_end) break;
+            const int j = 4 * jg + g;
+            const int2 init = kinit[min(j, K - 1)];
+            v4i acc[kTiles];
+#pragma unroll
+            for (int t = 0; t < kTiles; ++t) acc[t] = v4i{init.x, 0, init.y, 0};
+            // the next group's A fragments load while this group's MFMAs and
+            // epilogue run (each K-step's fragment is reloaded once its MFMAs
+            // are issued)
+            const int jn = min(jg + 1, groups - 1);
+#pragma unroll
+            for (int ks = 0; ks < KT; ++ks) {
+#pragma unroll
+                for (int t = 0; t < kTiles; ++t) acc[t] = mfma16(an[ks], bf[t][ks], acc[t]);
+                an[ks] = dig[(jn * KT + ks) * 64 + lane];
+            }
+            const uint32_t p = kp[min(j, K - 1)], pneg = kpneg[min(j, K - 1)];
+#pragma unroll
+            for (int t = 0; t < kTiles; ++t) {
+                // the fold of decompose_tile: R in (0, 2^48), one Montgomery step
+                const int lo = acc[t][0] + acc[t][1] * 256;
+                const uint32_t hi = static_cast<uint32_t>(acc[t][2]) + static_cast<uint32_t>(acc[t][3]) * 256u;
+                const uint64_t Rw = static_cast<uint64_t>(hi) * k16 + static_cast<uint64_t>(static_cast<int64_t>(lo));
+                const uint32_t m = static_cast<uint32_t>(Rw) * pneg;
+                const uint32_t v = static_cast<uint32_t>((static_cast<uint64_t>(m) * p + Rw) >> 32);
+                xe[(4 * q + g) * kFcStride + (4 * rq + t) * kFcCols + 16 * sh + cl] = kMode == 1 ? v : min(v, v - p);
+            }
+        }
+        __syncthreads();
+        const int j = 4 * jb + tpl;
+        if (j < 4 * g_end && j < K) {
+            uint32_t x[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) x[r] = xe[tpl * kFcStride + r * kFcCols + tsl];
+            const OuterTw ot{twb + 16 * tpl};
+            phase_b<true, false, kMode>(x, x, ot, 0, kp[j]);
+            uint32_t* out = res + static_cast<size_t>(j) * P.N + s0 + tsl;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) out[N1 * r] = x[r];
+        }
+    }
+}
+
+// A batch: blockIdx.z = 2 (product) + (0: a into A, 1: b into B), or, for
+// side 0 / 1, blockIdx.z = product and that side only; blockIdx.y = the range
+// of prime groups [y gps, (y + 1) gps) (gps a multiple of 4).
+template <int KT, int kMode>
+__global__ void __launch_bounds__(64 * kFcWaves, 4) k_decompose_cols_batch(DevPlan P, BatchArgs ba, int side, int gps) {
+    const bool both = side < 0;
+    const int i = both ? blockIdx.z >> 1 : blockIdx.z;
+    const bool second = both ? (blockIdx.z & 1) != 0 : side == 1;
+    const auto d = as_const(ba.d[i]);
+    const int groups = (d->K + 3) >> 2;
+    const int g0 = static_cast<int>(blockIdx.y) * gps;
+    if (g0 >= groups) return;
+    DevPlan Q = P;
+    Q.K = d->K;
+    Q.L = d->L;
+    Q.dec_digits = second ? d->dec_digits_b : d->dec_digits;
+    Q.dec_init = second ? d->dec_init_b : d->dec_init;
+    decompose_cols_block<KT, kMode>(second ? ba.b[i] : ba.a[i], second ? ba.B[i] : ba.A[i], Q, g0,
+                                    min(groups, g0 + gps));
 }
 
 // CRT, one workgroup (four waves) per 64 coefficients, in one kernel:
@@ -938,6 +1105,35 @@ void launch_crt_mfma_range(const DevPlan& P, const uint32_t* Y, uint32_t* c, uin
 // may stay in [0, 2p) when the plan's butterflies accept that.
 void launch_decompose_mfma_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st) {
     dec_dispatch(P, nullptr, nullptr, 0, P.N, st, P.lazy != 0, &ba, count);
+}
+
+namespace {
+
+template <int KT>
+void fc_launch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st, int side) {
+    // every product of the launch takes its own K; the grid covers the longest
+    const int groups = (P.K + 3) / 4;
+    const int splits = 1;
+    const int gps = 4 * ((groups + 4 * splits - 1) / (4 * splits));
+    const dim3 grid((P.N >> 4) / kFcCols, (groups + gps - 1) / gps, side < 0 ? 2 * count : count);
+    const size_t lds = fc_lds_bytes(P.K);
+    if (P.lazy)
+        launch(k_decompose_cols_batch<KT, 1>, grid, dim3(64 * kFcWaves), lds, st, P, ba, side, gps);
+    else
+        launch(k_decompose_cols_batch<KT, 0>, grid, dim3(64 * kFcWaves), lds, st, P, ba, side, gps);
+}
+
+}  // namespace
+
+void launch_decompose_cols_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st, int side) {
+    if (!fused_cols_ok(P)) throw std::logic_error("fused column stages need N = 65536 and the matrix-core tables");
+    switch (P.dec_ksteps) {
+        case 1: fc_launch<1>(P, ba, count, st, side); return;
+        case 2: fc_launch<2>(P, ba, count, st, side); return;
+        case 3: fc_launch<3>(P, ba, count, st, side); return;
+        case 4: fc_launch<4>(P, ba, count, st, side); return;
+        default: throw std::runtime_error("L out of range");
+    }
 }
 
 void launch_crt_mfma_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st) {

@@ -19,6 +19,7 @@
 #include "pm_device.h"
 
 #include "pm_arith.h"
+#include "pm_bfly.h"
 
 #include <stdexcept>
 
@@ -29,90 +30,6 @@ void launch_ntt_rows_generic(const DevPlan& P, uint32_t* X, bool forward, hipStr
 void launch_ntt_rows_fused_generic(const DevPlan& P, uint32_t* A, const uint32_t* B, hipStream_t st);
 
 namespace {
-
-__device__ __forceinline__ void ct(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    const uint32_t v = mul_shoup(y, w.x, w.y, p);
-    y = sub_mod(x, v, p);
-    x = add_mod(x, v, p);
-}
-
-// x - y + p lies in (0, 2p), below 2^32, which the Shoup product accepts as is.
-__device__ __forceinline__ void gs(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    const uint32_t u = x;
-    x = add_mod(u, y, p);
-    y = mul_shoup(u - y + p, w.x, w.y, p);
-}
-
-// The forward butterfly for any p < 2^31 with residues in [0, 2p): x comes
-// down to [0, p); the lazy Shoup remainder v of y w (in [0, 2p)) is formed as
-// m = -v - 1 mod 2^32, and max(m, m + p) = -v_r - 1 with v_r = v mod p in
-// [0, p) (the max picks m + p exactly when it does not wrap, that is when
-// v >= p + 1, and otherwise m, whose v is then p or below; v = p itself gives
-// m + p = 2^32 - 1). Then x + v_r = ~M + x and x - v_r + p = x + M + p + 1
-// take one instruction each: 10 VALU ops against 12 for the canonical form.
-__device__ __forceinline__ void ct_half(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    const uint32_t m = mulhi32(y, w.y) * p - y * w.x - 1u;
-    const uint32_t M = max(m, m + p);
-    const uint32_t xr = csub(x, p);
-    x = ~M + xr;
-    y = xr + M + (p + 1u);
-}
-
-// The same butterflies with a Montgomery twiddle pair (w.x = w' = w 2^32 mod p,
-// canonical, and w.y = w' p^-1 mod 2^32; mont_tw_diff in pm_arith.h): the
-// inverse tables hold this form, which saves one VALU op per inverse
-// butterfly, and the row pass generates it for 7 VALU ops against 13 for a
-// Shoup pair. The product's difference of high words d lies in (-p, p).
-
-// Canonical CT: x, y in [0, p); v = d mod p.
-__device__ __forceinline__ void ct_m(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    const uint32_t v = canon_signed(mont_tw_diff(y, w.x, w.y, p), p);
-    y = sub_mod(x, v, p);
-    x = add_mod(x, v, p);
-}
-
-// ct_half's form (p < 2^31, x in [0, 2p)): a = h2 + ~h1 = -d - 1, and
-// max(a, a - p) = -v - 1 with v = d mod p (for d >= 0, a >= 2^32 - p and a - p
-// does not wrap; for d < 0, a < p and a - p wraps above it). 10 VALU ops, as
-// ct_half.
-__device__ __forceinline__ void ct_half_m(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    const uint32_t h1 = mulhi32(y, w.x), h2 = mulhi32(y * w.y, p);
-    const uint32_t a = h2 + ~h1;
-    const uint32_t M = max(a, a - p);
-    const uint32_t xr = csub(x, p);
-    x = ~M + xr;
-    y = xr + M + (p + 1u);
-}
-
-// Canonical GS: x, y in [0, p); x - y in (-p, p) as a signed word,
-// |(x - y) w'| < p^2.
-__device__ __forceinline__ void gs_m(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    const uint32_t r = mont_tw_diff_signed(x - y, w.x, w.y, p);
-    x = add_mod(x, y, p);
-    y = canon_signed(r, p);
-}
-
-// Butterfly forms (kMode): 0 canonical, residues in [0, p); 1 forward with
-// residues in [0, 2p) (ct_half), inverse canonical. kMont: the twiddle is a
-// Montgomery pair (generated row twiddles, inverse tables), else a Shoup pair.
-template <bool kFwd, int kMode, bool kMont>
-__device__ __forceinline__ void bfly(uint32_t& x, uint32_t& y, const uint2 w, uint32_t p) {
-    if constexpr (kMont) {
-        if (kFwd && kMode == 1)
-            ct_half_m(x, y, w, p);
-        else if (kFwd)
-            ct_m(x, y, w, p);
-        else
-            gs_m(x, y, w, p);
-    } else {
-        if (kFwd && kMode == 1)
-            ct_half(x, y, w, p);
-        else if (kFwd)
-            ct(x, y, w, p);
-        else
-            gs(x, y, w, p);
-    }
-}
 
 // Twiddle sources: tw(e, b) is the twiddle of block b (counted from the start
 // of the sub-transform) of the stage with distance 2^e.
@@ -203,55 +120,6 @@ __device__ __forceinline__ void phase_a(uint32_t (&x)[16], uint32_t (&y)[16], co
         }
     }
 }
-
-template <bool kFwd, bool kTwo, int kMode, class TW>
-__device__ __forceinline__ void phase_b(uint32_t (&x)[16], uint32_t (&y)[16], const TW& tw, uint32_t t,
-                                        uint32_t p) {
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int e = kFwd ? 3 - s : s;
-        const int d = 1 << e;
-#pragma unroll
-        for (int blk = 0; blk < 8 / d; ++blk) {
-            const uint2 w = tw.b(e, t, blk);
-#pragma unroll
-            for (int k = blk * 2 * d; k < blk * 2 * d + d; ++k) {
-                bfly<kFwd, kMode, TW::kMont>(x[k], x[k + d], w, p);
-                if (kTwo) bfly<kFwd, kMode, TW::kMont>(y[k], y[k + d], w, p);
-            }
-        }
-    }
-}
-
-__device__ __forceinline__ uint32_t padded(uint32_t u) { return u + (u >> 4); }
-
-// The M / 16 threads of a row are consecutive lanes of one wave, so a row's
-// LDS exchange needs only wave-level ordering (no workgroup barrier).
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// A raw buffer over `bytes` bytes from `base` (gfx9 resource word 3): a lane's
-// address is its byte offset plus a wave-uniform offset in a scalar register,
-// so strided accesses cost no per-access address arithmetic, and accesses past
-// `bytes` read 0 / are dropped by the hardware.
-struct Buf {
-    __amdgpu_buffer_rsrc_t r;
-    __device__ __forceinline__ Buf(const void* base, uint32_t bytes)
-        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), 0x00020000)) {}
-    __device__ __forceinline__ uint32_t ld(uint32_t voff, uint32_t soff) const {
-        return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
-    }
-    __device__ __forceinline__ void st(uint32_t v, uint32_t voff, uint32_t soff) const {
-        __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, 0);
-    }
-    __device__ __forceinline__ uint2 ld2(uint32_t voff, uint32_t soff) const {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-        return make_uint2(v[0], v[1]);
-    }
-};
 
 // The column pass's twiddles (the first R pairs of the prime's table, all
 // that its stages use) through a buffer: phase A's are
@@ -524,6 +392,73 @@ __global__ void __launch_bounds__(512) k_ntt_rows_fused_batch(DevPlan P, BatchAr
     rows_fused<LOGM, kGen, kMode>(ba.A[blockIdx.z], ba.B[blockIdx.z], P, logG);
 }
 
+// The forward column stages of distance S .. 8 S at rows rho0 .. rho0 + 15 (a
+// block of 16 S elements, rho0 a multiple of 16): global index
+// (N + k) / (2 S 2^e) = (R + rho0 + i) >> (e + 1) for the element k of block row
+// i, which is ((R + rho0) >> (e + 1)) + (i >> (e + 1)), phase B's form for
+// t = 0 and B = R + rho0. Uniform over the workgroup: scalar loads from the
+// first R pairs of the prime's forward table.
+struct MidTw {
+    static constexpr bool kMont = false;
+    kconst_ptr<uint64_t> tab;
+    uint32_t B;
+    __device__ __forceinline__ uint2 b(int e, uint32_t, int blk) const {
+        const uint64_t w = tab[(B >> (e + 1)) + blk];
+        return make_uint2(static_cast<uint32_t>(w), static_cast<uint32_t>(w >> 32));
+    }
+};
+
+// Row pass fused, for products whose decompose ran the first four column
+// stages (DevPlan::fused_cols, N = 65536, R = S = 256): the workgroup's 16
+// rows are the block rho0 .. rho0 + 15, and the four column stages left
+// (distance S .. 8 S) mix them. Thread c first holds column c of the block
+// (element c + S i in register i) for A and for B, runs those stages on both
+// with shared twiddles, and passes them through the rows' exchange buffer into
+// the phase-A layout of row_item (one workgroup barrier after each write, and
+// one before the buffer is written again); row_item then runs as in
+// rows_fused. The loads are 1 KB contiguous per instruction.
+template <bool kGen, int kMode>
+__device__ __forceinline__ void rows_fused_mid(uint32_t* __restrict__ A, const uint32_t* __restrict__ Bm,
+                                               const DevPlan& P) {
+    extern __shared__ uint32_t lds[];
+    constexpr uint32_t M = 256, GA = 16, ROW = M + M / 16;
+    const uint32_t c = threadIdx.x, t = c & 15, rl = c >> 4;
+    const uint32_t rho0 = blockIdx.x << 4;
+    const int j = blockIdx.y;
+    const size_t base = static_cast<size_t>(j) * P.N + static_cast<size_t>(rho0) * M;
+    uint32_t a[16], b[16];
+#pragma unroll
+    for (uint32_t i = 0; i < 16; ++i) {
+        a[i] = A[base + i * M + c];
+        b[i] = Bm[base + i * M + c];
+    }
+    const uint32_t R = 1u << P.logR;
+    const MidTw mt{as_const(reinterpret_cast<const uint64_t*>(P.ctw) + static_cast<size_t>(j) * R), R + rho0};
+    const uint32_t p = P.pc[j].p;
+    phase_b<true, true, kMode>(a, b, mt, 0, p);
+    uint32_t* la = lds + rl * ROW;
+#pragma unroll
+    for (uint32_t i = 0; i < 16; ++i) lds[i * ROW + padded(c)] = a[i];
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) a[k] = la[padded(t + 16 * k)];
+    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < 16; ++i) lds[i * ROW + padded(c)] = b[i];
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < 16; ++k) b[k] = la[padded(t + 16 * k)];
+    uint2* twa = reinterpret_cast<uint2*>(lds + 16 * ROW) + rl * GA;
+    row_item<8, kGen, kMode>(A, P, j, rho0 + rl, base + static_cast<size_t>(rl) * M, t, la, twa, a, b);
+}
+
+// A batch: blockIdx.z = product.
+template <bool kGen, int kMode>
+__global__ void __launch_bounds__(256) k_ntt_rows_fused_mid_batch(DevPlan P, BatchArgs ba) {
+    if (static_cast<int>(blockIdx.y) >= ba.K[blockIdx.z]) return;   // primes past this product's K
+    rows_fused_mid<kGen, kMode>(ba.A[blockIdx.z], ba.B[blockIdx.z], P);
+}
+
 bool reg_ok(uint32_t logm) { return logm >= 4 && logm <= 8; }
 
 // Columns per column-pass workgroup: up to 512 threads, at most S columns.
@@ -716,6 +651,22 @@ void launch_ntt_cols_batch(const DevPlan& P, const BatchArgs& ba, int count, boo
         case 7 * 16 + 6: col_launch2<7, 6>(P, nullptr, forward, st, &ba, count); return;
         case 8 * 16 + 5: col_launch2<8, 5>(P, nullptr, forward, st, &ba, count); return;
         default: throw std::runtime_error("no batched column pass for this geometry");
+    }
+}
+
+bool fused_cols_ok(const DevPlan& P) {
+    return P.logN == 16 && P.logR == 8 && P.logS == 8 && P.dec_digits && P.ctw && (P.row_gen || (P.tw && P.itw));
+}
+
+void launch_ntt_rows_fused_mid_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st) {
+    if (!fused_cols_ok(P)) throw std::logic_error("fused column stages need N = 65536 and the matrix-core tables");
+    const dim3 grid(16, P.K, count);   // 256 rows in blocks of 16
+    const size_t lds = sizeof(uint32_t) * (16 * (256 + 16)) + sizeof(uint2) * 16 * 16;
+    switch (P.lazy * 2 + (P.row_gen ? 1 : 0)) {
+        case 3: launch((k_ntt_rows_fused_mid_batch<true, 1>), grid, dim3(256), lds, st, P, ba); break;
+        case 2: launch((k_ntt_rows_fused_mid_batch<false, 1>), grid, dim3(256), lds, st, P, ba); break;
+        case 1: launch((k_ntt_rows_fused_mid_batch<true, 0>), grid, dim3(256), lds, st, P, ba); break;
+        default: launch((k_ntt_rows_fused_mid_batch<false, 0>), grid, dim3(256), lds, st, P, ba); break;
     }
 }
 

@@ -80,6 +80,8 @@ def _load() -> ctypes.CDLL:
         "pm_plan_set_twiddles": (ctypes.c_int, [_vp, ctypes.c_int]),
         "pm_plan_set_ntt_form": (ctypes.c_int, [_vp, ctypes.c_int]),
         "pm_plan_get_ntt_form": (ctypes.c_int, [_vp]),
+        "pm_plan_set_fused_cols": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "pm_plan_get_fused_cols": (ctypes.c_int, [_vp]),
         "pm_plan_get_twiddles": (ctypes.c_int, [_vp]),
         "pm_batch_destroy": (None, [_vp]),
         "pm_stage_decompose": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
@@ -307,6 +309,15 @@ class Plan:
     @property
     def ntt_form(self) -> int:
         return lib.pm_plan_get_ntt_form(self.handle)
+
+    def set_fused_cols(self, on: bool) -> None:
+        """Run the first forward column stages inside decompose and the row pass
+        (N = 65536, matrix-core form), or the separate column passes."""
+        _check(lib.pm_plan_set_fused_cols(self.handle, 1 if on else 0))
+
+    @property
+    def fused_cols(self) -> int:
+        return lib.pm_plan_get_fused_cols(self.handle)
 
     def set_twiddles(self, source: int) -> None:
         """Row-pass twiddle source: TWIDDLES_TABLE or TWIDDLES_GEN (same results)."""

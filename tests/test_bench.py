@@ -47,7 +47,7 @@ def test_golden_digests_cover_the_batch():
 
 
 def test_kernel_names_match_the_committed_profiles():
-    plan = SimpleNamespace(twiddles=1, K=134)
+    plan = SimpleNamespace(twiddles=1, K=134, fused_cols=0)
     names = bench.phase_kernels(plan)
     assert names["ntt_rows_fused"] == "k_ntt_rows_fused_reg<8,true,1>"
     for phase, kernel in names.items():

@@ -159,3 +159,80 @@ def test_register_schedule_matches_oracle_ntt(N):
         itw = np.array([pow(ipsi, e, p) for e in brv], dtype=object)
         assert emulate(res[j], tw, itw, N, p, True, psi) == fwd[j].tolist()
         assert emulate(fwd[j], tw, itw, N, p, False, psi) == inv[j].tolist()
+
+
+def fused_cols_forward(x, tw, N, p):
+    """The forward NTT as a product with fused column stages runs it (N = 2^16,
+    R = S = 256): the decompose kernel's four stages over r (coefficient
+    s + (N/16) r, phase B with t = 0 and B = 16), the row pass's four stages over
+    the 16 rows of each block (phase B with t = 0 and B = R + rho0, thread c
+    holding column c), the exchange into the rows' phase-A layout, then the row
+    pass's own eight stages (read here from the table; the generated twiddles
+    are checked against it above)."""
+    logN = N.bit_length() - 1
+    logS, logR = geometry(logN)
+    S, R, N1 = 1 << logS, 1 << logR, N // 16
+    x = list(map(int, x))
+    for s in range(N1):
+        col = [x[s + N1 * r] for r in range(16)]
+        phase_b(col, True, table_tw(tw, 16), 0, p)
+        for r in range(16):
+            x[s + N1 * r] = col[r]
+    ROW = S + S // 16
+    for rho0 in range(0, R, 16):
+        lds = [None] * (16 * ROW)
+        for c in range(S):
+            col = [x[(rho0 + i) * S + c] for i in range(16)]
+            phase_b(col, True, table_tw(tw, R + rho0), 0, p)
+            for i in range(16):
+                assert lds[i * ROW + padded(c)] is None
+                lds[i * ROW + padded(c)] = col[i]
+        for rl in range(16):
+            row = [lds[rl * ROW + padded(u)] for u in range(S)]
+            t = table_tw(tw, N + (rho0 + rl) * S)
+            x[(rho0 + rl) * S:(rho0 + rl + 1) * S] = sub_transform(row, logS, True, t, t, p)
+    return x
+
+
+def test_fused_cols_exchange_maps():
+    """decompose_cols_block's exchange: a write at (batch prime 4 q + g, r, s) and
+    the transform thread's reads use one address pl 528 + 32 r + s, and the
+    transform threads cover the 16 primes x 32 s once; the two primes of a
+    wave sit 32 banks apart, the four of a write 16 apart (mod 64)."""
+    stride = 16 * 32 + 16
+    writes = set()
+    for q in range(4):
+        for g in range(4):
+            for rq in range(4):
+                for t in range(4):
+                    for sh in range(2):
+                        for cl in range(16):
+                            writes.add((4 * q + g) * stride + (4 * rq + t) * 32 + 16 * sh + cl)
+    assert len(writes) == 16 * 16 * 32 and max(writes) < 16 * stride
+    threads = set()
+    for tid in range(512):
+        w, th = tid >> 6, (tid >> 5) & 1
+        threads.add((4 * (w >> 1) + (w & 1) + 2 * th, tid & 31))
+    assert threads == {(pl, s) for pl in range(16) for s in range(32)}
+    assert {(g * stride) % 64 for g in range(4)} == {0, 16, 32, 48}
+    for w in range(8):
+        a, b = (4 * (w >> 1) + (w & 1)) * stride, (4 * (w >> 1) + (w & 1) + 2) * stride
+        assert (b - a) % 64 == 32
+
+
+def test_fused_cols_schedule_matches_oracle_ntt():
+    N = 65536
+    q, a, _ = oracle.gen_point(22, N, 128)
+    plan = oracle.Plan(N, q)
+    res = plan.decompose(a)
+    fwd = plan.ntt_forward(res)
+    logN = N.bit_length() - 1
+    brv = np.array([int(format(i, "016b")[::-1], 2) for i in range(N)])
+    j = plan.K - 1
+    p = int(plan.primes[j])
+    psi = int(plan.psi[j])
+    pw = [1] * N
+    for e in range(1, N):
+        pw[e] = pw[e - 1] * psi % p
+    tw = [pw[e] for e in brv]
+    assert fused_cols_forward(res[j], tw, N, p) == fwd[j].tolist()
