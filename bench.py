@@ -117,7 +117,7 @@ def phase_kernels(plan):
     col = "k_ntt_col_reg<8,%s,5," + lazy + ">"
     if plan.fused_cols == 1:
         # the first four column stages inside decompose, the next four in the row pass
-        return {"decompose_a": "k_decompose_cols_batch<4,1>", "decompose_b": "k_decompose_cols_batch<4,1>",
+        return {"decompose_a": "k_decompose_cols_batch<4,1>@1", "decompose_b": "k_decompose_cols_batch<4,1>@1",
                 "ntt_rows_fused": "k_ntt_rows_fused_mid_batch<%s,%s>" % (gen, lazy),
                 "ntt_cols_inv": col % "false", "crt": "k_crt_mfma<%d>" % ks}
     return {"decompose_a": "k_decompose_mfma<4,true>", "decompose_b": "k_decompose_mfma<4,true>",
@@ -153,6 +153,8 @@ CLOCK_GHZ = 2.4
 # per-wave count; it matches SQ_INSTS_VALU per wave).
 VALU_MIX = {
     "k_ntt_rows_fused_reg<8,true,1>": {"other": 1584, "mul_hi": 448, "mul_lo": 321, "mad64": 1},
+    # with the four column stages of distance 256 .. 2048 on A and B before the rows
+    "k_ntt_rows_fused_mid_batch<true,1>": {"other": 2055, "mul_hi": 512, "mul_lo": 448},
     "k_ntt_col_reg<8,true,5,1>": {"other": 464, "mul_hi": 64, "mul_lo": 129},
     "k_ntt_col_reg<8,false,5,1>": {"other": 464, "mul_hi": 128, "mul_lo": 65},
 }
@@ -180,8 +182,8 @@ def valu_issue(kernel, kernel_ms, cus):
                         "clock_ghz": CLOCK_GHZ, "simds": simds, "issue_ms": round(ms, 5),
                         "frac": round(ms / kernel_ms, 4),
                         "note": "issue time of the kernel's VALU instructions at the probe's rates over its "
-                                "measured time; the rest is dependency and LDS latency that eight waves per "
-                                "SIMD do not hide (fewer, persistent waves that prefetch were slower)"}
+                                "measured time; the rest is dependency, barrier and LDS latency that its waves "
+                                "per SIMD do not hide (fewer, persistent waves that prefetch were slower)"}
     return None
 
 
@@ -310,6 +312,96 @@ def trace_stats(path, log):
                   "trimmed_mean_us": round(sum(trim) / len(trim) / 1e3, 2), "min_us": round(v[0] / 1e3, 2)}
         log("%-40s %5d launches  median %8.2f us  trimmed mean %8.2f us  min %8.2f us"
             % (k, len(v), out[k]["median_us"], out[k]["trimmed_mean_us"], out[k]["min_us"]))
+    return out
+
+
+def _counter_means(path):
+    """{kernel: {counter: mean per dispatch}, kernel: dispatches} from a rocprofv3
+    counter collection (CSV). A decompose kernel launched over one operand and
+    over both (a and b) is keyed apart: `name@1` and `name@2`."""
+    import csv
+    vals, disp = {}, {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if "k_" not in r["Kernel_Name"]:
+                continue
+            k = short_kernel(r["Kernel_Name"])
+            if "batch" in k and not k.startswith(("k_ntt_rows", "k_crt")):
+                # operands per launch: the grid over the one-operand grid at the bench point
+                one = {"k_decompose_cols": N, "k_decompose_mfma": 4 * N, "k_ntt_col": 4096 * 134}
+                unit = next((v for pre, v in one.items() if k.startswith(pre)), None)
+                if unit:
+                    k += "@%d" % max(1, int(r["Grid_Size"]) // unit)
+            vals.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+            disp.setdefault(k, set()).add(r["Dispatch_Id"])
+    return ({k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in vals.items()},
+            {k: len(d) for k, d in disp.items()})
+
+
+# Algorithmic bytes per launch at the bench point (K = 134, L = 64): what each
+# kernel must read and write, for the traffic table's last column.
+def _alg_bytes(kernel):
+    kn, nl = 4 * 134 * N, 4 * N * 64
+    if kernel.startswith(("k_decompose", "k_crt")):
+        return (kn + nl) * (int(kernel.split("@")[1]) if "@" in kernel else 1)
+    if kernel.startswith("k_ntt_rows_fused"):
+        return 3 * kn
+    if kernel.startswith("k_ntt_col"):
+        return 2 * kn * (int(kernel.split("@")[1]) if "@" in kernel else 1)
+    if kernel == "k_copy16":
+        return 2 * (256 << 20)
+    if kernel == "k_sum16":
+        return 256 << 20
+    return None
+
+
+def traffic_table(fetch_csv, write_csv, log):
+    """HBM-side bytes per dispatch from a FETCH_SIZE pass and a WRITE_SIZE pass of
+    `--phases-only --copy-yardstick` (rocprofv3 counter CSVs). The factors come
+    from the yardsticks in the same runs: the copy reads and writes 256 MiB, the
+    read-only pass reads 256 MiB."""
+    fetch, disp = _counter_means(fetch_csv)
+    write, _ = _counter_means(write_csv)
+    f_read = (256 << 10) / fetch["k_sum16"]["FETCH_SIZE"]        # KiB read / counter
+    f_write = (256 << 10) / write["k_copy16"]["WRITE_SIZE"]
+    log("# bytes = (FETCH_SIZE * %.4f + WRITE_SIZE * %.4f) * 1024, mean over the dispatches of each kernel"
+        % (f_read, f_write))
+    log("# %-40s %12s %11s %11s %11s %12s %9s" % ("kernel", "bytes", "fetch_KiB", "write_KiB", "dispatches",
+                                                   "algorithmic", "bytes/alg"))
+    out = {}
+    for k in fetch:
+        fk, wk = fetch[k].get("FETCH_SIZE", 0.0), write.get(k, {}).get("WRITE_SIZE", 0.0)
+        b = (fk * f_read + wk * f_write) * 1024
+        alg = _alg_bytes(k)
+        out[k] = {"bytes": round(b), "fetch_KiB": round(fk, 1), "write_KiB": round(wk, 1), "dispatches": disp[k],
+                  "algorithmic": alg}
+        log("%-42s %12d %11.1f %11.1f %11d %12s %9s" % (k, b, fk, wk, disp[k], alg if alg else "-",
+                                                       "%.3f" % (b / alg) if alg else "-"))
+    return out
+
+
+SQ_COLUMNS = ["SQ_INSTS_VALU", "SQ_INSTS_MFMA", "SQ_INSTS_LDS", "SQ_INSTS_SALU", "SQ_INSTS_SMEM",
+              "SQ_INSTS_VMEM_RD", "SQ_INSTS_VMEM_WR", "SQ_LDS_BANK_CONFLICT", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES",
+              "SQ_WAIT_INST_ANY", "SQ_WAIT_ANY", "SQ_ACTIVE_INST_LDS", "SQ_VALU_MFMA_BUSY_CYCLES",
+              "SQ_ACTIVE_INST_VALU"]
+
+
+def sq_table(paths, log):
+    """SQ counters per wave (counter mean per dispatch / SQ_WAVES) from one or
+    more rocprofv3 counter passes over the same run (CSV)."""
+    merged = {}
+    for path in paths:
+        for k, cs in _counter_means(path)[0].items():
+            merged.setdefault(k, {}).update(cs)
+    log("# %-38s %7s " % ("kernel", "waves") + " ".join("%12s" % c[3:15].lower() for c in SQ_COLUMNS))
+    out = {}
+    for k, cs in merged.items():
+        w = cs.get("SQ_WAVES")
+        if not w:
+            continue
+        out[k] = {"waves": round(w), **{c: round(cs[c] / w, 1) for c in SQ_COLUMNS if c in cs}}
+        log("%-40s %7d " % (k, round(w)) + " ".join("%12s" % ("%.1f" % (cs[c] / w) if c in cs else "-")
+                                                     for c in SQ_COLUMNS))
     return out
 
 
@@ -533,6 +625,11 @@ def main():
     ap.add_argument("--trace-shares", metavar="CSV",
                     help="read a rocprofv3 kernel trace (CSV) of a --graph-only run and print how the wall time "
                          "of its last replay divides among the kernels running at each instant")
+    ap.add_argument("--traffic-table", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="HBM bytes per dispatch from a FETCH_SIZE and a WRITE_SIZE rocprofv3 pass of "
+                         "--phases-only --copy-yardstick, scaled by the yardsticks")
+    ap.add_argument("--sq-table", nargs="+", metavar="CSV",
+                    help="SQ counters per wave from rocprofv3 counter passes of one workload")
     ap.add_argument("--issue-probe", action="store_true",
                     help="only the issue-rate probes (cycles per instruction kind on each SIMD), as JSON")
     args = ap.parse_args()
@@ -554,6 +651,12 @@ def main():
         return
     if args.trace_shares:
         print(json.dumps(trace_shares(args.trace_shares, args.steps + args.warmup + 1, log)), flush=True)
+        return
+    if args.traffic_table:
+        print(json.dumps(traffic_table(*args.traffic_table, log)), flush=True)
+        return
+    if args.sq_table:
+        print(json.dumps(sq_table(args.sq_table, log)), flush=True)
         return
 
     if args.copy_yardstick:

@@ -46,14 +46,57 @@ def test_golden_digests_cover_the_batch():
     assert all(bench.SEED0 + i in gold for i in range(bench.BATCH))
 
 
-def test_kernel_names_match_the_committed_profiles():
-    plan = SimpleNamespace(twiddles=1, K=134, fused_cols=0)
+@pytest.mark.parametrize("fused,rows", [(1, "k_ntt_rows_fused_mid_batch<true,1>"),
+                                        (0, "k_ntt_rows_fused_reg<8,true,1>")])
+def test_kernel_names_match_the_committed_profiles(fused, rows):
+    plan = SimpleNamespace(twiddles=1, K=134, fused_cols=fused)
     names = bench.phase_kernels(plan)
-    assert names["ntt_rows_fused"] == "k_ntt_rows_fused_reg<8,true,1>"
+    assert names["ntt_rows_fused"] == rows
+    assert ("ntt_cols_fwd_a" in names) == (fused == 0)
     for phase, kernel in names.items():
         assert bench.measured_traffic(kernel) is not None, (phase, kernel)
         if phase.startswith("ntt"):
             assert bench.valu_issue(kernel, 0.05, 256) is not None, (phase, kernel)
+
+
+def _counters(path, rows):
+    with open(path, "w") as f:
+        f.write("Dispatch_Id,Grid_Size,Kernel_Name,Counter_Name,Counter_Value\n")
+        for d, grid, name, counter, value in rows:
+            f.write('%d,%d,"%s",%s,%r\n' % (d, grid, name, counter, value))
+
+
+def test_traffic_table_scales_by_the_yardsticks(tmp_path):
+    """The read and write factors come from the yardsticks (here 2 and 1), a
+    decompose launch over both operands is keyed apart from one over one."""
+    n = bench.N
+    fetch = [(1, 1, "k_copy16(x)", "FETCH_SIZE", 131072.0), (2, 1, "k_sum16(x)", "FETCH_SIZE", 131072.0),
+             (3, n, "void pm::k_decompose_cols_batch<4, 1>(P)", "FETCH_SIZE", 100.0),
+             (4, 2 * n, "void pm::k_decompose_cols_batch<4, 1>(P)", "FETCH_SIZE", 300.0),
+             (5, 1, "void pm::k_crt_mfma<9>(P)", "FETCH_SIZE", 50.0)]
+    write = [(1, 1, "k_copy16(x)", "WRITE_SIZE", 262144.0), (2, 1, "k_sum16(x)", "WRITE_SIZE", 1.0),
+             (3, n, "void pm::k_decompose_cols_batch<4, 1>(P)", "WRITE_SIZE", 10.0),
+             (4, 2 * n, "void pm::k_decompose_cols_batch<4, 1>(P)", "WRITE_SIZE", 20.0),
+             (5, 1, "void pm::k_crt_mfma<9>(P)", "WRITE_SIZE", 5.0)]
+    fp, wp = tmp_path / "f.csv", tmp_path / "w.csv"
+    _counters(fp, fetch)
+    _counters(wp, write)
+    out = bench.traffic_table(str(fp), str(wp), lambda *a: None)
+    assert out["k_decompose_cols_batch<4,1>@1"]["bytes"] == (2 * 100 + 10) * 1024
+    assert out["k_decompose_cols_batch<4,1>@2"]["bytes"] == (2 * 300 + 20) * 1024
+    assert out["k_decompose_cols_batch<4,1>@2"]["algorithmic"] == 2 * out["k_decompose_cols_batch<4,1>@1"]["algorithmic"]
+    assert out["k_crt_mfma<9>"]["bytes"] == (2 * 50 + 5) * 1024
+
+
+def test_sq_table_per_wave(tmp_path):
+    rows = [(1, 1, "void pm::k_crt_mfma<9>(P)", "SQ_WAVES", 2048.0),
+            (1, 1, "void pm::k_crt_mfma<9>(P)", "SQ_INSTS_VALU", 2048.0 * 2522),
+            (2, 1, "void pm::k_crt_mfma<9>(P)", "SQ_WAVES", 2048.0),
+            (2, 1, "void pm::k_crt_mfma<9>(P)", "SQ_INSTS_VALU", 2048.0 * 2524)]
+    path = tmp_path / "sq.csv"
+    _counters(path, rows)
+    out = bench.sq_table([str(path)], lambda *a: None)
+    assert out["k_crt_mfma<9>"]["waves"] == 2048 and out["k_crt_mfma<9>"]["SQ_INSTS_VALU"] == 2523.0
 
 
 def test_roofline_helpers_read_the_profiles():
