@@ -755,6 +755,11 @@ def main():
                 "kernel_name": kname, "kernel_ms": round(phases[dom], 5),
                 "algorithmic_bytes": nbytes[dom]}
     prod_bytes = 12 * N * L + 56 * K * N
+    # the bytes a product of the measured schedule must move: the two decompositions
+    # 8 N L + 8 K N and the CRT 4 N L + 4 K N, the fused row pass 12 K N and the
+    # inverse column pass 8 K N; the reference's schedule adds two forward column
+    # passes, 16 K N
+    sched_bytes = 12 * N * L + (32 if plans[0].fused_cols == 1 else 48) * K * N
     cpu = None if args.no_cpu else cpu_baseline(points[:1])
     name, cus = polymult.device_info()
     roofline["valu_issue"] = valu_issue(kname, phases[dom], cus)
@@ -776,6 +781,13 @@ def main():
         "uneven_batch": uneven,
         "reference_points": ref_points,
         "roofline": roofline,
+        # the whole product of the measured batch against the same roofline: the
+        # schedule's algorithmic bytes per product over the batch's time per product
+        "roofline_product": None if value is None else {
+            "algorithmic_bytes": sched_bytes, "us_per_product": round(1e6 / value, 2),
+            "achieved_GBps": round(sched_bytes * value / 1e9, 1),
+            "frac": round(sched_bytes * value / 1e9 / HBM_PEAK_GBS, 4),
+            "schedule": "fused column stages" if plans[0].fused_cols == 1 else "the reference's passes"},
         # every phase against the same roofline (the CRT and the decompositions take a larger
         # share of the batch graph's wall time than the fused row pass, which is slowest alone)
         "roofline_by_phase": {k: {"algorithmic_bytes": nbytes[k], "ms": round(v, 5),

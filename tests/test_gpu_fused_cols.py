@@ -141,3 +141,25 @@ def test_fused_cols_environment(golden, monkeypatch):
     with pytest.raises(polymult.PolymulError) as e:
         polymult.Plan(case["N"], case["W"], q)
     assert e.value.code == polymult.PM_EINVAL
+
+
+@pytest.mark.parametrize("W", [2, 31, 33, 64, 65, 100, 127, 129, 255, 511, 513, 1023, 1025])
+def test_fused_cols_small_limb_counts_match_oracle(W):
+    """N = 65536 with 1 .. 33 limbs (one to three decompose K-steps, L not a
+    multiple of 4, K from 18 up): the device path with the fused stages against
+    the oracle."""
+    N = 65536
+    q, a, b = oracle.gen_point(97000 + W, N, W)
+    L = a.shape[1]
+    say("fused cols small L", W, "L", L)
+    ref = oracle.Plan(N, q).multiply(a, b)
+    with polymult.Plan(N, W, q, form=polymult.FORM_MFMA) as plan:
+        assert plan.fused_cols == 1
+        da, db = polymult.DeviceBuffer.from_array(a), polymult.DeviceBuffer.from_array(b)
+        dc = polymult.DeviceBuffer(a.nbytes)
+        plan.multiply_device(da, db, dc)
+        polymult.synchronize()
+        got = dc.download((N, L))
+        for x in (da, db, dc):
+            x.close()
+    assert np.array_equal(got, ref), (W, L)
