@@ -117,7 +117,7 @@ def phase_kernels(plan):
     col = "k_ntt_col_reg<8,%s,5," + lazy + ">"
     if plan.fused_cols == 1:
         # the first four column stages inside decompose, the next four in the row pass
-        return {"decompose_a": "k_decompose_cols_batch<4,1>@1", "decompose_b": "k_decompose_cols_batch<4,1>@1",
+        return {"decompose_a": "k_decompose_cols_batch<16,4,1>@1", "decompose_b": "k_decompose_cols_batch<16,4,1>@1",
                 "ntt_rows_fused": "k_ntt_rows_fused_mid_batch<%s,%s>" % (gen, lazy),
                 "ntt_cols_inv": col % "false", "crt": "k_crt_mfma<%d>" % ks}
     return {"decompose_a": "k_decompose_mfma<4,true>", "decompose_b": "k_decompose_mfma<4,true>",

@@ -972,7 +972,7 @@ int pm_plan_set_fused_cols(pm_plan* plan, int on) {
     return guarded([&] {
         need(plan != nullptr, "plan is null");
         need(on == 0 || on == 1, "fused_cols must be 0 or 1");
-        need(on == 0 || pm::fused_cols_ok(plan->dev), "fused column stages need N = 65536 and the matrix-core tables");
+        need(on == 0 || pm::fused_cols_ok(plan->dev), "fused column stages need N = 32768 or 65536 and the matrix-core tables");
         plan->dev.fused_cols = on;
     });
 }

@@ -42,11 +42,11 @@ struct DevPlan {
     const uint8_t* colm_fwd;
     const uint8_t* colm_inv;
     int col_mfma;             // 1: the column passes run on the matrix cores (tables present)
-    // 1: a product's forward NTT runs its first four column stages (16-point
-    // columns of stride N/16) inside the matrix-core decompose and the next
+    // 1: a product's forward NTT runs its first column stages (R/16-point
+    // columns of stride 16 N/R) inside the matrix-core decompose and the next
     // four (those within a block of 16 rows) at the start of the fused row
     // pass (launch_decompose_cols_batch, launch_ntt_rows_fused_mid_batch):
-    // N = 65536 (R = S = 256), matrix-core form. Read only by the product
+    // N = 65536 or 32768 (S = 256), matrix-core form. Read only by the product
     // paths that launch those kernels; every other launcher ignores it.
     int fused_cols;
     // Full tables, null unless the row pass reads its twiddles from tables or

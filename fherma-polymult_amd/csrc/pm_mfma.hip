@@ -233,44 +233,64 @@ __global__ void __launch_bounds__(64 * kWaves) k_decompose_mfma_batch(DevPlan P,
     decompose_tile<KT, kLazyOut>(second ? ba.b[i] : ba.a[i], second ? ba.B[i] : ba.A[i], Q, 0, P.N);
 }
 
-// Decompose fused with the first four forward column stages of N = 65536
-// (DevPlan::fused_cols): coefficient n = s + N1 r with N1 = N / 16, r < 16;
-// the stages of distance N/2 .. N/16 are 16-point transforms over r for each
-// inner position s, with twiddle index (16 + r) >> (e + 1) for the element r
-// (phase B's form for t = 0, B = 16: the first 16 pairs of the prime's
-// forward table). A workgroup of eight waves takes 32 consecutive s and all
-// 16 r (512 coefficients): wave w holds the B fragments of the four tiles
-// r = 4 (w & 3) .. + 3 at s = s0 + 16 (w >> 2) + [0, 16) in registers (loaded
-// straight from the limbs, no LDS) and loops over the prime groups
-// [g_begin, g_end). After each group the residues go to an LDS exchange buffer
-// [prime of the batch][r][s]; after every four groups (16 primes) one barrier,
-// and each thread runs one (prime, s) column through the four stages in
-// registers and stores its 16 residues (two primes x 32 s per wave: 128-byte
-// segments). Two exchange buffers alternate, so one barrier per batch keeps
-// every read of a buffer ahead of its next write. The decompose's [0, 2p)
-// residues feed ct_half as they are (kMode 1).
-constexpr int kFcWaves = 8;
-constexpr int kFcCols = 32;                    // inner positions per workgroup
-constexpr int kFcStride = 16 * kFcCols + 16;   // exchange words per prime; 16 mod 64 banks apart
+// Decompose fused with the first forward column stages (DevPlan::fused_cols;
+// N = 65536 or 32768, R1 = R / 16 = 16 or 8): coefficient n = s + N1 r with
+// N1 = N / R1, r < R1; the stages of distance N/2 .. N1 are R1-point transforms
+// over r for each inner position s, with twiddle index (R1 + r) >> (e + 1) for
+// the element r of the stage of distance N1 2^e (the column pass's form with
+// R1 rows: the first R1 pairs of the prime's forward table). A workgroup of
+// R1 / 2 waves takes 32 consecutive s and all R1 r (32 R1 coefficients): wave
+// w holds the B fragments of the four tiles r = 4 (w mod R1/4) .. + 3 at
+// s = s0 + 16 (w div R1/4) + [0, 16) in registers (loaded straight from the
+// limbs, no LDS) and loops over the prime groups [g_begin, g_end). After each
+// group the residues go to an LDS exchange buffer [prime of the batch][r][s];
+// after every four groups (16 primes) one barrier, and the threads run the
+// 16 x 32 (prime, s) columns through the stages in registers (one column per
+// thread for R1 = 16, two for R1 = 8) and store them, two primes x 32 s per
+// wave and step (128-byte segments). Two exchange buffers alternate, so one
+// barrier per batch keeps every read of a buffer ahead of its next write. The
+// decompose's [0, 2p) residues feed ct_half as they are (kMode 1).
+constexpr int kFcCols = 32;   // inner positions per workgroup
+
+template <int R1>
+struct FcShape {
+    static constexpr int kWaves = R1 / 2;                 // four tiles per wave
+    static constexpr int kRq = R1 / 4;                    // waves per 16-s half
+    static constexpr int kStride = R1 * kFcCols + 16;     // exchange words per prime: 16 mod 64 banks apart
+    static constexpr int kSteps = 512 / (64 * kWaves);    // transform columns per thread
+};
 
 // LDS: two exchange buffers, two sets of the batch's 16 x 16 twiddle pairs,
 // the per-prime constants
+template <int R1>
 __host__ __device__ constexpr uint32_t fc_lds_bytes(int K) {
-    return 4u * 2u * 16u * kFcStride + 2u * 16u * 16u * 8u + 16u * static_cast<uint32_t>(K);
+    return 4u * 2u * 16u * FcShape<R1>::kStride + 2u * 16u * 16u * 8u + 16u * static_cast<uint32_t>(K);
 }
 
-struct OuterTw {   // the first 16 pairs of the prime's forward table (Shoup pairs), staged in LDS
-    static constexpr bool kMont = false;
-    const uint2* tab;
-    __device__ __forceinline__ uint2 b(int e, uint32_t, int blk) const { return tab[(16u >> (e + 1)) + blk]; }
-};
+// The R1-point forward stages of one column, twiddles from the prime's first
+// pairs (Shoup pairs) staged in LDS.
+template <int R1, int kMode>
+__device__ __forceinline__ void outer_stages(uint32_t (&x)[R1], const uint2* tw, uint32_t p) {
+    constexpr int LOG = R1 == 16 ? 4 : 3;
+#pragma unroll
+    for (int e = LOG - 1; e >= 0; --e) {
+        const int d = 1 << e;
+#pragma unroll
+        for (int blk = 0; blk < R1 / (2 * d); ++blk) {
+            const uint2 w = tw[(R1 >> (e + 1)) + blk];
+#pragma unroll
+            for (int k = blk * 2 * d; k < blk * 2 * d + d; ++k) bfly<true, kMode, false>(x[k], x[k + d], w, p);
+        }
+    }
+}
 
-template <int KT, int kMode>
+template <int R1, int KT, int kMode>
 __device__ __forceinline__ void decompose_cols_block(const uint32_t* __restrict__ limbs, uint32_t* __restrict__ res,
                                                      const DevPlan& P, int g_begin, int g_end) {
+    using F = FcShape<R1>;
     extern __shared__ uint32_t xs[];
     const int K = P.K, L = P.L;
-    uint2* twl = reinterpret_cast<uint2*>(xs + 2 * 16 * kFcStride);   // [2][16 primes][16 pairs]
+    uint2* twl = reinterpret_cast<uint2*>(xs + 2 * 16 * F::kStride);   // [2][16 primes][16 pairs]
     int2* kinit = reinterpret_cast<int2*>(twl + 2 * 16 * 16);
     uint32_t* kp = reinterpret_cast<uint32_t*>(kinit + K);
     uint32_t* kpneg = kp + K;
@@ -279,11 +299,11 @@ __device__ __forceinline__ void decompose_cols_block(const uint32_t* __restrict_
         kp[j] = P.pc[j].p;
         kpneg[j] = P.pc[j].pneg;
     }
-    const uint32_t N1 = P.N >> 4;
+    const uint32_t N1 = P.N / R1;
     const uint32_t s0 = blockIdx.x * kFcCols;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, cl = lane & 15;
-    const int sh = wave >> 2, rq = wave & 3;
+    const int sh = wave / F::kRq, rq = wave % F::kRq;
     // B fragments: tile t is r = 4 rq + t; lane (g, cl) holds words 16 ks + 4 g
     // .. + 3 of coefficient s0 + 16 sh + cl + N1 r as int8 digits u ^ 0x80,
     // words past L as 0 (digit 0x80)
@@ -317,15 +337,10 @@ __device__ __forceinline__ void decompose_cols_block(const uint32_t* __restrict_
     for (int ks = 0; ks < KT; ++ks) an[ks] = dig[(min(g_begin, groups - 1) * KT + ks) * 64 + lane];
     uint32_t k16 = 1u << 16;
     asm volatile("" : "+s"(k16));
-    // the transform's thread: primes of a batch in pairs 2 apart per wave (the
-    // pair's exchange rows then sit 32 banks apart), 32 s per prime
-    const int tw_ = threadIdx.x >> 6, th = (threadIdx.x >> 5) & 1;
-    const int tpl = 4 * (tw_ >> 1) + (tw_ & 1) + 2 * th;
-    const uint32_t tsl = threadIdx.x & 31;
     __syncthreads();   // the per-prime constants
     int buf = 0;
     for (int jb = g_begin; jb < g_end; jb += 4, buf ^= 1) {
-        uint32_t* xe = xs + buf * 16 * kFcStride;
+        uint32_t* xe = xs + buf * 16 * F::kStride;
         uint2* twb = twl + buf * 256;
         // the batch's twiddles: thread i < 256 copies pair i % 16 of prime 4 jb + i / 16
         // (a prime past K copies the last prime's, never read)
@@ -360,20 +375,28 @@ __device__ __forceinline__ void decompose_cols_block(const uint32_t* __restrict_
                 const uint64_t Rw = static_cast<uint64_t>(hi) * k16 + static_cast<uint64_t>(static_cast<int64_t>(lo));
                 const uint32_t m = static_cast<uint32_t>(Rw) * pneg;
                 const uint32_t v = static_cast<uint32_t>((static_cast<uint64_t>(m) * p + Rw) >> 32);
-                xe[(4 * q + g) * kFcStride + (4 * rq + t) * kFcCols + 16 * sh + cl] = kMode == 1 ? v : min(v, v - p);
+                xe[(4 * q + g) * F::kStride + (4 * rq + t) * kFcCols + 16 * sh + cl] = kMode == 1 ? v : min(v, v - p);
             }
         }
         __syncthreads();
-        const int j = 4 * jb + tpl;
-        if (j < 4 * g_end && j < K) {
-            uint32_t x[16];
+        // the columns: step it takes column it (64 kWaves) + thread; a wave's two
+        // primes sit 2 apart (their exchange rows then 32 banks apart), 32 s each
 #pragma unroll
-            for (int r = 0; r < 16; ++r) x[r] = xe[tpl * kFcStride + r * kFcCols + tsl];
-            const OuterTw ot{twb + 16 * tpl};
-            phase_b<true, false, kMode>(x, x, ot, 0, kp[j]);
-            uint32_t* out = res + static_cast<size_t>(j) * P.N + s0 + tsl;
+        for (int it = 0; it < F::kSteps; ++it) {
+            const int idx = it * 64 * F::kWaves + static_cast<int>(threadIdx.x);
+            const int tw_ = idx >> 6, th = (idx >> 5) & 1;
+            const int tpl = 4 * (tw_ >> 1) + (tw_ & 1) + 2 * th;
+            const uint32_t tsl = idx & 31;
+            const int j = 4 * jb + tpl;
+            if (j < 4 * g_end && j < K) {
+                uint32_t x[R1];
 #pragma unroll
-            for (int r = 0; r < 16; ++r) out[N1 * r] = x[r];
+                for (int r = 0; r < R1; ++r) x[r] = xe[tpl * F::kStride + r * kFcCols + tsl];
+                outer_stages<R1, kMode>(x, twb + 16 * tpl, kp[j]);
+                uint32_t* out = res + static_cast<size_t>(j) * P.N + s0 + tsl;
+#pragma unroll
+                for (int r = 0; r < R1; ++r) out[N1 * r] = x[r];
+            }
         }
     }
 }
@@ -381,8 +404,8 @@ __device__ __forceinline__ void decompose_cols_block(const uint32_t* __restrict_
 // A batch: blockIdx.z = 2 (product) + (0: a into A, 1: b into B), or, for
 // side 0 / 1, blockIdx.z = product and that side only; blockIdx.y = the range
 // of prime groups [y gps, (y + 1) gps) (gps a multiple of 4).
-template <int KT, int kMode>
-__global__ void __launch_bounds__(64 * kFcWaves, 4) k_decompose_cols_batch(DevPlan P, BatchArgs ba, int side, int gps) {
+template <int R1, int KT, int kMode>
+__global__ void __launch_bounds__(32 * R1, 4) k_decompose_cols_batch(DevPlan P, BatchArgs ba, int side, int gps) {
     const bool both = side < 0;
     const int i = both ? blockIdx.z >> 1 : blockIdx.z;
     const bool second = both ? (blockIdx.z & 1) != 0 : side == 1;
@@ -395,8 +418,8 @@ __global__ void __launch_bounds__(64 * kFcWaves, 4) k_decompose_cols_batch(DevPl
     Q.L = d->L;
     Q.dec_digits = second ? d->dec_digits_b : d->dec_digits;
     Q.dec_init = second ? d->dec_init_b : d->dec_init;
-    decompose_cols_block<KT, kMode>(second ? ba.b[i] : ba.a[i], second ? ba.B[i] : ba.A[i], Q, g0,
-                                    min(groups, g0 + gps));
+    decompose_cols_block<R1, KT, kMode>(second ? ba.b[i] : ba.a[i], second ? ba.B[i] : ba.A[i], Q, g0,
+                                        min(groups, g0 + gps));
 }
 
 // CRT, one workgroup (four waves) per 64 coefficients, in one kernel:
@@ -1109,31 +1132,38 @@ void launch_decompose_mfma_batch(const DevPlan& P, const BatchArgs& ba, int coun
 
 namespace {
 
-template <int KT>
+template <int R1, int KT>
 void fc_launch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st, int side) {
     // every product of the launch takes its own K; the grid covers the longest
     const int groups = (P.K + 3) / 4;
-    const int splits = 1;
-    const int gps = 4 * ((groups + 4 * splits - 1) / (4 * splits));
-    const dim3 grid((P.N >> 4) / kFcCols, (groups + gps - 1) / gps, side < 0 ? 2 * count : count);
-    const size_t lds = fc_lds_bytes(P.K);
+    const int gps = 4 * ((groups + 3) / 4);
+    const dim3 grid((P.N / R1) / kFcCols, (groups + gps - 1) / gps, side < 0 ? 2 * count : count);
+    const size_t lds = fc_lds_bytes<R1>(P.K);
     if (P.lazy)
-        launch(k_decompose_cols_batch<KT, 1>, grid, dim3(64 * kFcWaves), lds, st, P, ba, side, gps);
+        launch(k_decompose_cols_batch<R1, KT, 1>, grid, dim3(32 * R1), lds, st, P, ba, side, gps);
     else
-        launch(k_decompose_cols_batch<KT, 0>, grid, dim3(64 * kFcWaves), lds, st, P, ba, side, gps);
+        launch(k_decompose_cols_batch<R1, KT, 0>, grid, dim3(32 * R1), lds, st, P, ba, side, gps);
+}
+
+template <int R1>
+void fc_dispatch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st, int side) {
+    switch (P.dec_ksteps) {
+        case 1: fc_launch<R1, 1>(P, ba, count, st, side); return;
+        case 2: fc_launch<R1, 2>(P, ba, count, st, side); return;
+        case 3: fc_launch<R1, 3>(P, ba, count, st, side); return;
+        case 4: fc_launch<R1, 4>(P, ba, count, st, side); return;
+        default: throw std::runtime_error("L out of range");
+    }
 }
 
 }  // namespace
 
 void launch_decompose_cols_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st, int side) {
-    if (!fused_cols_ok(P)) throw std::logic_error("fused column stages need N = 65536 and the matrix-core tables");
-    switch (P.dec_ksteps) {
-        case 1: fc_launch<1>(P, ba, count, st, side); return;
-        case 2: fc_launch<2>(P, ba, count, st, side); return;
-        case 3: fc_launch<3>(P, ba, count, st, side); return;
-        case 4: fc_launch<4>(P, ba, count, st, side); return;
-        default: throw std::runtime_error("L out of range");
-    }
+    if (!fused_cols_ok(P)) throw std::logic_error("fused column stages need N = 32768 or 65536 and the matrix-core tables");
+    if (P.logR == 8)
+        fc_dispatch<16>(P, ba, count, st, side);
+    else
+        fc_dispatch<8>(P, ba, count, st, side);
 }
 
 void launch_crt_mfma_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st) {

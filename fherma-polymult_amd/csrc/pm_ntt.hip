@@ -408,10 +408,10 @@ struct MidTw {
     }
 };
 
-// Row pass fused, for products whose decompose ran the first four column
-// stages (DevPlan::fused_cols, N = 65536, R = S = 256): the workgroup's 16
-// rows are the block rho0 .. rho0 + 15, and the four column stages left
-// (distance S .. 8 S) mix them. Thread c first holds column c of the block
+// Row pass fused, for products whose decompose ran the first column stages
+// (DevPlan::fused_cols, N = 65536 or 32768, S = 256): the workgroup's 16 rows
+// are the block rho0 .. rho0 + 15, and the four column stages left (distance
+// S .. 8 S) mix them. Thread c first holds column c of the block
 // (element c + S i in register i) for A and for B, runs those stages on both
 // with shared twiddles, and passes them through the rows' exchange buffer into
 // the phase-A layout of row_item (one workgroup barrier after each write, and
@@ -654,13 +654,15 @@ void launch_ntt_cols_batch(const DevPlan& P, const BatchArgs& ba, int count, boo
     }
 }
 
+// N = 65536 (R = 256: 16 outer rows) or 32768 (R = 128: 8), rows of S = 256.
 bool fused_cols_ok(const DevPlan& P) {
-    return P.logN == 16 && P.logR == 8 && P.logS == 8 && P.dec_digits && P.ctw && (P.row_gen || (P.tw && P.itw));
+    return (P.logN == 16 || P.logN == 15) && P.logS == 8 && P.logR == P.logN - 8 && P.dec_digits && P.ctw &&
+           (P.row_gen || (P.tw && P.itw));
 }
 
 void launch_ntt_rows_fused_mid_batch(const DevPlan& P, const BatchArgs& ba, int count, hipStream_t st) {
-    if (!fused_cols_ok(P)) throw std::logic_error("fused column stages need N = 65536 and the matrix-core tables");
-    const dim3 grid(16, P.K, count);   // 256 rows in blocks of 16
+    if (!fused_cols_ok(P)) throw std::logic_error("fused column stages need N = 32768 or 65536 and the matrix-core tables");
+    const dim3 grid(1u << (P.logR - 4), P.K, count);   // R rows in blocks of 16
     const size_t lds = sizeof(uint32_t) * (16 * (256 + 16)) + sizeof(uint2) * 16 * 16;
     switch (P.lazy * 2 + (P.row_gen ? 1 : 0)) {
         case 3: launch((k_ntt_rows_fused_mid_batch<true, 1>), grid, dim3(256), lds, st, P, ba); break;

@@ -93,12 +93,13 @@ typedef enum {
 int pm_plan_set_ntt_form(pm_plan* plan, int form);
 int pm_plan_get_ntt_form(const pm_plan* plan);   /* pm_ntt_form, or -1 for a null plan */
 
-/* Fused column stages (N = 65536, matrix-core form): 1 runs the product's first
- * four forward column stages (16-point columns of stride N/16) inside the
- * decompose kernel and the next four (those within a block of 16 rows) at the
- * start of the fused row pass, so the forward column passes, and their write
- * and re-read of both K x N residue arrays, are gone; 0 runs the passes of the
- * reference's schedule (solve.cu:128-233). The NTT-domain values are the same.
+/* Fused column stages (N = 65536 or 32768, matrix-core form): 1 runs the
+ * product's first forward column stages (four or three: R/16-point columns of
+ * stride 16 N/R) inside the decompose kernel and the next four (those within a
+ * block of 16 rows) at the start of the fused row pass, so the forward column
+ * passes, and their write and re-read of both K x N residue arrays, are gone; 0
+ * runs the passes of the reference's schedule (solve.cu:128-233). The
+ * NTT-domain values are the same.
  * New plans take 1 where the geometry allows unless PM_FUSED_COLS=0 (1 or auto:
  * the default; any other value makes plan creation fail with PM_EINVAL). Turning
  * it on where the geometry does not allow it fails with PM_EINVAL. The stage

@@ -1,5 +1,6 @@
 """Products whose first forward column stages run inside the decompose kernel
-and the next ones inside the row pass (fused column stages, N = 65536), on the
+and the next ones inside the row pass (fused column stages, N = 65536 and
+32768), on the
 GPU: one product at a time on the device path, the allocation-free path,
 batched launches over products of different K and L, and the captured batch,
 each product against its GMP digest and against the same plan with the fused
@@ -25,6 +26,12 @@ CASES = ["config-n65536-w2048-00", "config-n65536-w1747-s1", "matrix-n65536-w174
          "pattern-alternating-n65536-w2048", "pattern-identity-n65536-w2048", "config-uneven-37",
          "config-uneven-59", "config-uneven-35", "config-uneven-28", "config-uneven-24", "config-uneven-45",
          "config-uneven-40", "config-uneven-11", "config-uneven-15"]
+# N = 32768 (three outer stages): the reference's second point (868-bit prime q),
+# random q, all-(q - 1) operands, the patterns and the uneven workload's products.
+CASES_32K = ["config-n32768-w868-s1", "matrix-n32768-w868-prime", "matrix-n32768-w868-prime-allqm1",
+             "pattern-all_q_minus_1-n32768-w868", "pattern-wrap-n32768-w868", "pattern-alternating-n32768-w868",
+             "config-uneven-02", "config-uneven-03", "config-uneven-19", "config-uneven-23", "config-uneven-46",
+             "config-uneven-47", "config-uneven-54", "config-uneven-55", "config-uneven-56"]
 
 
 def say(*args):
@@ -32,7 +39,7 @@ def say(*args):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", CASES + CASES_32K)
 def test_fused_cols_device_product_matches_gmp(golden, name):
     case = golden[name]
     q, a, b = pointgen.operands(case)
@@ -62,7 +69,8 @@ def test_fused_cols_device_product_matches_gmp(golden, name):
 def test_fused_cols_batched_launches_match_gmp(golden):
     """Products of uneven K and L at N = 65536 in batched launches (the decompose
     launched once per K-step count), fused stages on, then off."""
-    names = [n for n in CASES if n.startswith("config-uneven-")] + ["config-n65536-w2048-01"]
+    names = [n for n in CASES + CASES_32K if n.startswith("config-uneven-")] + ["config-n65536-w2048-01",
+                                                                              "config-n32768-w868-s2"]
     cases = [golden[n] for n in names]
     plans, bufs = [], []
     for case in cases:
@@ -117,8 +125,8 @@ def test_fused_cols_graph_replays_match_gmp(golden):
         p.close()
 
 
-def test_fused_cols_needs_n65536(golden):
-    case = golden["config-n32768-w868-s1"]
+def test_fused_cols_needs_n32768_or_65536(golden):
+    case = golden["config-uneven-04"]   # N = 16384
     q, _, _ = pointgen.operands(case)
     with polymult.Plan(case["N"], case["W"], q) as plan:
         assert plan.fused_cols == 0
@@ -143,15 +151,14 @@ def test_fused_cols_environment(golden, monkeypatch):
     assert e.value.code == polymult.PM_EINVAL
 
 
+@pytest.mark.parametrize("N", [65536, 32768])
 @pytest.mark.parametrize("W", [2, 31, 33, 64, 65, 100, 127, 129, 255, 511, 513, 1023, 1025])
-def test_fused_cols_small_limb_counts_match_oracle(W):
-    """N = 65536 with 1 .. 33 limbs (one to three decompose K-steps, L not a
-    multiple of 4, K from 18 up): the device path with the fused stages against
-    the oracle."""
-    N = 65536
-    q, a, b = oracle.gen_point(97000 + W, N, W)
+def test_fused_cols_small_limb_counts_match_oracle(N, W):
+    """1 .. 33 limbs (one to three decompose K-steps, L not a multiple of 4, K
+    from 18 up): the device path with the fused stages against the oracle."""
+    q, a, b = oracle.gen_point(97000 + W + N, N, W)
     L = a.shape[1]
-    say("fused cols small L", W, "L", L)
+    say("fused cols small L", N, W, "L", L)
     ref = oracle.Plan(N, q).multiply(a, b)
     with polymult.Plan(N, W, q, form=polymult.FORM_MFMA) as plan:
         assert plan.fused_cols == 1

@@ -161,23 +161,38 @@ def test_register_schedule_matches_oracle_ntt(N):
         assert emulate(fwd[j], tw, itw, N, p, False, psi) == inv[j].tolist()
 
 
+def outer_network(col, tw, p):
+    """decompose_cols_block's outer stages on one column of R1 values: the stage
+    of distance 2^e in r takes twiddle (R1 >> (e + 1)) + blk for its block blk
+    (the column pass's form with R1 rows)."""
+    R1 = len(col)
+    for e in reversed(range(R1.bit_length() - 1)):
+        d = 1 << e
+        for blk in range(R1 // (2 * d)):
+            w = int(tw[(R1 >> (e + 1)) + blk])
+            for k in range(blk * 2 * d, blk * 2 * d + d):
+                col[k], col[k + d] = ct(col[k], col[k + d], w, p)
+
+
 def fused_cols_forward(x, tw, N, p):
-    """The forward NTT as a product with fused column stages runs it (N = 2^16,
-    R = S = 256): the decompose kernel's four stages over r (coefficient
-    s + (N/16) r, phase B with t = 0 and B = 16), the row pass's four stages over
+    """The forward NTT as a product with fused column stages runs it (N = 2^16 or
+    2^15, S = 256, R1 = R / 16): the decompose kernel's stages over r
+    (coefficient s + (N/R1) r, outer_network), the row pass's four stages over
     the 16 rows of each block (phase B with t = 0 and B = R + rho0, thread c
     holding column c), the exchange into the rows' phase-A layout, then the row
     pass's own eight stages (read here from the table; the generated twiddles
     are checked against it above)."""
     logN = N.bit_length() - 1
     logS, logR = geometry(logN)
-    S, R, N1 = 1 << logS, 1 << logR, N // 16
+    S, R = 1 << logS, 1 << logR
+    R1 = R // 16
+    N1 = N // R1
     x = list(map(int, x))
-    for s in range(N1):
-        col = [x[s + N1 * r] for r in range(16)]
-        phase_b(col, True, table_tw(tw, 16), 0, p)
-        for r in range(16):
-            x[s + N1 * r] = col[r]
+    for s_ in range(N1):
+        col = [x[s_ + N1 * r] for r in range(R1)]
+        outer_network(col, tw, p)
+        for r in range(R1):
+            x[s_ + N1 * r] = col[r]
     ROW = S + S // 16
     for rho0 in range(0, R, 16):
         lds = [None] * (16 * ROW)
@@ -194,40 +209,57 @@ def fused_cols_forward(x, tw, N, p):
     return x
 
 
-def test_fused_cols_exchange_maps():
-    """decompose_cols_block's exchange: a write at (batch prime 4 q + g, r, s) and
-    the transform thread's reads use one address pl 528 + 32 r + s, and the
-    transform threads cover the 16 primes x 32 s once; the two primes of a
-    wave sit 32 banks apart, the four of a write 16 apart (mod 64)."""
-    stride = 16 * 32 + 16
+def test_outer_network_is_phase_b_for_16_rows():
+    p = 2013265921
+    tw = [pow(31, i * 7 + 3, p) for i in range(16)]
+    a = [pow(5, i, p) for i in range(16)]
+    b = list(a)
+    outer_network(a, tw, p)
+    phase_b(b, True, table_tw(tw, 16), 0, p)
+    assert a == b
+
+
+@pytest.mark.parametrize("R1", [16, 8])
+def test_fused_cols_exchange_maps(R1):
+    """decompose_cols_block's exchange (R1 = 16: eight waves, one column per
+    thread; R1 = 8: four waves, two): a write at (batch prime 4 q + g, r, s)
+    and a column read use one address pl stride + 32 r + s, the writes fill the
+    buffer once, the transform steps cover the 16 primes x 32 s once; the two
+    primes of a wave's step sit 32 banks apart, the four of a write 16 apart
+    (mod 64)."""
+    stride = R1 * 32 + 16
+    waves, rqs = R1 // 2, R1 // 4
     writes = set()
     for q in range(4):
         for g in range(4):
-            for rq in range(4):
+            for w in range(waves):
+                sh, rq = w // rqs, w % rqs
                 for t in range(4):
-                    for sh in range(2):
-                        for cl in range(16):
-                            writes.add((4 * q + g) * stride + (4 * rq + t) * 32 + 16 * sh + cl)
-    assert len(writes) == 16 * 16 * 32 and max(writes) < 16 * stride
-    threads = set()
-    for tid in range(512):
-        w, th = tid >> 6, (tid >> 5) & 1
-        threads.add((4 * (w >> 1) + (w & 1) + 2 * th, tid & 31))
-    assert threads == {(pl, s) for pl in range(16) for s in range(32)}
+                    for cl in range(16):
+                        writes.add((4 * q + g) * stride + (4 * rq + t) * 32 + 16 * sh + cl)
+    assert len(writes) == 16 * R1 * 32 and max(writes) < 16 * stride
+    cols = []
+    steps = 512 // (64 * waves)
+    for it in range(steps):
+        for tid in range(64 * waves):
+            idx = it * 64 * waves + tid
+            w, th = idx >> 6, (idx >> 5) & 1
+            cols.append((4 * (w >> 1) + (w & 1) + 2 * th, idx & 31))
+    assert sorted(cols) == [(pl, s_) for pl in range(16) for s_ in range(32)]
     assert {(g * stride) % 64 for g in range(4)} == {0, 16, 32, 48}
     for w in range(8):
         a, b = (4 * (w >> 1) + (w & 1)) * stride, (4 * (w >> 1) + (w & 1) + 2) * stride
         assert (b - a) % 64 == 32
 
 
-def test_fused_cols_schedule_matches_oracle_ntt():
-    N = 65536
+@pytest.mark.parametrize("N", [65536, 32768])
+def test_fused_cols_schedule_matches_oracle_ntt(N):
     q, a, _ = oracle.gen_point(22, N, 128)
     plan = oracle.Plan(N, q)
     res = plan.decompose(a)
     fwd = plan.ntt_forward(res)
     logN = N.bit_length() - 1
-    brv = np.array([int(format(i, "016b")[::-1], 2) for i in range(N)])
+    brv = np.array([int(format(i, "0%db" % logN)[::-1], 2) for i in range(N)])
     j = plan.K - 1
     p = int(plan.primes[j])
     psi = int(plan.psi[j])
