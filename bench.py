@@ -405,6 +405,32 @@ def sq_table(paths, log):
     return out
 
 
+def clock_table(path, log, xcds=8, min_us=200.0):
+    """The engine clock each kernel ran at, from a rocprofv3 GRBM_COUNT pass
+    (CSV with the dispatches' timestamps): GRBM_COUNT counts engine clocks
+    summed over the device's XCDs, so count / XCDs / duration is the clock.
+    Only dispatches of at least `min_us` count: the counter's window is a few
+    microseconds wider than the dispatch, which skews short ones."""
+    import csv
+    per = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != "GRBM_COUNT" or "k_" not in r["Kernel_Name"]:
+                continue
+            ns = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+            if ns < min_us * 1e3:
+                continue
+            per.setdefault(short_kernel(r["Kernel_Name"]), []).append((ns, float(r["Counter_Value"])))
+    log("# %-48s %10s %12s %9s" % ("kernel", "dispatches", "mean us", "GHz"))
+    out = {}
+    for k, v in per.items():
+        ghz = sum(c / xcds / ns for ns, c in v) / len(v)
+        out[k] = {"dispatches": len(v), "mean_us": round(sum(ns for ns, _ in v) / len(v) / 1e3, 1),
+                  "ghz": round(ghz, 3)}
+        log("%-50s %10d %12.1f %9.3f" % (k, len(v), out[k]["mean_us"], ghz))
+    return out
+
+
 def trace_shares(path, replays, log):
     """The last of `replays` graph replays in a rocprofv3 kernel trace (CSV,
     columns Kernel_Name, Start_Timestamp, End_Timestamp in ns): every instant
@@ -630,6 +656,9 @@ def main():
                          "--phases-only --copy-yardstick, scaled by the yardsticks")
     ap.add_argument("--sq-table", nargs="+", metavar="CSV",
                     help="SQ counters per wave from rocprofv3 counter passes of one workload")
+    ap.add_argument("--clock-table", metavar="CSV",
+                    help="the engine clock per kernel from a GRBM_COUNT rocprofv3 pass (dispatches of 200 us "
+                         "and more)")
     ap.add_argument("--issue-probe", action="store_true",
                     help="only the issue-rate probes (cycles per instruction kind on each SIMD), as JSON")
     args = ap.parse_args()
@@ -657,6 +686,9 @@ def main():
         return
     if args.sq_table:
         print(json.dumps(sq_table(args.sq_table, log)), flush=True)
+        return
+    if args.clock_table:
+        print(json.dumps(clock_table(args.clock_table, log)), flush=True)
         return
 
     if args.copy_yardstick:

@@ -156,3 +156,18 @@ def test_trace_stats_median_and_trim(tmp_path):
     out = bench.trace_stats(str(path), lambda *a: None)["k_c<1>"]
     assert out["launches"] == 20 and out["median_us"] == 1.0 and out["trimmed_mean_us"] == 1.0
     assert out["min_us"] == 0.1
+
+
+def test_clock_table_divides_by_xcds_and_duration(tmp_path):
+    """GRBM_COUNT over 8 XCDs and the dispatch's duration; short dispatches and
+    other counters are left out."""
+    path = tmp_path / "clk.csv"
+    with open(path, "w") as f:
+        f.write("Dispatch_Id,Kernel_Name,Counter_Name,Counter_Value,Start_Timestamp,End_Timestamp\n")
+        f.write('1,"void pm::k_e<2>(int)",GRBM_COUNT,%r,0,500000\n' % (8 * 2.0 * 500000))
+        f.write('2,"void pm::k_e<2>(int)",GRBM_COUNT,%r,0,1000000\n' % (8 * 1.9 * 1000000))
+        f.write('2,"void pm::k_e<2>(int)",SQ_WAVES,64.0,0,1000000\n')
+        f.write('3,"void pm::k_e<2>(int)",GRBM_COUNT,%r,0,100000\n' % (8 * 9.0 * 100000))
+    out = bench.clock_table(str(path), lambda *a: None)
+    assert out["k_e<2>"]["dispatches"] == 2 and abs(out["k_e<2>"]["ghz"] - 1.95) < 1e-9
+    assert out["k_e<2>"]["mean_us"] == 750.0
