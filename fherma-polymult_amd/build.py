@@ -24,6 +24,12 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-mllvm", "-amdgpu-mfma-vgpr-form",
          "-Wall", "-Wno-unused-function", "-I" + os.path.join(HERE, "..", "include")]
+# The NTT and matrix-core kernels raise each wave's priority (s_setprio) until
+# its last global load is issued, so that a wave starting a block sends its loads
+# ahead of the other waves' arithmetic: the inverse column pass alone 17.1 -> 16.6
+# us, the 16-product batch 0.2-0.3 % faster (ABBA on one box).
+FILE_FLAGS = {"pm_ntt.hip": ["-mllvm", "-amdgpu-set-wave-priority"],
+              "pm_mfma.hip": ["-mllvm", "-amdgpu-set-wave-priority"]}
 
 
 def sources():
@@ -32,7 +38,7 @@ def sources():
 
 def _compile(src):
     obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [HIPCC] + FLAGS + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + FLAGS + FILE_FLAGS.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     if not src.endswith(".hip"):
         cmd[1:1] = ["-x", "hip"]
     r = subprocess.run(cmd, capture_output=True, text=True)
