@@ -497,9 +497,16 @@ __device__ __forceinline__ void transpose_bytes(uint32_t w0, uint32_t w1, uint32
     d[3] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
 }
 
-__device__ __forceinline__ long long fold_limb(const v4i& a) {
-    return static_cast<long long>(a[0]) + static_cast<long long>(a[1]) * 256 + static_cast<long long>(a[2]) * 65536 +
-           static_cast<long long>(a[3]) * 16777216;
+// a0 + 2^8 a1 + 2^16 a2 + 2^24 a3 as a signed 64-bit value. The factors come in
+// as opaque scalars (k8 = 2^8, k16 = 2^16, k24 = 2^24), so each term is one
+// signed 32 x 32 + 64-bit multiply-add (v_mad_i64_i32) instead of a sign
+// extension, a 64-bit shift and a 64-bit add.
+__device__ __forceinline__ long long fold_limb(const v4i& a, int k8, int k16, int k24) {
+    long long x = static_cast<long long>(a[0]);
+    x += static_cast<long long>(a[1]) * k8;
+    x += static_cast<long long>(a[2]) * k16;
+    x += static_cast<long long>(a[3]) * k24;
+    return x;
 }
 
 // A segment's low 64 bits, as far as a carry-in |C| < 2^30 can tell: lo itself
@@ -696,11 +703,13 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
                 for (int t = 0; t < kTiles; ++t) acc[i][t] = mfma16(af[i], bf[t], acc[i][t]);
             acc1 = mfma16(a4, bx, acc1);
         }
+        int k8 = 1 << 8, k16 = 1 << 16, k24 = 1 << 24;
+        asm volatile("" : "+s"(k8), "+s"(k16), "+s"(k24));
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int t = 0; t < kTiles; ++t) p0[i][t] = fold_limb(acc[i][t]);
-        p1 = fold_limb(acc1);
+            for (int t = 0; t < kTiles; ++t) p0[i][t] = fold_limb(acc[i][t], k8, k16, k24);
+        p1 = fold_limb(acc1, k8, k16, k24);
     }
     __syncthreads();   // y digits and the digit table are dead from here
     // lane (g, cl), tile st, column tile t: limb 4 st + g of coefficient 16 t + cl
@@ -724,14 +733,18 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
     const kconst<uint32_t> lcorr = lc + 144 + 2;
     auto qlimb = [&](int l) -> uint32_t { return lq[l]; };
     uint32_t x[kSegMax];
-    long long carry = 0;
+    // s's high word is the carry into the next limb (|carry| < 2^17); it enters
+    // the next sum sign-extended by a multiply-add with an opaque 1
+    long long s = 0;
+    int one = 1;
+    asm volatile("" : "+s"(one));
 #pragma unroll
     for (int u = 0; u < kSegMax; ++u) {
         const int l = l0 + u;
-        const long long s = carry + part[l * 64 + lane] + static_cast<long long>(lcorr[l]);
+        s = part[l * 64 + lane] + static_cast<long long>(lcorr[l]) + static_cast<long long>(static_cast<int>(s >> 32)) * one;
         x[u] = static_cast<uint32_t>(s);
-        carry = s >> 32;
     }
+    const long long carry = s >> 32;
     {
         bool uz = true, uo = true;
 #pragma unroll
@@ -756,24 +769,22 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
         Ctop = C;   // R's sign: |R| < 2^(32 L + 1)
     }
     {
-        long long C = Cin;
+        // the segment plus Cin sign-extended, modulo 2^(32 kSegMax): one
+        // add-with-carry per limb
+        const uint32_t ext = Cin < 0 ? 0xffffffffu : 0u;
+        unsigned int c = 0;
+        x[0] = __builtin_addc(x[0], static_cast<uint32_t>(Cin), 0u, &c);
 #pragma unroll
-        for (int u = 0; u < kSegMax; ++u) {
-            const long long t = static_cast<long long>(x[u]) + C;
-            x[u] = static_cast<uint32_t>(t);
-            C = t >> 32;
-        }
+        for (int u = 1; u < kSegMax; ++u) x[u] = __builtin_addc(x[u], ext, c, &c);
     }
-    // R >= q, segment by segment from the top
+    // the segment against q's: the borrow out of the segment minus q's limbs
+    // (one subtract-with-borrow per limb) says below, the OR of the differences
+    // equal; +1, 0 or -1
     {
-        int cmp = 0;
+        unsigned int b = 0, diff = 0;
 #pragma unroll
-        for (int u = kSegMax - 1; u >= 0; --u) {
-            const uint32_t ql = qlimb(l0 + u);
-            const int d = x[u] > ql ? 1 : (x[u] < ql ? -1 : 0);
-            cmp = cmp != 0 ? cmp : d;
-        }
-        cmpw[wave * 64 + lane] = cmp;
+        for (int u = 0; u < kSegMax; ++u) diff |= __builtin_subc(x[u], qlimb(l0 + u), b, &b);
+        cmpw[wave * 64 + lane] = diff == 0u ? 0 : (b != 0u ? -1 : 1);
     }
     __syncthreads();
     long long sign;
