@@ -540,6 +540,15 @@ __device__ __forceinline__ kconst<T> laundered(const T* p) {
     return k;
 }
 
+// An LDS word pointer whose value the compiler cannot see through (it stays one
+// register, and constant offsets from it fold into the DS instructions).
+using lds_u32 = const uint32_t __attribute__((address_space(3)))*;
+__device__ __forceinline__ lds_u32 lds_opaque(const uint32_t* p) {
+    uint32_t a = static_cast<uint32_t>((uintptr_t)(lds_u32)p);
+    asm volatile("" : "+v"(a));
+    return (lds_u32)(uintptr_t)a;
+}
+
 template <int KS>
 __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32_t* __restrict__ c, const DevPlan& P,
                                           uint32_t k0, uint32_t count, uint32_t tile0, uint32_t tstep) {
@@ -602,7 +611,8 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
     const uint32_t nb = k0 + tile * 64;
     const uint32_t nk = min(64u, k0 + count - nb);
     {
-        const bool lane_in = static_cast<uint32_t>(lane) < nk;
+        // lanes past nk (a last, partial tile) hold copies of coefficient
+        // nb + nk - 1 (load_y clamps the column): their results are not stored
         // loaded per tile (scalar loads), not hoisted out of the tile loop:
         // 2 x 4 KS doubles would not fit in scalar registers
         const kconst<double> fc = laundered(P.crt_fc);
@@ -615,7 +625,7 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
             for (int r = 0; r < 4; ++r) {
                 // y rows past K read 0; the constants (160 entries) are zero past K
                 const int j = 4 * J + r;
-                w[r] = lane_in ? y[4 * i + r] : 0u;
+                w[r] = y[4 * i + r];
                 f = fma(static_cast<double>(w[r]), fc[j], f);
                 f2 = fma(static_cast<double>(w[r]), fc[4 * 4 * kCrtMaxKSteps + j], f2);
             }
@@ -678,8 +688,11 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
         // unrolled: the accumulators keep fixed registers, the reads immediate offsets
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-            const uint32_t* tk = ta + 4 * ks * kTabRows;
-            const uint32_t* tk4 = ta4 + 4 * ks * kTabRows;
+            // the K-step's table rows through one opaque LDS address each, so
+            // that every pair read takes its word offsets as immediates (the
+            // compiler otherwise forms one address per pair)
+            lds_u32 tk = lds_opaque(ta + 4 * ks * kTabRows);
+            lds_u32 tk4 = lds_opaque(ta4 + 4 * ks * kTabRows);
             const uint4* yk = yb + 4 * ks * 64;
             v4i af[4], a4, bf[kTiles], bx;
 #pragma unroll
