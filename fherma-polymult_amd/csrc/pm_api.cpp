@@ -700,21 +700,20 @@ void capture_batch(pm_batch& bt, int count, pm_plan* const* plans, const uint32_
     bt.keep = std::move(keep);
 }
 
-// Median milliseconds of one replay of bt's graph: after one warm-up, five
-// samples of two back-to-back replays each.
-double time_batch(const pm_batch& bt, hipStream_t st, hipEvent_t e0, hipEvent_t e1) {
+// Median milliseconds of one replay of bt's graph: after one warm-up,
+// `samples` samples of `reps` back-to-back replays each.
+double time_batch(const pm_batch& bt, hipStream_t st, hipEvent_t e0, hipEvent_t e1, int reps = 2, int samples = 5) {
     check(hipGraphLaunch(bt.exec, st), "hipGraphLaunch");
     check(hipStreamSynchronize(st), "hipStreamSynchronize");
     std::vector<double> t;
-    for (int r = 0; r < 5; ++r) {
+    for (int r = 0; r < samples; ++r) {
         check(hipEventRecord(e0, st), "hipEventRecord");
-        check(hipGraphLaunch(bt.exec, st), "hipGraphLaunch");
-        check(hipGraphLaunch(bt.exec, st), "hipGraphLaunch");
+        for (int k = 0; k < reps; ++k) check(hipGraphLaunch(bt.exec, st), "hipGraphLaunch");
         check(hipEventRecord(e1, st), "hipEventRecord");
         check(hipEventSynchronize(e1), "hipEventSynchronize");
         float ms = 0;
         check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-        t.push_back(ms / 2.0);
+        t.push_back(ms / reps);
     }
     std::sort(t.begin(), t.end());
     return t[t.size() / 2];
@@ -781,20 +780,45 @@ void tune_batch(pm_batch& bt, int count, pm_plan* const* plans, const uint32_t* 
         check(hipEventCreate(&e1), "hipEventCreate");
         pm_batch best;
         double best_ms = 0;
+        std::vector<std::pair<double, pm_batch_shape>> timed;   // every (runs, streams) candidate
+        auto take = [&](pm_batch& trial, double ms) {
+            std::swap(best.graph, trial.graph);
+            std::swap(best.exec, trial.exec);
+            best.shape = trial.shape;
+            best.phased = trial.phased;
+            best.keep.swap(trial.keep);
+            best_ms = ms;
+        };
         auto consider = [&](const pm_batch_shape& sh) {
             pm_batch trial;
             capture_batch(trial, count, plans, a, b, c, sh);
             const double ms = time_batch(trial, st, e0, e1);
-            if (!best.exec || ms < 0.99 * best_ms) {
-                std::swap(best.graph, trial.graph);
-                std::swap(best.exec, trial.exec);
-                best.shape = trial.shape;
-                best.phased = trial.phased;
-                best.keep.swap(trial.keep);
-                best_ms = ms;
-            }
+            timed.emplace_back(ms, trial.shape);
+            if (!best.exec || ms < 0.99 * best_ms) take(trial, ms);
         };
         for (const auto& sh : cands) consider(sh);
+        // Short samples on a GPU that idles between candidates run at a higher
+        // clock than a sustained run of replays, and not by the same amount for
+        // every shape: the three fastest meet again in alternation, three
+        // rounds of ten back-to-back replays each, and the mean decides.
+        std::sort(timed.begin(), timed.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+        if (timed.size() > 1) {
+            const size_t n = std::min<size_t>(3, timed.size());
+            std::vector<std::unique_ptr<pm_batch>> fin(n);
+            std::vector<double> tot(n, 0.0);
+            for (size_t i = 0; i < n; ++i) {
+                fin[i].reset(new pm_batch);
+                capture_batch(*fin[i], count, plans, a, b, c, timed[i].second);
+            }
+            for (int r = 0; r < 3; ++r)
+                for (size_t i = 0; i < n; ++i) tot[i] += time_batch(*fin[i], st, e0, e1, 10, 1);
+            size_t w = 0;
+            for (size_t i = 1; i < n; ++i)
+                if (tot[i] < tot[w]) w = i;
+            take(*fin[w], 0.0);
+            // the forms below are timed as the shapes were, against the same kind of sample
+            best_ms = time_batch(best, st, e0, e1);
+        }
         if (best.phased && !std::getenv("PM_NTT_FORM")) {
             bool mfma_tables = false;
             for (int i = 0; i < count; ++i) mfma_tables = mfma_tables || (plans[i]->dev.colm_fwd && plans[i]->dev.colm_inv);
