@@ -28,7 +28,11 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
 # its last global load is issued, so that a wave starting a block sends its loads
 # ahead of the other waves' arithmetic: the inverse column pass alone 17.1 -> 16.6
 # us, the 16-product batch 0.2-0.3 % faster (ABBA on one box).
-FILE_FLAGS = {"pm_ntt.hip": ["-mllvm", "-amdgpu-set-wave-priority"],
+# The NTT kernels are scheduled for instruction-level parallelism rather than
+# occupancy: the butterfly code issues faster from fewer waves with more
+# independent work interleaved (the fused row pass at 124 VGPRs and four waves
+# per SIMD against 69 and seven: 56.9 -> 53.0 us alone, the batch +0.8-1.9 %).
+FILE_FLAGS = {"pm_ntt.hip": ["-mllvm", "-amdgpu-set-wave-priority", "-mllvm", "-amdgpu-sched-strategy=max-ilp"],
               "pm_mfma.hip": ["-mllvm", "-amdgpu-set-wave-priority"]}
 
 

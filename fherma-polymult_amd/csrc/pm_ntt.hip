@@ -452,9 +452,12 @@ __device__ __forceinline__ void rows_fused_mid(uint32_t* __restrict__ A, const u
     row_item<8, kGen, kMode>(A, P, j, rho0 + rl, base + static_cast<size_t>(rl) * M, t, la, twa, a, b);
 }
 
-// A batch: blockIdx.z = product.
+// A batch: blockIdx.z = product. Four waves per SIMD: the build schedules this
+// file for instruction-level parallelism, and at up to 128 VGPRs the butterflies
+// of A and B interleave (faster than seven waves of a tighter schedule).
 template <bool kGen, int kMode>
-__global__ void __launch_bounds__(256) k_ntt_rows_fused_mid_batch(DevPlan P, BatchArgs ba) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_ntt_rows_fused_mid_batch(DevPlan P, BatchArgs ba) {
     if (static_cast<int>(blockIdx.y) >= ba.K[blockIdx.z]) return;   // primes past this product's K
     rows_fused_mid<kGen, kMode>(ba.A[blockIdx.z], ba.B[blockIdx.z], P);
 }
