@@ -155,8 +155,8 @@ VALU_MIX = {
     "k_ntt_rows_fused_reg<8,true,1>": {"other": 1584, "mul_hi": 448, "mul_lo": 321, "mad64": 1},
     # with the four column stages of distance 256 .. 2048 on A and B before the rows
     "k_ntt_rows_fused_mid_batch<true,1>": {"other": 2055, "mul_hi": 512, "mul_lo": 448},
-    "k_ntt_col_reg<8,true,5,1>": {"other": 464, "mul_hi": 64, "mul_lo": 129},
-    "k_ntt_col_reg<8,false,5,1>": {"other": 464, "mul_hi": 128, "mul_lo": 65},
+    "k_ntt_col_reg<8,true,5,1>": {"other": 465, "mul_hi": 64, "mul_lo": 128},
+    "k_ntt_col_reg<8,false,5,1>": {"other": 465, "mul_hi": 128, "mul_lo": 64},
 }
 
 
