@@ -181,9 +181,10 @@ def valu_issue(kernel, kernel_ms, cus):
                         "cycles_per_instruction": round(cycles / sum(mix.values()), 3), "waves": waves,
                         "clock_ghz": CLOCK_GHZ, "simds": simds, "issue_ms": round(ms, 5),
                         "frac": round(ms / kernel_ms, 4),
-                        "note": "issue time of the kernel's VALU instructions at the probe's rates over its "
-                                "measured time; the rest is dependency, barrier and LDS latency that its waves "
-                                "per SIMD do not hide (fewer, persistent waves that prefetch were slower)"}
+                        "note": "issue time of the kernel's VALU instructions at the probe's rates (independent "
+                                "chains) over its measured time; copies of the row pass without loads, stores or "
+                                "LDS exchanges kept most of the rest, so it is the issue of the butterflies' "
+                                "dependent chains, not memory"}
     return None
 
 
