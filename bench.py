@@ -610,7 +610,7 @@ def issue_probe(log):
     simds = 4 * cus
     rows = []
     for kind, iters in ((0, 4096), (1, 4096), (2, 4096), (3, 2048), (4, 2048), (5, 4096), (6, 4096),
-                        (7, 2048), (8, 2048), (9, 1024)):
+                        (7, 2048), (8, 2048), (9, 1024), (10, 2048), (11, 1024)):
         r = polymult.probe_issue(kind, iters)
         cycles = r["ms"] * 1e-3 * r["clock_mhz"] * 1e6
         n = r["valu_per_simd"] + r["mfma_per_simd"]

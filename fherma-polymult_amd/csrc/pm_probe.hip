@@ -11,6 +11,8 @@
 //        2  v_mul_hi_u32         7  v_mfma_i32_16x16x64_i8
 //        3  v_mad_u64_u32        8  MFMA waves and v_mul_lo_u32 waves on the same SIMDs
 //        4  v_mad_i64_i32        9  one wave stream alternating MFMA and v_mul_lo_u32
+//                               10  MFMA waves and v_add_u32 waves on the same SIMDs
+//                               11  one wave stream alternating MFMA and v_add_u32
 #include "pm_state.h"
 
 #include <hip/hip_runtime.h>
@@ -117,20 +119,20 @@ __global__ void __launch_bounds__(512) k_probe(const uint32_t* __restrict__ in, 
     const v4i b = {static_cast<int>(x[4]), static_cast<int>(x[5]), static_cast<int>(x[6]), static_cast<int>(x[7])};
 #pragma unroll
     for (int k = 0; k < kMfmaChains; ++k) m[k] = v4i{k, 0, 0, 0};
-    const bool mfma_wave = KIND == 7 || (KIND == 8 && (threadIdx.x >> 6) < 4);
+    const bool mfma_wave = KIND == 7 || ((KIND == 8 || KIND == 10) && (threadIdx.x >> 6) < 4);
     for (int it = 0; it < iters; ++it) {
         if constexpr (KIND <= 6) {
             valu_iter<KIND>(x, acc, y);
         } else if constexpr (KIND == 7) {
             mfma_iter(m, a, b);
-        } else if constexpr (KIND == 8) {
+        } else if constexpr (KIND == 8 || KIND == 10) {
             if (mfma_wave)
                 mfma_iter(m, a, b);
             else
-                valu_iter<1>(x, acc, y);
+                valu_iter<KIND == 8 ? 1 : 0>(x, acc, y);
         } else {
             mfma_iter(m, a, b);
-            valu_iter<1>(x, acc, y);
+            valu_iter<KIND == 9 ? 1 : 0>(x, acc, y);
         }
     }
     uint32_t s = 0;
@@ -149,7 +151,7 @@ void probe_launch(const uint32_t* in, uint32_t* out, int cus, int iters, hipStre
 }  // namespace
 
 // Runs probe `kind` and returns its wall time (ms) and the instructions each
-// SIMD issued: VALU instructions, MFMAs (kinds 7-9 count both).
+// SIMD issued: VALU instructions, MFMAs (kinds 7-11 count both).
 void probe_issue(int kind, int iters, double* ms, double* valu_per_simd, double* mfma_per_simd, double* clock_mhz) {
     int dev = 0, cus = 0, khz = 0;
     check(hipGetDevice(&dev), "hipGetDevice");
@@ -183,6 +185,8 @@ void probe_issue(int kind, int iters, double* ms, double* valu_per_simd, double*
                 case 7: probe_launch<7>(in, out, cus, n, nullptr); break;
                 case 8: probe_launch<8>(in, out, cus, n, nullptr); break;
                 case 9: probe_launch<9>(in, out, cus, n, nullptr); break;
+                case 10: probe_launch<10>(in, out, cus, n, nullptr); break;
+                case 11: probe_launch<11>(in, out, cus, n, nullptr); break;
                 default: throw std::invalid_argument("no such probe");
             }
         };
@@ -193,15 +197,17 @@ void probe_issue(int kind, int iters, double* ms, double* valu_per_simd, double*
         check(hipEventSynchronize(e1), "hipEventSynchronize");
         float t = 0;
         check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
-        // 16 waves per CU = 4 per SIMD; kind 8 puts 2 MFMA and 2 VALU waves on each SIMD
+        // 16 waves per CU = 4 per SIMD; kinds 8 and 10 put 2 MFMA and 2 VALU waves on each SIMD
         const double waves = 4.0, valu_inst = kChains * kUnroll, mfma_inst = kMfmaChains * kMfmaUnroll;
         const bool pair = kind == 5 || kind == 6;   // one swap instruction serves two chains
         *ms = t;
         *valu_per_simd = kind <= 6   ? waves * iters * valu_inst / (pair ? 2 : 1)
-                         : kind == 8 ? 2.0 * iters * valu_inst
-                         : kind == 9 ? waves * iters * valu_inst
-                                     : 0.0;
-        *mfma_per_simd = kind == 7 || kind == 9 ? waves * iters * mfma_inst : kind == 8 ? 2.0 * iters * mfma_inst : 0.0;
+                         : kind == 8 || kind == 10 ? 2.0 * iters * valu_inst
+                         : kind == 9 || kind == 11 ? waves * iters * valu_inst
+                                                   : 0.0;
+        *mfma_per_simd = kind == 7 || kind == 9 || kind == 11 ? waves * iters * mfma_inst
+                         : kind == 8 || kind == 10             ? 2.0 * iters * mfma_inst
+                                                               : 0.0;
         *clock_mhz = khz / 1000.0;
     } catch (...) {
         cleanup();

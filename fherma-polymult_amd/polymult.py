@@ -246,7 +246,8 @@ def device_info():
 
 PROBE_KINDS = ["v_add_u32", "v_mul_lo_u32", "v_mul_hi_u32", "v_mad_u64_u32", "v_mad_i64_i32",
                "v_permlane32_swap_b32", "v_permlane16_swap_b32", "v_mfma_i32_16x16x64_i8",
-               "mfma_waves_beside_mul_lo_waves", "mfma_and_mul_lo_in_one_wave"]
+               "mfma_waves_beside_mul_lo_waves", "mfma_and_mul_lo_in_one_wave", "mfma_waves_beside_add_waves",
+               "mfma_and_add_in_one_wave"]
 
 
 def probe_issue(kind: int, iters: int) -> dict:

@@ -283,7 +283,8 @@ int pm_device_synchronize(void);
  * instruction follow. Kinds: 0 v_add_u32, 1 v_mul_lo_u32, 2 v_mul_hi_u32,
  * 3 v_mad_u64_u32, 4 v_mad_i64_i32, 5 v_permlane32_swap, 6 v_permlane16_swap,
  * 7 v_mfma_i32_16x16x64_i8, 8 MFMA waves beside v_mul_lo_u32 waves on each
- * SIMD, 9 one wave stream alternating MFMA and v_mul_lo_u32. */
+ * SIMD, 9 one wave stream alternating MFMA and v_mul_lo_u32, 10 MFMA waves
+ * beside v_add_u32 waves, 11 one wave stream alternating MFMA and v_add_u32. */
 int pm_probe_issue(int kind, int iters, double* ms, double* valu_per_simd, double* mfma_per_simd, double* clock_mhz);
 
 /* Device name and compute-unit count of the current device. */
