@@ -611,10 +611,10 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
     const uint32_t nb = k0 + tile * 64;
     const uint32_t nk = min(64u, k0 + count - nb);
     {
-        // lanes past nk (a last, partial tile) hold copies of coefficient
-        // nb + nk - 1 (load_y clamps the column): their results are not stored
-        // loaded per tile (scalar loads), not hoisted out of the tile loop:
-        // 2 x 4 KS doubles would not fit in scalar registers
+        // Lanes past nk (a last, partial tile) hold copies of coefficient
+        // nb + nk - 1 (load_y clamps the column); their results are not stored.
+        // The constants are loaded per tile (scalar loads), not hoisted out of
+        // the tile loop: 2 x 4 KS doubles would not fit in scalar registers.
         const kconst<double> fc = laundered(P.crt_fc);
         double f = 0, f2 = 0;
 #pragma unroll
@@ -746,8 +746,8 @@ __device__ __forceinline__ void crt_tiles(const uint32_t* __restrict__ Y, uint32
     const kconst<uint32_t> lcorr = lc + 144 + 2;
     auto qlimb = [&](int l) -> uint32_t { return lq[l]; };
     uint32_t x[kSegMax];
-    // s's high word is the carry into the next limb (|carry| < 2^17); it enters
-    // the next sum sign-extended by a multiply-add with an opaque 1
+    // s's high word is the signed carry into the next limb; it enters the next
+    // sum sign-extended, by a multiply-add with an opaque 1
     long long s = 0;
     int one = 1;
     asm volatile("" : "+s"(one));
